@@ -1,0 +1,731 @@
+// capi.hip -- the C ABI of include/cleo_hip.h: context, data movement and the SDM step driver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctx.hpp"
+#include "mlp.hpp"
+#include "sdm_kernels.hpp"
+
+using namespace cleo;
+
+#define HIP_TRY(c, expr)                                                             \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      if (c) (c)->msg = std::string(#expr) + ": " + hipGetErrorString(e_);           \
+      return CLEO_E_HIP;                                                             \
+    }                                                                                \
+  } while (0)
+
+namespace {
+
+int fail(cleo_ctx* c, int code, const std::string& m) {
+  if (c) c->msg = m;
+  return code;
+}
+
+double step2dimlesstime(uint32_t s) { return static_cast<double>(s) / 100000.0; }
+double step2realtime(uint32_t s) { return (static_cast<double>(s) / 100000.0) * dimless::TIME0; }
+
+template <typename T>
+hipError_t dalloc(T** p, uint64_t n) {
+  return hipMalloc(reinterpret_cast<void**>(p), std::max<uint64_t>(n, 1) * sizeof(T));
+}
+
+hipError_t alloc_soa(SoA& s, uint64_t n) {
+  hipError_t e;
+  if ((e = dalloc(&s.gbx, n)) != hipSuccess) return e;
+  if ((e = dalloc(&s.c3, n)) != hipSuccess) return e;
+  if ((e = dalloc(&s.c1, n)) != hipSuccess) return e;
+  if ((e = dalloc(&s.c2, n)) != hipSuccess) return e;
+  if ((e = dalloc(&s.xi, n)) != hipSuccess) return e;
+  if ((e = dalloc(&s.r, n)) != hipSuccess) return e;
+  if ((e = dalloc(&s.msol, n)) != hipSuccess) return e;
+  return dalloc(&s.id, n);
+}
+void free_soa(SoA& s) {
+  hipFree(s.gbx);
+  hipFree(s.c3);
+  hipFree(s.c1);
+  hipFree(s.c2);
+  hipFree(s.xi);
+  hipFree(s.r);
+  hipFree(s.msol);
+  hipFree(s.id);
+  s = SoA{};
+}
+
+// synchronise, then report a recorded device error
+int sync_check(cleo_ctx* c) {
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(c->h_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
+  if (c->h_ctl->err_code != 0) {
+    c->msg = "device error " + std::to_string(c->h_ctl->err_code) + " at gridbox " +
+             std::to_string(c->h_ctl->err_gbx) + " drop " + std::to_string(c->h_ctl->err_drop);
+    return CLEO_E_DEVICE;
+  }
+  return CLEO_OK;
+}
+
+int launch_check(cleo_ctx* c) {
+  HIP_TRY(c, hipGetLastError());
+  return CLEO_OK;
+}
+
+int ensure_radix(cleo_ctx* c, uint64_t n) {
+  if (c->rs_n >= n && c->rs_key[0]) return CLEO_OK;
+  for (int i = 0; i < 2; ++i) {
+    hipFree(c->rs_key[i]);
+    hipFree(c->rs_idx[i]);
+  }
+  hipFree(c->rs_cnt);
+  const uint64_t ntiles = (n + 255) / 256;
+  for (int i = 0; i < 2; ++i) {
+    HIP_TRY(c, dalloc(&c->rs_key[i], n));
+    HIP_TRY(c, dalloc(&c->rs_idx[i], n));
+  }
+  HIP_TRY(c, dalloc(&c->rs_cnt, 32 * ntiles + 1));
+  const size_t need = scan_tmp_words(16 * ntiles);
+  if (need > c->scan_tmp_n) {
+    hipFree(c->scan_tmp);
+    HIP_TRY(c, dalloc(&c->scan_tmp, need));
+    c->scan_tmp_n = need;
+  }
+  c->rs_n = n;
+  return CLEO_OK;
+}
+
+uint32_t wpb_or1(uint32_t v) { return v == 0 ? 1u : std::min<uint32_t>(v, 4u); }
+
+CondArgs cond_args(cleo_ctx* c, const cleo_cond_params& p, bool fuse_qcond) {
+  CondArgs a{};
+  a.s = c->buf[c->cur];
+  a.off = c->off;
+  a.nact = c->nact;
+  a.st = c->st;
+  a.sol = c->solute;
+  a.cfg = NewtonCfg{step2dimlesstime(p.interval), p.minsubdelt_s / dimless::TIME0, p.rtol, p.atol, p.maxniters};
+  a.ngbx = c->grid.ngbx;
+  a.wpb = wpb_or1(c->tiles.cond_waves_per_block);
+  a.alter = p.do_alter_thermo;
+  a.fuse_qcond = fuse_qcond ? 1 : 0;
+  a.ctl = c->ctl;
+  a.niters = c->rec_niters ? c->niters : nullptr;
+  a.warm = nullptr;
+  a.warm_limit = p.warm_restart_iters;
+  return a;
+}
+
+// enqueue one condensation pass (learned guesses first when requested)
+int enqueue_condensation(cleo_ctx* c, const cleo_cond_params& p, bool fuse_qcond) {
+  if (p.interval == 0) return fail(c, CLEO_E_ARG, "condensation interval 0");
+  if (!(p.minsubdelt_s > 0.0)) return fail(c, CLEO_E_ARG, "MINSUBTSTEP must be positive");
+  CondArgs a = cond_args(c, p, fuse_qcond);
+  if (p.warm_start) {
+    const int rc = mlp_enqueue_guess(c, p, &a.warm);
+    if (rc != CLEO_OK) return rc;
+    if (c->rec_guess && c->guess)
+      HIP_TRY(c, hipMemcpyAsync(c->guess, a.warm, c->ntot * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  }
+  launch_condensation(c, a);
+  return launch_check(c);
+}
+
+// offsets of the injected stream per gridbox; validates the sizes the caller gave
+int prepare_injected(cleo_ctx* c) {
+  launch_inj_sizes(c, c->tot, c->oobpre);
+  scan_u32(c, c->tot, c->grid.ngbx, 1u, 0u, c->inj_oj, nullptr);
+  scan_u32(c, c->oobpre, c->grid.ngbx, 1u, 0u, c->inj_op, nullptr);
+  uint32_t tj = 0, tp = 0;
+  HIP_TRY(c, hipMemcpyAsync(&tj, c->inj_oj + c->grid.ngbx, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(&tp, c->inj_op + c->grid.ngbx, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (tj != c->inj_nj || tp != c->inj_nphi)
+    return fail(c, CLEO_E_ARG, "injected stream sizes (" + std::to_string(c->inj_nj) + ", " +
+                                   std::to_string(c->inj_nphi) + ") != required (" + std::to_string(tj) + ", " +
+                                   std::to_string(tp) + ")");
+  return CLEO_OK;
+}
+
+// one collision event over every gridbox: buf[cur] -> buf[1-cur], cur flips
+int enqueue_collision_event(cleo_ctx* c, const cleo_coll_params& p) {
+  if (p.interval == 0) return fail(c, CLEO_E_ARG, "collision interval 0");
+  if (c->rng_mode == 0) return fail(c, CLEO_E_STATE, "no random-number stream set");
+  const bool inj = c->rng_mode == 2;
+  if (inj) {
+    const int rc = prepare_injected(c);
+    if (rc != CLEO_OK) return rc;
+  }
+  CollArgs a{};
+  a.src = c->buf[c->cur];
+  a.dst = c->buf[1 - c->cur];
+  a.off = c->off;
+  a.nact = c->nact;
+  a.vol = c->st.vol;
+  a.sol = c->solute;
+  a.k = CollCfg{p.prob_kind, p.prob_param, p.enact_kind, p.nfrags, p.coalbure_coaleff, step2realtime(p.interval)};
+  a.ngbx = c->grid.ngbx;
+  a.wpb = wpb_or1(c->tiles.coll_waves_per_block);
+  a.ctl = c->ctl;
+  a.seed = c->seed;
+  a.inj_j = inj ? c->inj_j : nullptr;
+  a.inj_phi = inj ? c->inj_phi : nullptr;
+  a.inj_oj = c->inj_oj;
+  a.inj_op = c->inj_op;
+  a.big = c->big_scratch;
+  launch_collisions(c, a);
+  launch_copy_oob_region(c, c->cur, 1 - c->cur);
+  launch_advance_event(c);
+  c->cur = 1 - c->cur;
+  if (inj) c->rng_mode = 0;  // consumed
+  return launch_check(c);
+}
+
+// MoveSupersInDomain::run_step body: motion, stable sort + refs, (qcond unless fused later)
+int enqueue_motion(cleo_ctx* c, const cleo_motion_params& p, bool qcond) {
+  if (p.interval == 0) return fail(c, CLEO_E_ARG, "motion interval 0");
+  MotionArgs a{};
+  a.s = c->buf[c->cur];
+  a.off = c->off;
+  a.grid = c->grid;
+  a.st = c->st;
+  a.delt = step2dimlesstime(p.interval);
+  a.tv = p.terminalv;
+  a.wpb = wpb_or1(c->tiles.motion_waves_per_block);
+  a.cnt = c->cnt;
+  a.ctl = c->ctl;
+  launch_motion(c, a);
+  neighbour_sort(c, c->cur, 1 - c->cur, nullptr, false);
+  if (qcond) {
+    QcondArgs q{c->buf[c->cur], c->off, c->st, c->solute, c->grid.ngbx, wpb_or1(c->tiles.cond_waves_per_block), c->ctl};
+    launch_qcond(c, q);
+  }
+  return launch_check(c);
+}
+
+// SDMMethods::run_step(t_mdl, t_mdl_next) (sdmmethods.hpp:319-332): host loop over SDM sub-steps;
+// every gridbox follows the same schedule, so each process runs as one launch over all gridboxes.
+int enqueue_run_step(cleo_ctx* c, uint32_t t_mdl, uint32_t t_mdl_next, int* swaps) {
+  const cleo_sdm_params& p = c->sdm;
+  const bool mot = p.enable_motion != 0, cond = p.enable_cond != 0, coll = p.enable_colls != 0;
+  const uint32_t UMAX = 0xFFFFFFFFu;
+  auto next_of = [](uint32_t t, uint32_t iv) { return ((t / iv) + 1) * iv; };
+  uint32_t t_sdm = t_mdl;
+  while (t_sdm < t_mdl_next) {
+    const uint32_t next_move = mot ? next_of(t_sdm, p.motion.interval) : UMAX;
+    const uint32_t t_sdm_next = std::min(t_mdl_next, next_move);
+    bool pending_qcond = false;
+    if (mot && t_sdm % p.motion.interval == 0) {
+      const int rc = enqueue_motion(c, p.motion, false);
+      if (rc != CLEO_OK) return rc;
+      pending_qcond = true;
+      if (swaps) *swaps += 1;
+    }
+    bool any_coll = false;
+    for (uint32_t subt = t_sdm; subt < t_sdm_next;) {
+      const bool on_cond = cond && subt % p.cond.interval == 0;
+      const bool on_coll = coll && subt % p.coll.interval == 0;
+      if (on_cond) {
+        const int rc = enqueue_condensation(c, p.cond, pending_qcond);
+        if (rc != CLEO_OK) return rc;
+        pending_qcond = false;
+      }
+      if (pending_qcond) {  // qcond after the sort comes before any drop changes
+        QcondArgs q{c->buf[c->cur], c->off, c->st, c->solute, c->grid.ngbx, wpb_or1(c->tiles.cond_waves_per_block),
+                    c->ctl};
+        launch_qcond(c, q);
+        pending_qcond = false;
+      }
+      if (on_coll) {
+        const int rc = enqueue_collision_event(c, p.coll);
+        if (rc != CLEO_OK) return rc;
+        any_coll = true;
+        if (swaps) *swaps += 1;
+      }
+      const uint32_t a = cond ? next_of(subt, p.cond.interval) : UMAX;
+      const uint32_t b = coll ? next_of(subt, p.coll.interval) : UMAX;
+      subt = std::min(a, b);
+    }
+    if (pending_qcond) {
+      QcondArgs q{c->buf[c->cur], c->off, c->st, c->solute, c->grid.ngbx, wpb_or1(c->tiles.cond_waves_per_block), c->ctl};
+      launch_qcond(c, q);
+    }
+    if (any_coll) neighbour_sort(c, c->cur, 1 - c->cur, &c->ctl->resort_flag, true);
+    t_sdm = t_sdm_next;
+  }
+  return launch_check(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
+  if (!out || !dom || !dom->zhalf || !dom->xhalf || !dom->yhalf) return CLEO_E_ARG;
+  const uint64_t nz = dom->ndims[0], nx = dom->ndims[1], ny = dom->ndims[2];
+  if (nz == 0 || nx == 0 || ny == 0) return CLEO_E_ARG;
+  const uint64_t ngbx = nz * nx * ny;
+  if (ngbx >= 0xFFFFFFF0ull || maxn >= 0xFFFFFFF0ull) return CLEO_E_ARG;
+  auto increasing = [](const double* h, uint64_t n) {
+    for (uint64_t i = 0; i < n; ++i)
+      if (!(h[i] < h[i + 1])) return false;
+    return true;
+  };
+  if (!increasing(dom->zhalf, nz) || !increasing(dom->xhalf, nx) || !increasing(dom->yhalf, ny)) return CLEO_E_ARG;
+  cleo_ctx* c = new cleo_ctx();
+  HIP_TRY(c, hipGetDevice(&c->device));
+  HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->grid.nz = static_cast<uint32_t>(nz);
+  c->grid.nx = static_cast<uint32_t>(nx);
+  c->grid.ny = static_cast<uint32_t>(ny);
+  c->grid.ngbx = static_cast<uint32_t>(ngbx);
+  for (int i = 0; i < 3; ++i) c->grid.per[i] = dom->periodic[i] ? 1 : 0;
+  const uint64_t nh = nz + nx + ny + 3;
+  HIP_TRY(c, dalloc(&c->d_halves, nh));
+  HIP_TRY(c, hipMemcpy(c->d_halves, dom->zhalf, (nz + 1) * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_halves + nz + 1, dom->xhalf, (nx + 1) * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_halves + nz + nx + 2, dom->yhalf, (ny + 1) * 8, hipMemcpyHostToDevice));
+  c->grid.zh = c->d_halves;
+  c->grid.xh = c->d_halves + nz + 1;
+  c->grid.yh = c->d_halves + nz + nx + 2;
+  c->maxn = maxn;
+  HIP_TRY(c, alloc_soa(c->buf[0], maxn));
+  HIP_TRY(c, alloc_soa(c->buf[1], maxn));
+  HIP_TRY(c, dalloc(&c->off, ngbx + 1));
+  HIP_TRY(c, dalloc(&c->off_alt, ngbx + 1));
+  HIP_TRY(c, dalloc(&c->nact, ngbx));
+  HIP_TRY(c, dalloc(&c->d_state, 11 * ngbx));
+  double* s = c->d_state;
+  c->st = GState{s, s + ngbx, s + 2 * ngbx, s + 3 * ngbx, s + 4 * ngbx, s + 5 * ngbx,
+                 s + 6 * ngbx, s + 7 * ngbx, s + 8 * ngbx, s + 9 * ngbx, s + 10 * ngbx};
+  HIP_TRY(c, dalloc(&c->cnt, 28 * ngbx));
+  HIP_TRY(c, dalloc(&c->tot, ngbx + 1));
+  HIP_TRY(c, dalloc(&c->oobpre, ngbx + 1));
+  HIP_TRY(c, dalloc(&c->inj_oj, ngbx + 1));
+  HIP_TRY(c, dalloc(&c->inj_op, ngbx + 1));
+  c->scan_tmp_n = scan_tmp_words(28 * ngbx + 1);
+  HIP_TRY(c, dalloc(&c->scan_tmp, c->scan_tmp_n));
+  HIP_TRY(c, dalloc(&c->big_scratch, 2 * maxn));
+  c->big_scratch_n = 2 * maxn;
+  HIP_TRY(c, dalloc(&c->ctl, 1));
+  HIP_TRY(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_ctl), sizeof(DevCtl), hipHostMallocDefault));
+  HIP_TRY(c, hipMemset(c->ctl, 0, sizeof(DevCtl)));
+  HIP_TRY(c, hipMemset(c->off, 0, (ngbx + 1) * 4));
+  HIP_TRY(c, hipMemset(c->off_alt, 0, (ngbx + 1) * 4));
+  HIP_TRY(c, hipMemset(c->nact, 0, ngbx * 4));
+  HIP_TRY(c, hipMemset(c->d_state, 0, 11 * ngbx * 8));
+  c->solute = Solute{2016.5 / dimless::RHO0, 0.05844277 / dimless::MR0, 2.0};
+  HIP_TRY(c, mlp_create(c));
+  *out = c;
+  return CLEO_OK;
+}
+
+int cleo_destroy(cleo_ctx* c) {
+  if (!c) return CLEO_OK;
+  hipStreamSynchronize(c->stream);
+  mlp_destroy(c);
+  free_soa(c->buf[0]);
+  free_soa(c->buf[1]);
+  for (void* p : {static_cast<void*>(c->d_halves), static_cast<void*>(c->off), static_cast<void*>(c->off_alt),
+                  static_cast<void*>(c->nact), static_cast<void*>(c->d_state), static_cast<void*>(c->cnt),
+                  static_cast<void*>(c->tot), static_cast<void*>(c->oobpre), static_cast<void*>(c->inj_oj),
+                  static_cast<void*>(c->inj_op), static_cast<void*>(c->scan_tmp), static_cast<void*>(c->big_scratch),
+                  static_cast<void*>(c->ctl), static_cast<void*>(c->inj_j), static_cast<void*>(c->inj_phi),
+                  static_cast<void*>(c->niters), static_cast<void*>(c->guess), static_cast<void*>(c->rs_key[0]),
+                  static_cast<void*>(c->rs_key[1]), static_cast<void*>(c->rs_idx[0]), static_cast<void*>(c->rs_idx[1]),
+                  static_cast<void*>(c->rs_cnt)})
+    if (p) hipFree(p);
+  if (c->h_ctl) hipHostFree(c->h_ctl);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return CLEO_OK;
+}
+
+const char* cleo_last_error_message(const cleo_ctx* c) { return c ? c->msg.c_str() : "null context"; }
+
+int cleo_device_info(char* buf, size_t len) {
+  hipDeviceProp_t p;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return CLEO_E_HIP;
+  std::snprintf(buf, len, "%s %s CUs=%d", p.name, p.gcnArchName, p.multiProcessorCount);
+  return CLEO_OK;
+}
+
+int cleo_upload_supers(cleo_ctx* c, const cleo_supers_soa* h, uint64_t ntot) {
+  if (!c || !h) return CLEO_E_ARG;
+  if (ntot > c->maxn) return fail(c, CLEO_E_ARG, "ntot exceeds the context capacity");
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  SoA& d = c->buf[c->cur];
+  HIP_TRY(c, hipMemcpy(d.gbx, h->sdgbxindex, ntot * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(d.c3, h->coord3, ntot * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(d.c1, h->coord1, ntot * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(d.c2, h->coord2, ntot * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(d.xi, h->xi, ntot * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(d.r, h->radius, ntot * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(d.msol, h->msol, ntot * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(d.id, h->sdid, ntot * 8, hipMemcpyHostToDevice));
+  c->ntot = ntot;
+  const int rc = ensure_radix(c, ntot);
+  if (rc != CLEO_OK) return rc;
+  generic_sort(c);
+  HIP_TRY(c, hipGetLastError());
+  const int rs = sync_check(c);
+  // the upload's own sort is the reference's constructor sort, not a step sort
+  HIP_TRY(c, hipMemset(&c->ctl->stats[ST_SORTS], 0, 8));
+  return rs;
+}
+
+// reference Superdrop layout (superdrop.hpp:36-60): u32 gbx @0, f64 coord3 @8, coord1 @16,
+// coord2 @24, attrs {solute @32, u64 xi @40, f64 radius @48, f64 msol @56}, u64 id @64
+int cleo_upload_supers_aos(cleo_ctx* c, const void* aos, uint64_t ntot) {
+  if (!c || !aos) return CLEO_E_ARG;
+  std::vector<uint32_t> g(ntot);
+  std::vector<double> c3(ntot), c1(ntot), c2(ntot), r(ntot), m(ntot);
+  std::vector<uint64_t> xi(ntot), id(ntot);
+  const unsigned char* p = static_cast<const unsigned char*>(aos);
+  for (uint64_t k = 0; k < ntot; ++k, p += 72) {
+    std::memcpy(&g[k], p, 4);
+    std::memcpy(&c3[k], p + 8, 8);
+    std::memcpy(&c1[k], p + 16, 8);
+    std::memcpy(&c2[k], p + 24, 8);
+    std::memcpy(&xi[k], p + 40, 8);
+    std::memcpy(&r[k], p + 48, 8);
+    std::memcpy(&m[k], p + 56, 8);
+    std::memcpy(&id[k], p + 64, 8);
+  }
+  cleo_supers_soa s{g.data(), c3.data(), c1.data(), c2.data(), xi.data(), r.data(), m.data(), id.data()};
+  return cleo_upload_supers(c, &s, ntot);
+}
+
+int cleo_download_supers(cleo_ctx* c, cleo_supers_soa* h, uint64_t* ntot, uint64_t* ndomain) {
+  if (!c || !h) return CLEO_E_ARG;
+  const int rc = sync_check(c);
+  const uint64_t n = c->ntot;
+  const SoA& d = c->buf[c->cur];
+  HIP_TRY(c, hipMemcpy(h->sdgbxindex, d.gbx, n * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(h->coord3, d.c3, n * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(h->coord1, d.c1, n * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(h->coord2, d.c2, n * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(h->xi, d.xi, n * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(h->radius, d.r, n * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(h->msol, d.msol, n * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(h->sdid, d.id, n * 8, hipMemcpyDeviceToHost));
+  uint32_t nd = 0;
+  HIP_TRY(c, hipMemcpy(&nd, c->off + c->grid.ngbx, 4, hipMemcpyDeviceToHost));
+  if (ntot) *ntot = n;
+  if (ndomain) *ndomain = nd;
+  return rc;
+}
+
+int cleo_download_supers_aos(cleo_ctx* c, void* aos, uint64_t* ntot, uint64_t* ndomain) {
+  if (!c || !aos) return CLEO_E_ARG;
+  const uint64_t n = c->ntot;
+  std::vector<uint32_t> g(n);
+  std::vector<double> c3(n), c1(n), c2(n), r(n), m(n);
+  std::vector<uint64_t> xi(n), id(n);
+  cleo_supers_soa s{g.data(), c3.data(), c1.data(), c2.data(), xi.data(), r.data(), m.data(), id.data()};
+  const int rc = cleo_download_supers(c, &s, ntot, ndomain);
+  unsigned char* p = static_cast<unsigned char*>(aos);
+  for (uint64_t k = 0; k < n; ++k, p += 72) {
+    std::memset(p, 0, 72);
+    std::memcpy(p, &g[k], 4);
+    std::memcpy(p + 8, &c3[k], 8);
+    std::memcpy(p + 16, &c1[k], 8);
+    std::memcpy(p + 24, &c2[k], 8);
+    std::memcpy(p + 40, &xi[k], 8);
+    std::memcpy(p + 48, &r[k], 8);
+    std::memcpy(p + 56, &m[k], 8);
+    std::memcpy(p + 64, &id[k], 8);
+  }
+  return rc;
+}
+
+int cleo_download_refs(cleo_ctx* c, uint32_t* refs) {
+  if (!c || !refs) return CLEO_E_ARG;
+  const int rc = sync_check(c);
+  HIP_TRY(c, hipMemcpy(refs, c->off, (c->grid.ngbx + 1ull) * 4, hipMemcpyDeviceToHost));
+  return rc;
+}
+
+static double* const* state_fields(cleo_ctx* c, double* (&out)[11]) {
+  out[0] = c->st.vol; out[1] = c->st.press; out[2] = c->st.temp; out[3] = c->st.qv; out[4] = c->st.qc;
+  out[5] = c->st.wlo; out[6] = c->st.whi; out[7] = c->st.ulo; out[8] = c->st.uhi; out[9] = c->st.vlo;
+  out[10] = c->st.vhi;
+  return out;
+}
+
+int cleo_upload_gbx_state(cleo_ctx* c, const cleo_gbx_state* h) {
+  if (!c || !h) return CLEO_E_ARG;
+  double* f[11];
+  state_fields(c, f);
+  const double* src[11] = {h->volume, h->press, h->temp, h->qvap, h->qcond, h->wvel_lo,
+                           h->wvel_hi, h->uvel_lo, h->uvel_hi, h->vvel_lo, h->vvel_hi};
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < 11; ++i) {
+    if (!src[i]) return fail(c, CLEO_E_ARG, "null state field");
+    HIP_TRY(c, hipMemcpy(f[i], src[i], c->grid.ngbx * 8ull, hipMemcpyHostToDevice));
+  }
+  return CLEO_OK;
+}
+
+int cleo_download_gbx_state(cleo_ctx* c, cleo_gbx_state* h) {
+  if (!c || !h) return CLEO_E_ARG;
+  const int rc = sync_check(c);
+  double* f[11];
+  state_fields(c, f);
+  double* dst[11] = {h->volume, h->press, h->temp, h->qvap, h->qcond, h->wvel_lo,
+                     h->wvel_hi, h->uvel_lo, h->uvel_hi, h->vvel_lo, h->vvel_hi};
+  for (int i = 0; i < 11; ++i)
+    if (dst[i]) HIP_TRY(c, hipMemcpy(dst[i], f[i], c->grid.ngbx * 8ull, hipMemcpyDeviceToHost));
+  return rc;
+}
+
+int cleo_set_solute(cleo_ctx* c, const cleo_solute* s) {
+  if (!c || !s || !(s->rho_sol > 0) || !(s->mr_sol > 0)) return CLEO_E_ARG;
+  c->solute = Solute{s->rho_sol, s->mr_sol, s->ionic};
+  return CLEO_OK;
+}
+
+int cleo_set_rng_counter(cleo_ctx* c, uint64_t seed, uint64_t first_event) {
+  if (!c) return CLEO_E_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->rng_mode = 1;
+  c->seed = seed;
+  HIP_TRY(c, hipMemcpy(&c->ctl->event, &first_event, 8, hipMemcpyHostToDevice));
+  return CLEO_OK;
+}
+
+int cleo_set_rng_injected(cleo_ctx* c, const uint32_t* j, uint64_t nj, const double* phi, uint64_t nphi) {
+  if (!c || (nj && !j) || (nphi && !phi)) return CLEO_E_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (nj > c->inj_cap_j) {
+    hipFree(c->inj_j);
+    HIP_TRY(c, dalloc(&c->inj_j, nj));
+    c->inj_cap_j = nj;
+  }
+  if (nphi > c->inj_cap_phi) {
+    hipFree(c->inj_phi);
+    HIP_TRY(c, dalloc(&c->inj_phi, nphi));
+    c->inj_cap_phi = nphi;
+  }
+  if (!c->inj_j) HIP_TRY(c, dalloc(&c->inj_j, 1));
+  if (!c->inj_phi) HIP_TRY(c, dalloc(&c->inj_phi, 1));
+  if (nj) HIP_TRY(c, hipMemcpy(c->inj_j, j, nj * 4, hipMemcpyHostToDevice));
+  if (nphi) HIP_TRY(c, hipMemcpy(c->inj_phi, phi, nphi * 8, hipMemcpyHostToDevice));
+  c->inj_nj = nj;
+  c->inj_nphi = nphi;
+  c->rng_mode = 2;
+  return CLEO_OK;
+}
+
+int cleo_set_sdm(cleo_ctx* c, const cleo_sdm_params* p) {
+  if (!c || !p) return CLEO_E_ARG;
+  if ((p->enable_cond && p->cond.interval == 0) || (p->enable_colls && p->coll.interval == 0) ||
+      (p->enable_motion && p->motion.interval == 0))
+    return fail(c, CLEO_E_ARG, "zero interval");
+  c->sdm = *p;
+  c->sdm_set = true;
+  return CLEO_OK;
+}
+
+int cleo_sort(cleo_ctx* c) {
+  if (!c) return CLEO_E_ARG;
+  const int rc = ensure_radix(c, c->ntot);
+  if (rc != CLEO_OK) return rc;
+  generic_sort(c);
+  return launch_check(c);
+}
+
+int cleo_motion(cleo_ctx* c, const cleo_motion_params* p) {
+  if (!c || !p) return CLEO_E_ARG;
+  return enqueue_motion(c, *p, true);
+}
+
+static int set_substep_error(cleo_ctx* c) {
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(c->h_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
+  if (c->h_ctl->err_code == 0) {
+    int32_t code = CLEO_ERR_SUBSTEP;
+    HIP_TRY(c, hipMemcpy(&c->ctl->err_code, &code, 4, hipMemcpyHostToDevice));
+  }
+  c->msg = "timestep must be at least as large as the minimum sub-timestep";
+  return CLEO_E_DEVICE;
+}
+
+int cleo_condensation(cleo_ctx* c, const cleo_cond_params* p) {
+  if (!c || !p) return CLEO_E_ARG;
+  if (p->interval == 0) return fail(c, CLEO_E_ARG, "interval 0");
+  if (step2dimlesstime(p->interval) < p->minsubdelt_s / dimless::TIME0) return set_substep_error(c);
+  return enqueue_condensation(c, *p, false);
+}
+
+int cleo_collisions(cleo_ctx* c, const cleo_coll_params* p) {
+  if (!c || !p) return CLEO_E_ARG;
+  const int rc = enqueue_collision_event(c, *p);
+  if (rc != CLEO_OK) return rc;
+  neighbour_sort(c, c->cur, 1 - c->cur, &c->ctl->resort_flag, true);
+  return launch_check(c);
+}
+
+int cleo_run_step(cleo_ctx* c, uint32_t t_mdl, uint32_t t_mdl_next) {
+  if (!c) return CLEO_E_ARG;
+  if (!c->sdm_set) return fail(c, CLEO_E_STATE, "cleo_set_sdm not called");
+  if (c->sdm.enable_cond &&
+      step2dimlesstime(c->sdm.cond.interval) < c->sdm.cond.minsubdelt_s / dimless::TIME0)
+    return set_substep_error(c);
+  return enqueue_run_step(c, t_mdl, t_mdl_next, nullptr);
+}
+
+int cleo_run_steps_graph(cleo_ctx* c, uint32_t t_mdl, uint32_t span, uint32_t nsteps) {
+  if (!c || span == 0) return CLEO_E_ARG;
+  if (!c->sdm_set) return fail(c, CLEO_E_STATE, "cleo_set_sdm not called");
+  if (c->rng_mode != 1 && c->sdm.enable_colls) return fail(c, CLEO_E_STATE, "graph replay needs the counter RNG");
+  const cleo_sdm_params& p = c->sdm;
+  auto aligned = [&](bool on, uint32_t iv) { return !on || (span % iv == 0 && t_mdl % iv == 0); };
+  if (!aligned(p.enable_cond, p.cond.interval) || !aligned(p.enable_colls, p.coll.interval) ||
+      !aligned(p.enable_motion, p.motion.interval))
+    return fail(c, CLEO_E_ARG, "graph span must be a multiple of every process interval");
+  if (p.enable_cond && p.cond.warm_start) {
+    const int rc = mlp_check_ready(c);
+    if (rc != CLEO_OK) return rc;
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  // capture one step; if it flips the double buffer an odd number of times, capture two
+  const int cur0 = c->cur;
+  int swaps = 0;
+  hipGraph_t graph = nullptr;
+  HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_run_step(c, t_mdl, t_mdl + span, &swaps);
+  uint32_t per_graph = 1;
+  if (rc == CLEO_OK && (swaps % 2) != 0) {
+    rc = enqueue_run_step(c, t_mdl + span, t_mdl + 2 * span, &swaps);
+    per_graph = 2;
+  }
+  hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (rc != CLEO_OK) {
+    if (graph) hipGraphDestroy(graph);
+    c->cur = cur0;
+    return rc;
+  }
+  if (e != hipSuccess) return fail(c, CLEO_E_HIP, std::string("capture: ") + hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    hipGraphDestroy(graph);
+    return fail(c, CLEO_E_HIP, std::string("instantiate: ") + hipGetErrorString(e));
+  }
+  c->cur = cur0;  // every replay starts and ends on the same buffer
+  const uint32_t nreplay = nsteps / per_graph;
+  for (uint32_t i = 0; i < nreplay; ++i) {
+    e = hipGraphLaunch(exec, c->stream);
+    if (e != hipSuccess) break;
+  }
+  hipGraphExecDestroy(exec);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(c, CLEO_E_HIP, std::string("graph launch: ") + hipGetErrorString(e));
+  if (nsteps % per_graph) rc = enqueue_run_step(c, t_mdl + nreplay * per_graph * span,
+                                               t_mdl + (nreplay * per_graph + 1) * span, nullptr);
+  return rc;
+}
+
+int cleo_synchronize(cleo_ctx* c) {
+  if (!c) return CLEO_E_ARG;
+  return sync_check(c);
+}
+
+int cleo_get_error(cleo_ctx* c, cleo_error* err) {
+  if (!c || !err) return CLEO_E_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(c->h_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
+  err->code = c->h_ctl->err_code;
+  err->gbxindex = c->h_ctl->err_gbx;
+  err->drop = c->h_ctl->err_drop;
+  return CLEO_OK;
+}
+
+int cleo_clear_error(cleo_ctx* c) {
+  if (!c) return CLEO_E_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemset(c->ctl, 0, 16));
+  return CLEO_OK;
+}
+
+int cleo_get_stats(cleo_ctx* c, cleo_stats* s) {
+  if (!c || !s) return CLEO_E_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(c->h_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
+  const unsigned long long* v = c->h_ctl->stats;
+  s->cond_drops = v[ST_COND_DROPS];
+  s->newton_iters = v[ST_NEWTON_ITERS];
+  s->substep_drops = v[ST_SUBSTEP_DROPS];
+  s->warm_restarts = v[ST_WARM_RESTARTS];
+  s->coll_pairs = v[ST_COLL_PAIRS];
+  s->coll_events = v[ST_COLL_EVENTS];
+  s->nulls = v[ST_NULLS];
+  s->sorts = v[ST_SORTS];
+  s->motion_drops = v[ST_MOTION_DROPS];
+  return CLEO_OK;
+}
+
+int cleo_reset_stats(cleo_ctx* c) {
+  if (!c) return CLEO_E_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemset(c->ctl->stats, 0, sizeof(c->ctl->stats)));
+  return CLEO_OK;
+}
+
+int cleo_record_niters(cleo_ctx* c, int32_t enable) {
+  if (!c) return CLEO_E_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->rec_niters = enable != 0;
+  if (c->rec_niters && !c->niters) {
+    HIP_TRY(c, dalloc(&c->niters, c->maxn));
+  }
+  if (c->rec_niters) HIP_TRY(c, hipMemset(c->niters, 0, c->maxn * 4));
+  return CLEO_OK;
+}
+
+int cleo_download_niters(cleo_ctx* c, uint32_t* out, uint64_t n) {
+  if (!c || !out) return CLEO_E_ARG;
+  if (!c->niters) return fail(c, CLEO_E_STATE, "niters recording not enabled");
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(out, c->niters, std::min<uint64_t>(n, c->maxn) * 4, hipMemcpyDeviceToHost));
+  return CLEO_OK;
+}
+
+int cleo_record_guess(cleo_ctx* c, int32_t enable) {
+  if (!c) return CLEO_E_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->rec_guess = enable != 0;
+  if (c->rec_guess && !c->guess) HIP_TRY(c, dalloc(&c->guess, c->maxn));
+  return CLEO_OK;
+}
+
+int cleo_download_guess(cleo_ctx* c, double* out, uint64_t n) {
+  if (!c || !out) return CLEO_E_ARG;
+  if (!c->guess) return fail(c, CLEO_E_STATE, "guess recording not enabled");
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(out, c->guess, std::min<uint64_t>(n, c->maxn) * 8, hipMemcpyDeviceToHost));
+  return CLEO_OK;
+}
+
+int cleo_set_tiles(cleo_ctx* c, const cleo_tiles* t) {
+  if (!c || !t) return CLEO_E_ARG;
+  for (uint32_t v : {t->cond_waves_per_block, t->coll_waves_per_block, t->motion_waves_per_block})
+    if (v > 4) return fail(c, CLEO_E_ARG, "waves per block must be 0..4");
+  c->tiles = *t;
+  return CLEO_OK;
+}
+
+int cleo_get_tiles(cleo_ctx* c, cleo_tiles* t) {
+  if (!c || !t) return CLEO_E_ARG;
+  *t = c->tiles;
+  return CLEO_OK;
+}
+
+}  // extern "C"

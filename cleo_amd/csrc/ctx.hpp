@@ -1,0 +1,104 @@
+// ctx.hpp -- the device context behind the opaque cleo_ctx of include/cleo_hip.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "physics.hpp"
+
+namespace cleo {
+
+// Structure-of-arrays superdroplet store in HBM: 60 bytes per drop.
+struct SoA {
+  uint32_t* gbx;
+  double* c3;
+  double* c1;
+  double* c2;
+  uint64_t* xi;
+  double* r;
+  double* msol;
+  uint64_t* id;
+};
+
+// Rectilinear grid; gbxindex = iz + nz*(ix + nx*iy).
+struct Grid {
+  uint32_t nz, nx, ny, ngbx;
+  int32_t per[3];
+  const double* zh;
+  const double* xh;
+  const double* yh;
+};
+
+// Gridbox State, one array per field.
+struct GState {
+  double* vol;
+  double* press;
+  double* temp;
+  double* qv;
+  double* qc;
+  double* wlo;
+  double* whi;
+  double* ulo;
+  double* uhi;
+  double* vlo;
+  double* vhi;
+};
+
+struct MLPDev;  // mlp.hip
+
+}  // namespace cleo
+
+struct cleo_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  cleo::Grid grid{};
+  double* d_halves = nullptr;
+  uint64_t maxn = 0;
+  uint64_t ntot = 0;
+  cleo::SoA buf[2]{};
+  int cur = 0;
+  uint32_t* off = nullptr;      // ngbx+1: gridbox refs, off[ngbx] = in-domain count
+  uint32_t* off_alt = nullptr;  // ngbx+1: next refs while sorting
+  uint32_t* nact = nullptr;     // ngbx: active drops per gridbox (subview extent)
+  cleo::GState st{};
+  double* d_state = nullptr;
+  cleo::Solute solute{};
+  // neighbour-sort scratch
+  uint32_t* cnt = nullptr;      // ngbx*28
+  uint32_t* tot = nullptr;      // ngbx+1
+  uint32_t* oobpre = nullptr;   // ngbx+1
+  uint32_t* scan_tmp = nullptr;
+  size_t scan_tmp_n = 0;
+  // generic radix sort scratch (lazily allocated)
+  uint32_t* rs_key[2] = {nullptr, nullptr};
+  uint32_t* rs_idx[2] = {nullptr, nullptr};
+  uint32_t* rs_cnt = nullptr;
+  size_t rs_n = 0;
+  // control
+  cleo::DevCtl* ctl = nullptr;
+  cleo::DevCtl* h_ctl = nullptr;  // pinned mirror
+  // RNG
+  int rng_mode = 0;  // 0 none, 1 counter, 2 injected
+  uint64_t seed = 0;
+  uint32_t* inj_j = nullptr;
+  double* inj_phi = nullptr;
+  uint64_t inj_cap_j = 0, inj_cap_phi = 0, inj_nj = 0, inj_nphi = 0;
+  uint32_t* inj_oj = nullptr;  // ngbx+1
+  uint32_t* inj_op = nullptr;  // ngbx+1
+  // large-gridbox collision scratch (permutation + indices for gridboxes above the LDS capacity)
+  uint32_t* big_scratch = nullptr;
+  uint64_t big_scratch_n = 0;
+  // configuration
+  cleo_sdm_params sdm{};
+  bool sdm_set = false;
+  cleo_tiles tiles{1, 1, 1};
+  // per-drop Newton iteration record
+  bool rec_niters = false;
+  uint32_t* niters = nullptr;
+  // learned guess record (z) for tests
+  bool rec_guess = false;
+  double* guess = nullptr;
+  // MLP
+  cleo::MLPDev* mlp = nullptr;
+  std::string msg;
+};

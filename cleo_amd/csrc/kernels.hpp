@@ -1,0 +1,52 @@
+// kernels.hpp -- wave-level helpers shared by the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ctx.hpp"
+
+namespace cleo {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & (kWave - 1)); }
+__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+
+// LDS / memory ordering between lanes of one wavefront (no workgroup barrier needed)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int l) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), l));
+}
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+// Adds the per-lane values v of lanes 0..m-1 to `acc` strictly in lane order, so a
+// gridbox sum keeps the sequential order of the reference's Serial reduction.
+__device__ __forceinline__ double ordered_add(double acc, double v, int m) {
+  for (int i = 0; i < m; ++i) acc += readlane_f64(v, i);
+  return acc;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// first error wins; later errors are dropped
+__device__ __forceinline__ void set_error(DevCtl* ctl, int code, uint32_t gbx, uint64_t drop) {
+  if (code) record_error(ctl, code, gbx, drop);
+}
+
+__device__ __forceinline__ void add_stat(DevCtl* ctl, int idx, unsigned long long v) {
+  if (v) atomicAdd(&ctl->stats[idx], v);
+}
+
+}  // namespace cleo

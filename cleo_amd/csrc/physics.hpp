@@ -1,0 +1,546 @@
+// physics.hpp -- device-side super-droplet physics for gfx950.
+//
+// Each function states the reference formula it evaluates (CLEO libs/...); the floating-point
+// operation order follows the reference expression so that, compiled with -ffp-contract=off,
+// every +,-,*,/,sqrt rounds as the reference's host build does.  Transcendentals (pow, exp)
+// come from the device math library and may differ from glibc by an ulp.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/cleo_hip.h"
+
+namespace cleo {
+
+// ---- constants: libs/cleoconstants.hpp:32-104 -------------------------------------------
+namespace dimmed {
+constexpr double RGAS_UNIV = 8.314462618;
+constexpr double MR_WATER = 0.01801528;
+constexpr double MR_DRY = 0.028966216;
+constexpr double RGAS_DRY = RGAS_UNIV / MR_DRY;
+constexpr double RGAS_V = RGAS_UNIV / MR_WATER;
+constexpr double LATENT_V = 2500930;
+constexpr double CP_DRY = 1004.64;
+constexpr double CP_V = 1865.01;
+constexpr double C_L = 4192.664;
+constexpr double RHO_L = 998.203;
+}  // namespace dimmed
+namespace dimless {
+constexpr double W0 = 1.0;
+constexpr double TIME0 = 1000.0;
+constexpr double COORD0 = TIME0 * W0;
+constexpr double VOL0 = COORD0 * COORD0 * COORD0;
+constexpr double CP0 = dimmed::CP_DRY;
+constexpr double MR0 = dimmed::MR_DRY;
+constexpr double R0 = 1e-6;
+constexpr double P0 = 100000.0;
+constexpr double TEMP0 = 273.15;
+constexpr double RHO0 = P0 / (CP0 * TEMP0);
+constexpr double MASS0 = R0 * R0 * R0 * RHO0;
+constexpr double MASS0grams = MASS0 * 1000;
+constexpr double F0 = TIME0 / (RHO0 * R0 * R0);
+constexpr double Mr_ratio = dimmed::MR_WATER / dimmed::MR_DRY;
+constexpr double Cp_dry = dimmed::CP_DRY / CP0;
+constexpr double Cp_v = dimmed::CP_V / CP0;
+constexpr double C_l = dimmed::C_L / CP0;
+constexpr double Latent_v = dimmed::LATENT_V / (TEMP0 * CP0);
+constexpr double Rgas_dry = dimmed::RGAS_DRY / CP0;
+constexpr double Rgas_v = dimmed::RGAS_V / CP0;
+constexpr double Rho_l = dimmed::RHO_L / RHO0;
+constexpr double R0cubed_VOL0 = R0 * R0 * R0 / VOL0;
+}  // namespace dimless
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr uint32_t kOOB = 0xFFFFFFFFu;
+constexpr double kDblMax = 1.7976931348623157e308;
+
+struct Solute {
+  double rho_sol, mr_sol, ionic;
+};
+
+// ---- device control block ------------------------------------------------------------------
+enum StatIdx {
+  ST_COND_DROPS = 0,
+  ST_NEWTON_ITERS,
+  ST_SUBSTEP_DROPS,
+  ST_WARM_RESTARTS,
+  ST_COLL_PAIRS,
+  ST_COLL_EVENTS,
+  ST_NULLS,
+  ST_SORTS,
+  ST_MOTION_DROPS,
+  ST_COUNT
+};
+struct DevCtl {
+  int32_t err_code;
+  uint32_t err_gbx;
+  uint64_t err_drop;
+  uint32_t resort_flag;  // nulls appeared in this microphysics pass
+  uint32_t pad0;
+  uint64_t event;        // collision events so far (counter RNG)
+  unsigned long long stats[ST_COUNT];
+};
+
+__device__ __forceinline__ void record_error(DevCtl* ctl, int code, uint32_t gbx, uint64_t drop) {
+  if (atomicCAS(&ctl->err_code, 0, code) == 0) {
+    ctl->err_gbx = gbx;
+    ctl->err_drop = drop;
+  }
+}
+
+// ---- superdrop attributes: superdrop_attrs.{hpp,cpp} ---------------------------------------
+__device__ __forceinline__ double rcubed(double r) { return r * r * r; }
+__device__ __forceinline__ double dryradius(double msol, const Solute& s) {
+  constexpr double vconst = 3.0 / (4.0 * kPi);
+  const double dryrcubed = vconst * msol / s.rho_sol;
+  return pow(dryrcubed, 1.0 / 3.0);
+}
+__device__ __forceinline__ double drop_mass(double r, double msol, const Solute& s) {
+  constexpr double massconst(4.0 / 3.0 * kPi * dimless::Rho_l);
+  const double density_factor = 1.0 - dimless::Rho_l / s.rho_sol;
+  double m = msol * density_factor;
+  m += massconst * rcubed(r);
+  return m;
+}
+// condensate_mass; *bad set when below the reference's tolerance (superdrop_attrs.hpp:207-216)
+__device__ __forceinline__ double condensate_mass(double r, double msol, const Solute& s, bool* bad) {
+  const double m_cond = drop_mass(r, msol, s) - msol;
+  if (m_cond <= -0.0001 * msol) *bad = true;
+  return fmax(0.0, m_cond);
+}
+__device__ __forceinline__ bool radius_below_dry(double r, double msol, const Solute& s) {
+  return r - dryradius(msol, s) <= -1e-12 / dimless::R0;
+}
+
+// ---- thermodynamics: thermodynamic_equations.{hpp,cpp} -------------------------------------
+__device__ __forceinline__ double dry_air_density(double press, double temp, double qvap) {
+  const double rgas_equiv = dimless::Rgas_dry + dimless::Rgas_v * qvap;
+  return press / (rgas_equiv * temp);
+}
+__device__ __forceinline__ double moist_specific_heat(double qvap, double qcond) {
+  return dimless::Cp_dry + dimless::Cp_v * (qvap) + dimless::C_l * (qcond);
+}
+__device__ __forceinline__ double supersaturation_ratio(double press, double qvap, double psat) {
+  return (press * qvap) / ((dimless::Mr_ratio + qvap) * psat);
+}
+__device__ __forceinline__ double saturation_pressure(double temp) {
+  constexpr double A = 17.4146, B = 33.639, TREF = 273.16, PREF = 611.655;
+  const double T = temp * dimless::TEMP0;
+  return (PREF * exp(A * (T - TREF) / (T - B))) / dimless::P0;
+}
+__device__ __forceinline__ double diffusion_factor(double press, double temp, double psat) {
+  constexpr double A = 7.11756e-5;
+  constexpr double B = 4.38127686e-3;
+  constexpr double LATENT_RGAS_V = dimmed::LATENT_V / dimmed::RGAS_V;
+  constexpr double D = 4.012182971e-5;
+  const double TEMP = temp * dimless::TEMP0;
+  const double PRESS = press * dimless::P0;
+  const double PSAT = psat * dimless::P0;
+  const double THERMK = A * (TEMP * TEMP) + TEMP * B;  // pow(TEMP, 2.0) is exactly TEMP*TEMP rounded
+  const double DIFFUSE_V = (D / PRESS * pow(TEMP, 1.94)) / dimmed::RGAS_V;
+  const double fkl = (LATENT_RGAS_V / TEMP - 1.0) * dimmed::LATENT_V / (THERMK * dimless::F0);
+  const double fdl = TEMP / (DIFFUSE_V * PSAT) / dimless::F0;
+  return dimless::Rho_l * (fkl + fdl);
+}
+__device__ __forceinline__ double ventilation_factor(double radius) {
+  constexpr double vent_factor_max = 20;
+  constexpr double C1 = 6.954e+7, A = 1.963, C2 = 1.069e+3, B = 0.702;
+  const double a = 1.0 / (C1 * pow(radius * dimless::R0, A));
+  const double b = 1.0 / (C2 * pow(radius * dimless::R0, B));
+  const double vent = 1.0 + 1.0 / (a + b);
+  return fmin(vent, vent_factor_max);
+}
+__device__ __forceinline__ void kohler_factors(double msol, double temp, const Solute& s, double& akoh,
+                                               double& bkoh) {
+  constexpr double akoh_constant = 3.3e-7 / (dimless::TEMP0 * dimless::R0);
+  akoh = akoh_constant / temp;
+  constexpr double bkoh_constant = 4.3e-6 * dimless::RHO0 / dimless::MR0;
+  bkoh = bkoh_constant * msol * s.ionic / s.mr_sol;
+}
+
+// ---- implicit Euler: impliciteuler.{hpp,cpp} -------------------------------------------------
+struct ODEConsts {
+  double s_ratio, akoh, bkoh, ffactor_fv;
+};
+struct NewtonCfg {
+  double delt, minsubdelt, rtol, atol;
+  uint32_t maxniters;
+};
+// g(z) (impliciteuler.cpp:270-281); pow(x, 2.0) is exactly x*x rounded, as the host compilers fold it
+__device__ __forceinline__ double ode_gfunc(const ODEConsts& c, double subdelt, double rprev,
+                                            double rsqrd) {
+  const double radius = sqrt(rsqrd);
+  const double alpha = c.s_ratio - 1 - c.akoh / radius + c.bkoh / pow(radius, 3.0);
+  const double beta = 2.0 * subdelt / (rsqrd * c.ffactor_fv);
+  const double q = rprev / radius;
+  const double gamma = q * q;
+  return 1 - gamma - alpha * beta;
+}
+// dg/dz (impliciteuler.cpp:291-301)
+__device__ __forceinline__ double ode_gfuncderivative(const ODEConsts& c, double subdelt, double rsqrd) {
+  const double radius = sqrt(rsqrd);
+  const double alpha = c.akoh / radius - 3.0 * c.bkoh / pow(radius, 3.0);
+  const double beta = subdelt / (rsqrd * c.ffactor_fv);
+  return 1 - alpha * beta;
+}
+// one Newton update + convergence test (impliciteuler.cpp:240-255, impliciteuler.hpp:214-219)
+__device__ __forceinline__ bool newton_iterate(const NewtonCfg& cfg, const ODEConsts& c, double subdelt,
+                                               double rprev, double& ziter, uint32_t& iters) {
+  ++iters;
+  const double numerator = ode_gfunc(c, subdelt, rprev, ziter);
+  const double denominator = ode_gfuncderivative(c, subdelt, ziter);
+  ziter = ziter * (1 - numerator / denominator);
+  ziter = fmax(ziter, 1e-8);
+  const double newnumerator = ode_gfunc(c, subdelt, rprev, ziter);
+  const double converged = cfg.rtol * fabs(newnumerator) + cfg.atol;
+  const double current = fabs(newnumerator - numerator);
+  return current < converged;
+}
+// newtonraphson_untilconverged (impliciteuler.cpp:207-227): false when `limit` is exceeded
+__device__ __forceinline__ bool newton_until(const NewtonCfg& cfg, const ODEConsts& c, uint32_t limit,
+                                             double subdelt, double rprev, double& ziter,
+                                             uint32_t& iters) {
+  uint32_t niter = 1;
+  bool conv = false;
+  while (!conv) {
+    if (niter > limit) return false;
+    conv = newton_iterate(cfg, c, subdelt, rprev, ziter, iters);
+    niter += 1;
+  }
+  return true;
+}
+// integrate_condensation_ode (impliciteuler.cpp:128-141): 2 iterations, then restart from the guess
+__device__ __forceinline__ double newton_integrate(const NewtonCfg& cfg, const ODEConsts& c, double subdelt,
+                                                   double rprev, double zguess, uint32_t& iters,
+                                                   bool& maxed) {
+  double z = zguess;
+  bool conv = newton_iterate(cfg, c, subdelt, rprev, z, iters);
+  conv = newton_iterate(cfg, c, subdelt, rprev, z, iters);
+  if (conv) return z;
+  z = zguess;
+  if (!newton_until(cfg, c, cfg.maxniters, subdelt, rprev, z, iters)) maxed = true;
+  return z;
+}
+__device__ __forceinline__ double initialguess(const ODEConsts& c, double rprev) {
+  const double s_act = 1 + sqrt(4.0 * pow(c.akoh, 3.0) / 27 / c.bkoh);
+  if (c.s_ratio > s_act) {
+    constexpr double bigr(1e-3 / dimless::R0);
+    const double maxradius = fmax(bigr, rprev);
+    return maxradius * maxradius;
+  }
+  return rprev * rprev;
+}
+__device__ __forceinline__ bool first_unique_criteria(const ODEConsts& c, double rprev, double ziter) {
+  const double rcritsqrd = 3.0 * c.bkoh / c.akoh;
+  const bool is_ziter_unactivated = ziter < rcritsqrd;
+  const bool is_unactivated = (rprev * rprev < rcritsqrd && is_ziter_unactivated);
+  const double sqrdval = 4.0 * c.akoh * c.akoh * c.akoh / 27.0 / c.bkoh;
+  const bool is_subact = (c.s_ratio <= 1.0 + pow(sqrdval, 0.5));
+  return is_unactivated && is_subact;
+}
+__device__ __forceinline__ double critical_timestep(const ODEConsts& c) {
+  const double cuberoot = pow(5.0 * c.bkoh / c.akoh, 1.5);
+  return 2.5 * c.ffactor_fv / c.akoh * cuberoot;
+}
+
+struct SolveOut {
+  uint32_t iters;
+  bool substep, restarted, maxed;
+};
+// solve_condensation (impliciteuler.cpp:35-54). `warm` is a learned guess for z = r^2 or NaN.
+// Learned path: the unchanged two-phase Newton from `warm` with `warm_limit` as until-converged
+// limit; if that fails, the reference path from the plain guess.
+__device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, double s_ratio, double akoh,
+                                                     double bkoh, double ffactor, double rprev, double warm,
+                                                     uint32_t warm_limit, SolveOut& o) {
+  const double ffactor_fv = ffactor / ventilation_factor(rprev);
+  const ODEConsts c{s_ratio, akoh, bkoh, ffactor_fv};
+  double ziter = initialguess(c, rprev);
+  const bool ucrit1 = first_unique_criteria(c, rprev, ziter);
+  const bool ucrit2 = (cfg.delt <= critical_timestep(c));
+  double rsqrd;
+  if (ucrit1 || ucrit2) {
+    if (warm == warm) {
+      double z = warm;
+      bool conv = newton_iterate(cfg, c, cfg.delt, rprev, z, o.iters);
+      conv = newton_iterate(cfg, c, cfg.delt, rprev, z, o.iters);
+      if (conv) {
+        rsqrd = z;
+      } else {
+        z = warm;
+        if (newton_until(cfg, c, warm_limit, cfg.delt, rprev, z, o.iters)) {
+          rsqrd = z;
+        } else {
+          o.restarted = true;
+          rsqrd = newton_integrate(cfg, c, cfg.delt, rprev, ziter, o.iters, o.maxed);
+        }
+      }
+    } else {
+      rsqrd = newton_integrate(cfg, c, cfg.delt, rprev, ziter, o.iters, o.maxed);
+    }
+  } else {
+    o.substep = true;
+    const double critdelt = critical_timestep(c);
+    const double mindelt = fmax(critdelt, cfg.minsubdelt);
+    double remdelt = cfg.delt;
+    double rp = rprev;
+    while (remdelt > 0.0) {
+      const double subdelt = fmin(mindelt, remdelt);
+      ziter = newton_integrate(cfg, c, subdelt, rp, ziter, o.iters, o.maxed);
+      rp = pow(ziter, 0.5);
+      remdelt -= subdelt;
+    }
+    rsqrd = ziter;
+  }
+  return sqrt(rsqrd);
+}
+
+// ---- terminal velocity: terminalvelocity.cpp:33-146 ---------------------------------------
+__device__ __forceinline__ double tv_simmel(double radius) {
+  constexpr double r1 = 6.7215e-5 / dimless::R0;
+  constexpr double r2 = 7.5582e-4 / dimless::R0;
+  constexpr double r3 = 1.73892e-3 / dimless::R0;
+  constexpr double VELCONST = (100.0 * dimless::W0);
+  constexpr double a1 = 457950 / VELCONST;
+  constexpr double a2 = 4962 / VELCONST;
+  constexpr double a3 = 1732 / VELCONST;
+  constexpr double a4 = 917 / VELCONST;
+  if (radius >= r3) return a4;
+  constexpr double massconst(4.0 / 3.0 * kPi * dimless::Rho_l);
+  const double MASS = (massconst * radius * radius * radius) * dimless::MASS0grams;
+  if (radius >= r2) return a3 * pow(MASS, 1.0 / 6.0);
+  if (radius >= r1) return a2 * pow(MASS, 1.0 / 3.0);
+  return a1 * pow(MASS, 2.0 / 3.0);
+}
+__device__ __forceinline__ double tv_rogersyau(double radius) {
+  constexpr double r1 = 3e-5 / dimless::R0;
+  constexpr double r2 = 6e-4 / dimless::R0;
+  constexpr double r3 = 2e-3 / dimless::R0;
+  constexpr double k1 = 1.19e8 * dimless::R0 * dimless::R0 / dimless::W0;
+  constexpr double k2 = 8000 * dimless::R0 / dimless::W0;
+  constexpr double k3 = 201 / dimless::W0;
+  constexpr double k4 = 9 / dimless::W0;
+  if (radius < r1) return k1 * (radius * radius);
+  if (radius < r2) return k2 * radius;
+  if (radius < r3) return k3 * pow((radius * dimless::R0), 0.5);
+  return k4;
+}
+__device__ __forceinline__ double tv_rogersgk(double radius) {
+  constexpr double radius0 = 3.725 * 1e-4 / dimless::R0;
+  constexpr double kcaps = 2.0 * 4.0 * 1000 * dimless::R0 / dimless::W0;
+  constexpr double smallk = -1.0 * 2.0 * 12.0 * 1000 * dimless::R0;
+  constexpr double acaps = 9.65 / dimless::W0;
+  constexpr double bcaps = 10.43 / dimless::W0;
+  constexpr double ccaps = -1.0 * 2.0 * 0.6 * 1000 * dimless::R0;
+  if (radius < radius0) {
+    const double term = 1.0 - exp(smallk * radius);
+    return term * kcaps * radius;
+  }
+  return acaps - bcaps * exp(ccaps * radius);
+}
+__device__ __forceinline__ double terminal_velocity(int kind, double radius) {
+  switch (kind) {
+    case CLEO_TV_SIMMEL: return tv_simmel(radius);
+    case CLEO_TV_ROGERS_YAU: return tv_rogersyau(radius);
+    case CLEO_TV_ROGERS_GK: return tv_rogersgk(radius);
+    default: return 0.0;
+  }
+}
+
+// ---- pair probabilities ---------------------------------------------------------------------
+struct CollCfg {
+  int prob_kind;
+  double prob_param;
+  int enact_kind;
+  double nfrags;
+  double coaleff;
+  double DELT;
+};
+__device__ __forceinline__ double vol_of(double r) { return 4.0 / 3.0 * kPi * rcubed(r); }
+// golovinprob.cpp:32-43, longhydroprob.cpp:30-48 + hydrodynamicprob.hpp:50-66, constprob.hpp:45-48
+__device__ __forceinline__ double pair_probability(const CollCfg& k, double r1, double r2, double VOLUME) {
+  if (k.prob_kind == CLEO_PROB_GOLOVIN) {
+    constexpr double prob_jk_const = 1.5e3 * dimless::R0 * dimless::R0 * dimless::R0;
+    const double DELT_DELVOL = k.DELT / VOLUME;
+    const double kern = prob_jk_const * (vol_of(r1) + vol_of(r2));
+    return kern * DELT_DELVOL;
+  }
+  if (k.prob_kind == CLEO_PROB_LONG) {
+    constexpr double k1 = 4.5e4 * dimless::R0 * dimless::R0 * 100 * 100;
+    constexpr double k2 = 3e-4 / dimless::R0 / 100;
+    constexpr double rlim = 5e-5 / dimless::R0;
+    constexpr double colleff_min = 0.001;
+    const double rsmall = fmin(r1, r2);
+    const double rbig = fmax(r1, r2);
+    double colleff = 1.0;
+    if (rbig < rlim) {
+      colleff = k1 * rbig * rbig * (1 - (k2 / rsmall));
+      colleff = fmax(colleff, colleff_min);
+    }
+    const double eff = colleff * k.prob_param;
+    constexpr double prob_jk_const = kPi * dimless::R0 * dimless::R0 * dimless::W0;
+    const double DELT_DELVOL = k.DELT / VOLUME;
+    const double sumr = r1 + r2;
+    const double sumrsqrd = sumr * sumr;
+    const double vdiff = fabs(tv_simmel(r1) - tv_simmel(r2));
+    const double hydro_kernel = prob_jk_const * eff * sumrsqrd * vdiff;
+    return hydro_kernel * DELT_DELVOL;
+  }
+  return k.prob_param * k.DELT / VOLUME;
+}
+
+// ---- enactment on (xi, r, msol) registers -----------------------------------------------------
+struct DropAttrs {
+  uint64_t xi;
+  double r, msol;
+};
+// coalescence_gamma (coalescence.cpp:68-79)
+__device__ __forceinline__ uint64_t coal_gamma(uint64_t xi1, uint64_t xi2, double prob, double phi) {
+  uint64_t gamma = static_cast<uint64_t>(floor(prob));
+  if (phi < (prob - static_cast<double>(gamma))) ++gamma;
+  const uint64_t maxgamma = xi1 / xi2;
+  return static_cast<uint64_t>(fmin(static_cast<double>(gamma), static_cast<double>(maxgamma)));
+}
+// returns error code (0 ok); *null set when drop1 became null (coalescence.cpp:92-196)
+__device__ __forceinline__ int coalesce(uint64_t gamma, DropAttrs& d1, DropAttrs& d2, const Solute& s,
+                                        bool& null1) {
+  const uint64_t xi1 = d1.xi, xi2 = d2.xi;
+  int err = 0;
+  if (xi1 - gamma * xi2 > 0) {
+    const uint64_t new_xi = xi1 - gamma * xi2;
+    if (new_xi >= xi1) err = CLEO_ERR_COAL_UNDEFINED;
+    d1.xi = new_xi;
+    const double new_rcubed = rcubed(d2.r) + static_cast<double>(gamma) * rcubed(d1.r);
+    const double nr = pow(new_rcubed, (1.0 / 3.0));
+    if (radius_below_dry(nr, d2.msol, s)) err = err ? err : CLEO_ERR_RADIUS_BELOW_DRY;
+    d2.r = nr;
+    d2.msol = d2.msol + static_cast<double>(gamma) * d1.msol;
+    return err;
+  }
+  if (xi1 - gamma * xi2 == 0) {
+    const uint64_t old_xi = xi2;
+    const uint64_t new_xi = old_xi / 2;
+    if (new_xi >= old_xi) err = CLEO_ERR_COAL_UNDEFINED;
+    const double new_rcubed = rcubed(d2.r) + static_cast<double>(gamma) * rcubed(d1.r);
+    const double new_r = pow(new_rcubed, (1.0 / 3.0));
+    const double new_msol = d2.msol + static_cast<double>(gamma) * d1.msol;
+    if (new_xi == 0) {
+      d1.xi = 0;
+      null1 = true;
+    } else {
+      d1.xi = new_xi;
+    }
+    if (old_xi - new_xi == 0) err = err ? err : CLEO_ERR_XI_ZERO;
+    d2.xi = old_xi - new_xi;
+    if (radius_below_dry(new_r, d1.msol, s) || radius_below_dry(new_r, d2.msol, s))
+      err = err ? err : CLEO_ERR_RADIUS_BELOW_DRY;
+    d1.r = new_r;
+    d2.r = new_r;
+    d1.msol = new_msol;
+    d2.msol = new_msol;
+    return err;
+  }
+  return CLEO_ERR_COAL_UNDEFINED;
+}
+// breakup with ConstNFrags (breakup.hpp:176-268)
+__device__ __forceinline__ int breakup(double nfrags, DropAttrs& d1, DropAttrs& d2, const Solute& s) {
+  int err = 0;
+  if (d1.xi == d2.xi) {
+    const uint64_t old_xi = d2.xi;
+    const double totnfrags = nfrags * static_cast<double>(old_xi);
+    if ((totnfrags / static_cast<double>(old_xi)) <= 2.5) err = CLEO_ERR_BREAKUP_NFRAGS;
+    const uint64_t new_xi1 = static_cast<uint64_t>(round(totnfrags / 2));
+    const uint64_t new_xi2 = static_cast<uint64_t>(round(totnfrags - static_cast<double>(new_xi1)));
+    const uint64_t new_xitot = new_xi1 + new_xi2;
+    if (new_xi2 <= old_xi) err = err ? err : CLEO_ERR_BREAKUP_NFRAGS;
+    if (new_xitot <= (old_xi * 2)) err = err ? err : CLEO_ERR_BREAKUP_NFRAGS;
+    const double sum_rcubed = rcubed(d1.r) + rcubed(d2.r);
+    const double new_rcubed = sum_rcubed * static_cast<double>(old_xi) / static_cast<double>(new_xitot);
+    const double new_r = pow(new_rcubed, (1.0 / 3.0));
+    const double sum_msol = d1.msol + d2.msol;
+    const double new_msol = sum_msol * static_cast<double>(old_xi) / static_cast<double>(new_xitot);
+    if (new_xi1 == 0 || new_xi2 == 0) err = err ? err : CLEO_ERR_XI_ZERO;
+    d1.xi = new_xi1;
+    d2.xi = new_xi2;
+    if (radius_below_dry(new_r, d1.msol, s) || radius_below_dry(new_r, d2.msol, s))
+      err = err ? err : CLEO_ERR_RADIUS_BELOW_DRY;
+    d1.r = new_r;
+    d2.r = new_r;
+    d1.msol = new_msol;
+    d2.msol = new_msol;
+    return err;
+  }
+  const uint64_t old_xi1 = d1.xi, old_xi2 = d2.xi;
+  const uint64_t new_xi1 = old_xi1 - old_xi2;
+  if (new_xi1 == 0) err = CLEO_ERR_XI_ZERO;
+  d1.xi = new_xi1;
+  const double totnfrags = nfrags * static_cast<double>(old_xi2);
+  const uint64_t new_xi2 = static_cast<uint64_t>(round(totnfrags));
+  if ((totnfrags / static_cast<double>(old_xi2)) <= 2.5) err = err ? err : CLEO_ERR_BREAKUP_NFRAGS;
+  if (new_xi2 <= old_xi2) err = err ? err : CLEO_ERR_BREAKUP_NFRAGS;
+  if ((new_xi1 + new_xi2) <= (old_xi1 + old_xi2)) err = err ? err : CLEO_ERR_BREAKUP_NFRAGS;
+  const double sum_rcubed = rcubed(d1.r) + rcubed(d2.r);
+  const double new_rcubed = sum_rcubed * static_cast<double>(old_xi2) / static_cast<double>(new_xi2);
+  const double new_r = pow(new_rcubed, (1.0 / 3.0));
+  const double sum_msol = d1.msol + d2.msol;
+  const double new_msol = sum_msol * static_cast<double>(old_xi2) / static_cast<double>(new_xi2);
+  if (new_xi2 == 0) err = err ? err : CLEO_ERR_XI_ZERO;
+  d2.xi = new_xi2;
+  if (radius_below_dry(new_r, d2.msol, s)) err = err ? err : CLEO_ERR_RADIUS_BELOW_DRY;
+  d2.r = new_r;
+  d2.msol = new_msol;
+  return err;
+}
+// PairEnactX dispatch; bit0 of the return: drop1 null, bit1: enacted; error via *err
+__device__ __forceinline__ uint32_t enact_pair(const CollCfg& k, DropAttrs& d1, DropAttrs& d2, double prob,
+                                               double phi_coll, double phi_out, const Solute& s, int& err) {
+  bool null1 = false;
+  if (k.enact_kind == CLEO_ENACT_BREAKUP) {
+    if (phi_coll < (prob - floor(prob))) {
+      err = breakup(k.nfrags, d1, d2, s);
+      return 2u;
+    }
+    return 0u;
+  }
+  const uint64_t gamma = coal_gamma(d1.xi, d2.xi, prob, phi_coll);
+  if (gamma == 0) return 0u;
+  if (k.enact_kind == CLEO_ENACT_COAL || phi_out < k.coaleff) {
+    err = coalesce(gamma, d1, d2, s, null1);
+  } else {
+    err = breakup(k.nfrags, d1, d2, s);
+  }
+  return 2u | (null1 ? 1u : 0u);
+}
+
+// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) ---------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+    const uint32_t lo0 = 0xD2511F53u * c[0];
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    const uint32_t lo1 = 0xCD9E8D57u * c[2];
+    const uint32_t n0 = hi1 ^ c[1] ^ k0;
+    const uint32_t n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+// stream 0: Fisher-Yates index for `iter`; stream 1: (phi_coll, phi_out) for pair `jj`
+__device__ __forceinline__ void rng_block(uint64_t seed, uint64_t event, uint32_t stream, uint32_t gbx,
+                                          uint32_t item, uint32_t out[4]) {
+  out[0] = item;
+  out[1] = gbx;
+  out[2] = static_cast<uint32_t>(event);
+  out[3] = stream + 2u * static_cast<uint32_t>(event >> 32);
+  philox4x32_10(out, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+}
+__device__ __forceinline__ double u53(uint32_t lo, uint32_t hi) {
+  const uint64_t v = (static_cast<uint64_t>(hi) << 32) | lo;
+  return static_cast<double>(v >> 11) * 0x1.0p-53;
+}
+
+}  // namespace cleo

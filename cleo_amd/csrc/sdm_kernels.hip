@@ -1,0 +1,943 @@
+// sdm_kernels.hip -- gfx950 kernels of the super-droplet step.
+//
+// Launch shape: one wave64 per gridbox; `wpb` gridboxes (waves) per workgroup.  Drops of a
+// gridbox are contiguous in the SoA store (sorted by sdgbxindex), so every per-drop load and
+// store of a wave is a coalesced run.  No kernel uses atomics on drop data: sums keep the
+// reference's sequential order and the sort is stable by construction.
+#include "kernels.hpp"
+#include "sdm_kernels.hpp"
+
+namespace cleo {
+
+// ============================ condensation ====================================================
+// DoCondensation::do_condensation for one gridbox per wave (condensation.hpp:141-150,
+// condensation.cpp:44-137).  With `fuse_qcond`, the gridbox qcond of
+// EffectOnHydrometeorStatesFunctor (movesupersindomain.hpp:95-136) is recomputed first, as
+// happens between motion and condensation in the reference step.
+__global__ void __launch_bounds__(256) k_condensation(CondArgs a) {
+  const int lane = lane_id();
+  const uint32_t g = blockIdx.x * a.wpb + (threadIdx.x >> 6);
+  if (g >= a.ngbx) return;
+  const uint32_t base = a.off[g];
+  const uint32_t nfull = a.off[g + 1] - base;
+  const uint32_t n = a.nact[g];
+  const double temp = a.st.temp[g];
+  const double press = a.st.press[g];
+  double qv = a.st.qv[g];
+  double qc = a.st.qc[g];
+  const double volume = a.st.vol[g];
+  bool bad_cm = false;
+
+  if (a.fuse_qcond) {
+    double tot = 0.0;
+    for (uint32_t c = 0; c < nfull; c += kWave) {
+      const uint32_t k = base + c + lane;
+      double v = 0.0;
+      if (c + lane < nfull) {
+        v = condensate_mass(a.s.r[k], a.s.msol[k], a.sol, &bad_cm) * static_cast<double>(a.s.xi[k]);
+      }
+      const int m = static_cast<int>(min(static_cast<uint32_t>(kWave), nfull - c));
+      tot = ordered_add(tot, v, m);
+    }
+    const double rho_cond = tot / volume * dimless::R0cubed_VOL0;
+    const double rho_dry = dry_air_density(press, temp, qv);
+    qc = rho_cond / rho_dry;
+  }
+  if (temp <= 0 && lane == 0) set_error(a.ctl, CLEO_ERR_TEMP_NONPOSITIVE, g, base);
+  const double psat = saturation_pressure(temp);
+  const double s_ratio = supersaturation_ratio(press, qv, psat);
+  const double ffactor = diffusion_factor(press, temp, psat);
+
+  double tot = 0.0;
+  uint32_t iters = 0, subs = 0, restarts = 0;
+  for (uint32_t c = 0; c < n; c += kWave) {
+    const uint32_t k = base + c + lane;
+    double dm = 0.0;
+    if (c + lane < n) {
+      const double r = a.s.r[k];
+      const double msol = a.s.msol[k];
+      const uint64_t xi = a.s.xi[k];
+      const double old_m = condensate_mass(r, msol, a.sol, &bad_cm);
+      double akoh, bkoh;
+      kohler_factors(msol, temp, a.sol, akoh, bkoh);
+      const double warm = a.warm ? a.warm[k] : __longlong_as_double(0x7FF8000000000000ll);
+      SolveOut o{0, false, false, false};
+      const double newr = solve_condensation(a.cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
+      const double rnew = fmax(newr, dryradius(msol, a.sol));  // change_radius (superdrop_attrs.cpp:35-45)
+      a.s.r[k] = rnew;
+      dm = (condensate_mass(rnew, msol, a.sol, &bad_cm) - old_m) * static_cast<double>(xi);
+      if (o.maxed) set_error(a.ctl, CLEO_ERR_NEWTON_MAXITERS, g, k);
+      if (a.niters) a.niters[k] = o.iters;
+      iters += o.iters;
+      subs += o.substep ? 1u : 0u;
+      restarts += o.restarted ? 1u : 0u;
+    }
+    const int m = static_cast<int>(min(static_cast<uint32_t>(kWave), n - c));
+    tot = ordered_add(tot, dm, m);
+  }
+  if (bad_cm) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, g, base);
+
+  // EffectOnThermodynamicStateFunctor::state_change (condensation.cpp:126-137)
+  double tnew = temp;
+  if (a.alter) {
+    const double totrho = tot / volume * dimless::R0cubed_VOL0;
+    const double rho_dry = dry_air_density(press, temp, qv);
+    const double dq = totrho / rho_dry;
+    qv -= dq;
+    qc += dq;
+    const double dtemp = (dimless::Latent_v / moist_specific_heat(qv, qc)) * dq;
+    tnew += dtemp;
+  }
+  iters = wave_sum_u32(iters);
+  subs = wave_sum_u32(subs);
+  restarts = wave_sum_u32(restarts);
+  if (lane == 0) {
+    if (a.alter) {
+      a.st.temp[g] = tnew;
+      a.st.qv[g] = qv;
+    }
+    if (a.alter || a.fuse_qcond) a.st.qc[g] = qc;
+    add_stat(a.ctl, ST_COND_DROPS, n);
+    add_stat(a.ctl, ST_NEWTON_ITERS, iters);
+    add_stat(a.ctl, ST_SUBSTEP_DROPS, subs);
+    add_stat(a.ctl, ST_WARM_RESTARTS, restarts);
+  }
+}
+
+// EffectOnHydrometeorStatesFunctor (movesupersindomain.hpp:95-136), standalone
+__global__ void __launch_bounds__(256) k_qcond(QcondArgs a) {
+  const int lane = lane_id();
+  const uint32_t g = blockIdx.x * a.wpb + (threadIdx.x >> 6);
+  if (g >= a.ngbx) return;
+  const uint32_t base = a.off[g];
+  const uint32_t nfull = a.off[g + 1] - base;
+  bool bad = false;
+  double tot = 0.0;
+  for (uint32_t c = 0; c < nfull; c += kWave) {
+    const uint32_t k = base + c + lane;
+    double v = 0.0;
+    if (c + lane < nfull) v = condensate_mass(a.s.r[k], a.s.msol[k], a.sol, &bad) * static_cast<double>(a.s.xi[k]);
+    tot = ordered_add(tot, v, static_cast<int>(min(static_cast<uint32_t>(kWave), nfull - c)));
+  }
+  if (bad) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, g, base);
+  if (lane == 0) {
+    const double rho_cond = tot / a.st.vol[g] * dimless::R0cubed_VOL0;
+    const double rho_dry = dry_air_density(a.st.press[g], a.st.temp[g], a.st.qv[g]);
+    a.st.qc[g] = rho_cond / rho_dry;
+  }
+}
+
+// ============================ collisions ======================================================
+// DoCollisions::do_collisions for one gridbox per wave (collisions.hpp:140-278):
+// Fisher-Yates on a u32 index permutation in LDS (one lane walks the n-1 swaps, all lanes draw
+// the indices first), pairs (2jj, 2jj+1) one per lane, enactment, then the drops are written to
+// `dst` in shuffled order with nulls moved behind the active drops (stable partition).
+__global__ void __launch_bounds__(256) k_collisions(CollArgs a) {
+  extern __shared__ uint32_t lds_dyn[];
+  const int lane = lane_id();
+  const int w = static_cast<int>(threadIdx.x >> 6);
+  const uint32_t g = blockIdx.x * a.wpb + w;
+  if (g >= a.ngbx) return;
+  const uint32_t base = a.off[g];
+  const uint32_t nfull = a.off[g + 1] - base;
+  const uint32_t n = a.nact[g];
+  const uint64_t event = a.ctl->event;
+
+  uint32_t* perm;
+  uint32_t* jb;
+  if (n <= kCollLdsCap) {
+    perm = lds_dyn + static_cast<uint32_t>(w) * 2u * kCollLdsCap;
+    jb = perm + kCollLdsCap;
+  } else {
+    perm = a.big + 2ull * base;
+    jb = perm + n;
+  }
+  for (uint32_t k = lane; k < n; k += kWave) perm[k] = k;
+  if (n >= 2) {
+    for (uint32_t t = lane; t < n - 1; t += kWave) {
+      const uint32_t iter = n - 1 - t;
+      uint32_t j;
+      if (a.inj_j) {
+        j = a.inj_j[a.inj_oj[g] + t];
+        if (j > iter) {  // an injected index outside [0, iter] is an argument error
+          set_error(a.ctl, CLEO_ERR_INJECTED_INDEX, g, base + iter);
+          j = iter;
+        }
+      } else {
+        uint32_t r4[4];
+        rng_block(a.seed, event, 0u, g, iter, r4);
+        j = static_cast<uint32_t>((static_cast<uint64_t>(r4[0]) * (iter + 1ull)) >> 32);
+      }
+      jb[t] = j;
+    }
+    if (n > kCollLdsCap) __threadfence_block();
+    wave_sync();
+    if (lane == 0) {  // fisher_yates_shuffle (shuffle.hpp:80-87)
+      for (uint32_t t = 0; t + 1 < n; ++t) {
+        const uint32_t iter = n - 1 - t;
+        const uint32_t j = jb[t];
+        const uint32_t pi = perm[iter];
+        perm[iter] = perm[j];
+        perm[j] = pi;
+      }
+    }
+    if (n > kCollLdsCap) __threadfence_block();
+    wave_sync();
+  }
+
+  const uint32_t npairs = n / 2;
+  const double scale_p = n * (n - 1.0) / (2.0 * npairs);
+  const double VOLUME = a.vol[g] * dimless::VOL0;
+  uint32_t nulls = 0, events = 0;
+  for (uint32_t c = 0; c < npairs; c += kWave) {
+    const uint32_t jj = c + lane;
+    bool nullA = false, nullB = false;
+    if (jj < npairs) {
+      const uint32_t ka = base + perm[2 * jj];
+      const uint32_t kb = base + perm[2 * jj + 1];
+      DropAttrs A{a.src.xi[ka], a.src.r[ka], a.src.msol[ka]};
+      DropAttrs B{a.src.xi[kb], a.src.r[kb], a.src.msol[kb]};
+      const bool a_first = !(A.xi < B.xi);  // assign_drops (collisions.hpp:94-101)
+      DropAttrs& d1 = a_first ? A : B;
+      DropAttrs& d2 = a_first ? B : A;
+      const double prob_jk = pair_probability(a.k, d1.r, d2.r, VOLUME);
+      const double prob = scale_p * static_cast<double>(d1.xi) * prob_jk;
+      double phi_coll, phi_out;
+      if (a.inj_phi) {
+        const uint64_t o = a.inj_op[g] + 2ull * jj;
+        phi_coll = a.inj_phi[o];
+        phi_out = a.inj_phi[o + 1];
+      } else {
+        uint32_t r4[4];
+        rng_block(a.seed, event, 1u, g, jj, r4);
+        phi_coll = u53(r4[0], r4[1]);
+        phi_out = u53(r4[2], r4[3]);
+      }
+      int err = 0;
+      const uint32_t res = enact_pair(a.k, d1, d2, prob, phi_coll, phi_out, a.sol, err);
+      if (err) set_error(a.ctl, err, g, base + 2 * jj);
+      events += (res >> 1) & 1u;
+      if (res & 1u) {
+        nullA = a_first;
+        nullB = !a_first;
+      }
+      // write the pair at its shuffled positions (updated attributes, other fields carried)
+      const uint32_t oa = base + 2 * jj, ob = oa + 1;
+      a.dst.gbx[oa] = nullA ? kOOB : a.src.gbx[ka];
+      a.dst.c3[oa] = a.src.c3[ka];
+      a.dst.c1[oa] = a.src.c1[ka];
+      a.dst.c2[oa] = a.src.c2[ka];
+      a.dst.xi[oa] = A.xi;
+      a.dst.r[oa] = A.r;
+      a.dst.msol[oa] = A.msol;
+      a.dst.id[oa] = a.src.id[ka];
+      a.dst.gbx[ob] = nullB ? kOOB : a.src.gbx[kb];
+      a.dst.c3[ob] = a.src.c3[kb];
+      a.dst.c1[ob] = a.src.c1[kb];
+      a.dst.c2[ob] = a.src.c2[kb];
+      a.dst.xi[ob] = B.xi;
+      a.dst.r[ob] = B.r;
+      a.dst.msol[ob] = B.msol;
+      a.dst.id[ob] = a.src.id[kb];
+    }
+    nulls += static_cast<uint32_t>(__popcll(__ballot(nullA || nullB)));
+  }
+  // unpaired last drop and the inactive tail (nulls of earlier sub-steps) are carried unchanged
+  for (uint32_t k = 2 * npairs + lane; k < nfull; k += kWave) {
+    const uint32_t ks = base + (k < n ? perm[k] : k);
+    const uint32_t kd = base + k;
+    a.dst.gbx[kd] = a.src.gbx[ks];
+    a.dst.c3[kd] = a.src.c3[ks];
+    a.dst.c1[kd] = a.src.c1[ks];
+    a.dst.c2[kd] = a.src.c2[ks];
+    a.dst.xi[kd] = a.src.xi[ks];
+    a.dst.r[kd] = a.src.r[ks];
+    a.dst.msol[kd] = a.src.msol[ks];
+    a.dst.id[kd] = a.src.id[ks];
+  }
+  if (nulls > 0) {
+    // remove_null_supers (collisions.hpp:240-248): the reference's sort_team is restated as a
+    // stable partition, through the now unused `src` range of this gridbox.
+    __threadfence();
+    wave_sync();
+    const uint32_t nn = n - nulls;
+    uint32_t run_nn = 0, run_null = 0;
+    for (uint32_t c = 0; c < n; c += kWave) {
+      const uint32_t k = c + lane;
+      const bool act = k < n;
+      const bool isnull = act && a.dst.xi[base + k] == 0;
+      const uint64_t mnull = __ballot(isnull);
+      const uint64_t mnn = __ballot(act && !isnull);
+      if (act) {
+        const uint32_t pos = isnull ? nn + run_null + __popcll(mnull & lanemask_lt())
+                                    : run_nn + __popcll(mnn & lanemask_lt());
+        const uint32_t ks = base + k, kd = base + pos;
+        a.src.gbx[kd] = a.dst.gbx[ks];
+        a.src.c3[kd] = a.dst.c3[ks];
+        a.src.c1[kd] = a.dst.c1[ks];
+        a.src.c2[kd] = a.dst.c2[ks];
+        a.src.xi[kd] = a.dst.xi[ks];
+        a.src.r[kd] = a.dst.r[ks];
+        a.src.msol[kd] = a.dst.msol[ks];
+        a.src.id[kd] = a.dst.id[ks];
+      }
+      run_null += __popcll(mnull);
+      run_nn += __popcll(mnn);
+    }
+    __threadfence();
+    wave_sync();
+    for (uint32_t k = lane; k < n; k += kWave) {
+      const uint32_t kk = base + k;
+      a.dst.gbx[kk] = a.src.gbx[kk];
+      a.dst.c3[kk] = a.src.c3[kk];
+      a.dst.c1[kk] = a.src.c1[kk];
+      a.dst.c2[kk] = a.src.c2[kk];
+      a.dst.xi[kk] = a.src.xi[kk];
+      a.dst.r[kk] = a.src.r[kk];
+      a.dst.msol[kk] = a.src.msol[kk];
+      a.dst.id[kk] = a.src.id[kk];
+    }
+    if (lane == 0) {
+      a.nact[g] = nn;
+      a.ctl->resort_flag = 1u;
+    }
+  }
+  events = wave_sum_u32(events);
+  if (lane == 0) {
+    add_stat(a.ctl, ST_COLL_PAIRS, npairs);
+    add_stat(a.ctl, ST_COLL_EVENTS, events);
+    add_stat(a.ctl, ST_NULLS, nulls);
+  }
+}
+
+// per-gridbox sizes of the injected stream: n-1 indices and 2*floor(n/2) phis
+__global__ void k_inj_sizes(const uint32_t* nact, uint32_t ngbx, uint32_t* nj, uint32_t* np) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngbx) return;
+  const uint32_t n = nact[g];
+  nj[g] = n >= 2 ? n - 1 : 0;
+  np[g] = 2 * (n / 2);
+}
+
+__global__ void k_advance_event(DevCtl* ctl) { ctl->event += 1; }
+
+// ============================ motion ==========================================================
+__device__ __forceinline__ double interpolation(double blo, double bhi, double vlo, double vhi, double sd) {
+  const double coord = fmin(bhi, fmax(blo, sd));  // predcorr.cpp:31-42
+  const double alpha = (coord - blo) / (bhi - blo);
+  return alpha * vhi + (1 - alpha) * vlo;
+}
+
+// One axis of CartesianMaps::get_local_bounding_gridbox_index (cartesianmaps.cpp:68-286,
+// domainboundaries.hpp:34-109): step to the backward/forward neighbour when the coordinate left
+// [lower, upper) and wrap the coordinate at a periodic domain edge.  Returns the step taken.
+__device__ __forceinline__ int axis_step(uint32_t& idx, double& coord, uint32_t i, uint32_t nd, uint32_t inc,
+                                         bool periodic, const double* h) {
+  const double lo = h[i], hi = h[i + 1];
+  if (coord < lo) {
+    if (i == 0) {
+      if (periodic) {
+        const uint32_t nb = idx + (nd - 1) * inc;
+        coord = coord + h[nd] - lo;  // lim1 = upper bound of the backward neighbour
+        idx = nb;
+      } else {
+        idx = kOOB;
+      }
+    } else {
+      idx = idx - inc;
+    }
+    return -1;
+  }
+  if (coord >= hi) {
+    if (i == nd - 1) {
+      if (periodic) {
+        const uint32_t nb = idx - (nd - 1) * inc;
+        coord = coord + h[0] - hi;  // lim1 = lower bound of the forward neighbour
+        idx = nb;
+      } else {
+        idx = kOOB;
+      }
+    } else {
+      idx = idx + inc;
+    }
+    return 1;
+  }
+  return 0;
+}
+
+// MoveSupersInGridboxesFunctor (movesupersindomain.hpp:47-93) with PredCorrMotion over
+// CartesianMaps; also counts, per gridbox, how many drops go to each of the 27 neighbour
+// directions (and out of the domain) for the stable neighbour sort.
+__global__ void __launch_bounds__(256) k_motion(MotionArgs a) {
+  const int lane = lane_id();
+  const uint32_t g = blockIdx.x * a.wpb + (threadIdx.x >> 6);
+  if (g >= a.grid.ngbx) return;
+  const uint32_t base = a.off[g];
+  const uint32_t nfull = a.off[g + 1] - base;
+  const uint32_t nz = a.grid.nz, nx = a.grid.nx, ny = a.grid.ny;
+  const uint32_t iz = g % nz, ix = (g / nz) % nx, iy = g / (nz * nx);
+  const double zlo = a.grid.zh[iz], zhi = a.grid.zh[iz + 1];
+  const double xlo = a.grid.xh[ix], xhi = a.grid.xh[ix + 1];
+  const double ylo = a.grid.yh[iy], yhi = a.grid.yh[iy + 1];
+  const double wlo = a.st.wlo[g], whi = a.st.whi[g];
+  const double ulo = a.st.ulo[g], uhi = a.st.uhi[g];
+  const double vlo = a.st.vlo[g], vhi = a.st.vhi[g];
+  const double delt = a.delt;
+  uint32_t mycount = 0;
+  for (uint32_t c = 0; c < nfull; c += kWave) {
+    const uint32_t k = base + c + lane;
+    const bool act = c + lane < nfull;
+    int dir = 0;
+    if (act) {
+      // PredCorr::delta_coord3/1/2 (predcorr.hpp:84-145)
+      double coord3 = a.s.c3[k];
+      const double terminal = terminal_velocity(a.tv, a.s.r[k]);
+      double vel3 = interpolation(zlo, zhi, wlo, whi, coord3);
+      vel3 -= terminal;
+      coord3 += vel3 * delt;
+      double corrvel3 = interpolation(zlo, zhi, wlo, whi, coord3);
+      corrvel3 -= terminal;
+      const double delta3 = (vel3 + corrvel3) * (delt / 2);
+      double coord1 = a.s.c1[k];
+      const double vel1 = interpolation(xlo, xhi, ulo, uhi, coord1);
+      coord1 += vel1 * delt;
+      const double corrvel1 = interpolation(xlo, xhi, ulo, uhi, coord1);
+      const double delta1 = (vel1 + corrvel1) * (delt / 2);
+      double coord2 = a.s.c2[k];
+      const double vel2 = interpolation(ylo, yhi, vlo, vhi, coord2);
+      coord2 += vel2 * delt;
+      const double corrvel2 = interpolation(ylo, yhi, vlo, vhi, coord2);
+      const double delta2 = (vel2 + corrvel2) * (delt / 2);
+      // cfl_criteria (cfl_criteria.hpp:40-57)
+      bool cfl = (fabs(delta3) <= fabs(zhi - zlo));
+      cfl = (fabs(delta1) <= fabs(xhi - xlo)) && cfl;
+      cfl = (fabs(delta2) <= fabs(yhi - ylo)) && cfl;
+      if (!cfl) set_error(a.ctl, CLEO_ERR_CFL, g, k);
+      double c3 = a.s.c3[k] + delta3;
+      double c1 = a.s.c1[k] + delta1;
+      double c2 = a.s.c2[k] + delta2;
+      // superdrop_gbx (predcorrmotion.hpp:85-106): z, then x, then y
+      uint32_t idx = g;
+      const int dz = axis_step(idx, c3, iz, nz, 1u, a.grid.per[0] != 0, a.grid.zh);
+      int dx = 0, dy = 0;
+      if (idx != kOOB) dx = axis_step(idx, c1, ix, nx, nz, a.grid.per[1] != 0, a.grid.xh);
+      if (idx != kOOB) dy = axis_step(idx, c2, iy, ny, nz * nx, a.grid.per[2] != 0, a.grid.yh);
+      a.s.c3[k] = c3;
+      a.s.c1[k] = c1;
+      a.s.c2[k] = c2;
+      a.s.gbx[k] = idx;
+      if (idx != kOOB) {
+        // CartesianCheckBounds (cartesian_motion.hpp:38-52)
+        const uint32_t jz = idx % nz, jx = (idx / nz) % nx, jy = idx / (nz * nx);
+        const bool bad = (c3 < a.grid.zh[jz]) | (c3 >= a.grid.zh[jz + 1]) | (c1 < a.grid.xh[jx]) |
+                         (c1 >= a.grid.xh[jx + 1]) | (c2 < a.grid.yh[jy]) | (c2 >= a.grid.yh[jy + 1]);
+        if (bad) set_error(a.ctl, CLEO_ERR_BOUNDS, g, k);
+        dir = (dz + 1) + 3 * (dx + 1) + 9 * (dy + 1);
+      } else {
+        dir = 27;
+      }
+    }
+    uint64_t mask = __ballot(act);
+    while (mask) {
+      const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+      const int v = static_cast<int>(readlane_u32(static_cast<uint32_t>(dir), leader));
+      const uint64_t m = __ballot(act && dir == v);
+      if (lane == v) mycount += static_cast<uint32_t>(__popcll(m));
+      mask &= ~m;
+    }
+  }
+  if (lane < 28) a.cnt[28ull * g + lane] = mycount;
+  if (lane == 0) add_stat(a.ctl, ST_MOTION_DROPS, nfull);
+}
+
+// ============================ neighbour sort ==================================================
+// Source gridbox that reaches `d` by the neighbour step `s` (s = (dz+1) + 3(dx+1) + 9(dy+1)),
+// or kOOB when the step would come from outside a finite axis.
+__device__ __forceinline__ uint32_t source_of(const Grid& gr, uint32_t d, int s) {
+  const int dz = s % 3 - 1, dx = (s / 3) % 3 - 1, dy = s / 9 - 1;
+  int jz = static_cast<int>(d % gr.nz) - dz;
+  int jx = static_cast<int>((d / gr.nz) % gr.nx) - dx;
+  int jy = static_cast<int>(d / (gr.nz * gr.nx)) - dy;
+  const int nz = static_cast<int>(gr.nz), nx = static_cast<int>(gr.nx), ny = static_cast<int>(gr.ny);
+  if (jz < 0 || jz >= nz) {
+    if (!gr.per[0]) return kOOB;
+    jz = (jz + nz) % nz;
+  }
+  if (jx < 0 || jx >= nx) {
+    if (!gr.per[1]) return kOOB;
+    jx = (jx + nx) % nx;
+  }
+  if (jy < 0 || jy >= ny) {
+    if (!gr.per[2]) return kOOB;
+    jy = (jy + ny) % ny;
+  }
+  return static_cast<uint32_t>(jz + nz * (jx + nx * jy));
+}
+
+__device__ __forceinline__ bool guard_off(const uint32_t* flag) { return flag && *flag == 0u; }
+
+// drops arriving in each destination gridbox (thread per destination)
+__global__ void k_nsort_tot(NsortArgs a) {
+  if (guard_off(a.flag)) return;
+  const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= a.grid.ngbx) return;
+  uint32_t t = 0;
+  for (int s = 0; s < 27; ++s) {
+    const uint32_t src = source_of(a.grid, d, s);
+    if (src != kOOB) t += a.cnt[28ull * src + s];
+  }
+  a.tot[d] = t;
+}
+
+// Stable scatter (sortsupers.hpp:63-76,167-193 under Serial): drop p of source gridbox g with
+// key d lands at  base[d] + (drops reaching d from gridboxes < g) + (earlier drops of g with key d).
+__global__ void __launch_bounds__(256) k_nsort_scatter(NsortArgs a) {
+  if (guard_off(a.flag)) return;
+  const int lane = lane_id();
+  const uint32_t g = blockIdx.x * a.wpb + (threadIdx.x >> 6);
+  if (g >= a.grid.ngbx) return;
+  const uint32_t sbase = a.off[g];
+  const uint32_t nfull = a.off[g + 1] - sbase;
+  const uint32_t ngbx = a.grid.ngbx;
+  const uint32_t gz = g % a.grid.nz, gx = (g / a.grid.nz) % a.grid.nx, gy = g / (a.grid.nz * a.grid.nx);
+  uint32_t tkey = 0xFFFFFFFEu, trun = 0;  // lane i < 28 holds table slot i
+  int ntab = 0;
+  for (uint32_t c = 0; c < nfull; c += kWave) {
+    const uint32_t p = sbase + c + lane;
+    const bool act = c + lane < nfull;
+    uint32_t key = 0;
+    if (act) {
+      key = a.src.gbx[p];
+      if (key >= ngbx) key = kOOB;
+    }
+    uint32_t dest = 0;
+    uint64_t mask = __ballot(act);
+    while (mask) {
+      const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+      const uint32_t v = readlane_u32(key, leader);
+      const uint64_t m = __ballot(act && key == v);
+      const uint64_t hit = __ballot(lane < ntab && tkey == v);
+      uint32_t start;
+      int slot;
+      if (hit) {
+        slot = __ffsll(static_cast<unsigned long long>(hit)) - 1;
+        start = readlane_u32(trun, slot);
+      } else {
+        slot = ntab++;
+        if (v == kOOB) {
+          start = a.base[ngbx] + a.oobpre[g];
+        } else {
+          // v must be a neighbour of g
+          const int vz = static_cast<int>(v % a.grid.nz), vx = static_cast<int>((v / a.grid.nz) % a.grid.nx),
+                    vy = static_cast<int>(v / (a.grid.nz * a.grid.nx));
+          auto near = [](int q, int r, int nd, bool per) {
+            const int dd = q - r;
+            if (dd >= -1 && dd <= 1) return true;
+            return per && (dd == nd - 1 || dd == 1 - nd);
+          };
+          if (!(near(vz, static_cast<int>(gz), static_cast<int>(a.grid.nz), a.grid.per[0]) &&
+                near(vx, static_cast<int>(gx), static_cast<int>(a.grid.nx), a.grid.per[1]) &&
+                near(vy, static_cast<int>(gy), static_cast<int>(a.grid.ny), a.grid.per[2]))) {
+            if (lane == 0) set_error(a.ctl, CLEO_ERR_SORT_NONLOCAL, g, sbase + c + leader);
+          }
+          uint32_t part = 0;
+          if (lane < 27) {
+            const uint32_t src = source_of(a.grid, v, lane);
+            if (src != kOOB && src < g) part = a.cnt[28ull * src + lane];
+          }
+          start = a.base[v] + wave_sum_u32(part);
+        }
+        if (lane == slot) tkey = v;
+      }
+      if ((m >> lane) & 1ull) dest = start + static_cast<uint32_t>(__popcll(m & lanemask_lt()));
+      if (lane == slot) trun = start + static_cast<uint32_t>(__popcll(m));
+      mask &= ~m;
+    }
+    if (act && dest >= a.ntot) {  // counts inconsistent with keys: never write out of range
+      set_error(a.ctl, CLEO_ERR_SORT_NONLOCAL, g, p);
+    } else if (act) {
+      a.dst.gbx[dest] = a.src.gbx[p];
+      a.dst.c3[dest] = a.src.c3[p];
+      a.dst.c1[dest] = a.src.c1[p];
+      a.dst.c2[dest] = a.src.c2[p];
+      a.dst.xi[dest] = a.src.xi[p];
+      a.dst.r[dest] = a.src.r[p];
+      a.dst.msol[dest] = a.src.msol[p];
+      a.dst.id[dest] = a.src.id[p];
+    }
+  }
+}
+
+// drops already out of the domain stay behind the newly out-of-domain ones, in order
+__global__ void k_nsort_oob(NsortArgs a, uint64_t ntot) {
+  if (guard_off(a.flag)) return;
+  const uint64_t old_end = a.off[a.grid.ngbx];
+  const uint64_t dst0 = static_cast<uint64_t>(a.base[a.grid.ngbx]) + a.oobpre[a.grid.ngbx];
+  for (uint64_t p = old_end + blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; p < ntot;
+       p += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t d = dst0 + (p - old_end);
+    if (d >= ntot) {
+      set_error(a.ctl, CLEO_ERR_SORT_NONLOCAL, kOOB, p);
+      continue;
+    }
+    a.dst.gbx[d] = a.src.gbx[p];
+    a.dst.c3[d] = a.src.c3[p];
+    a.dst.c1[d] = a.src.c1[p];
+    a.dst.c2[d] = a.src.c2[p];
+    a.dst.xi[d] = a.src.xi[p];
+    a.dst.r[d] = a.src.r[p];
+    a.dst.msol[d] = a.src.msol[p];
+    a.dst.id[d] = a.src.id[p];
+  }
+}
+
+// counts of the re-sort after collisions: active drops stay, nulls leave the domain
+__global__ void k_cnt_from_nact(const uint32_t* off, const uint32_t* nact, uint32_t ngbx, uint32_t* cnt,
+                                const uint32_t* flag) {
+  if (guard_off(flag)) return;
+  const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (i >= 28ull * ngbx) return;
+  const uint32_t g = static_cast<uint32_t>(i / 28), s = static_cast<uint32_t>(i % 28);
+  const uint32_t n = nact[g], nfull = off[g + 1] - off[g];
+  cnt[i] = s == 13 ? n : (s == 27 ? nfull - n : 0u);
+}
+
+__global__ void k_set_nact(const uint32_t* off, uint32_t ngbx, uint32_t* nact, const uint32_t* flag) {
+  if (guard_off(flag)) return;
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < ngbx) nact[g] = off[g + 1] - off[g];
+}
+
+__global__ void k_copy_u32(const uint32_t* src, uint32_t* dst, uint64_t n, const uint32_t* flag) {
+  if (guard_off(flag)) return;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
+__global__ void k_copy_soa(SoA src, SoA dst, uint64_t lo, uint64_t hi, const uint32_t* flag) {
+  if (guard_off(flag)) return;
+  for (uint64_t p = lo + blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; p < hi;
+       p += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    dst.gbx[p] = src.gbx[p];
+    dst.c3[p] = src.c3[p];
+    dst.c1[p] = src.c1[p];
+    dst.c2[p] = src.c2[p];
+    dst.xi[p] = src.xi[p];
+    dst.r[p] = src.r[p];
+    dst.msol[p] = src.msol[p];
+    dst.id[p] = src.id[p];
+  }
+}
+
+// copies [off_old[ngbx], ntot) from src to dst (the out-of-domain region after collisions)
+__global__ void k_copy_oob_region(SoA src, SoA dst, const uint32_t* off, uint32_t ngbx, uint64_t ntot) {
+  const uint64_t lo = off[ngbx];
+  for (uint64_t p = lo + blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; p < ntot;
+       p += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    dst.gbx[p] = src.gbx[p];
+    dst.c3[p] = src.c3[p];
+    dst.c1[p] = src.c1[p];
+    dst.c2[p] = src.c2[p];
+    dst.xi[p] = src.xi[p];
+    dst.r[p] = src.r[p];
+    dst.msol[p] = src.msol[p];
+    dst.id[p] = src.id[p];
+  }
+}
+
+__global__ void k_clear_flag(DevCtl* ctl) { ctl->resort_flag = 0u; }
+__global__ void k_sorts_stat(DevCtl* ctl, const uint32_t* flag) {
+  if (guard_off(flag)) return;
+  ctl->stats[ST_SORTS] += 1;
+}
+
+// ============================ generic stable radix sort =======================================
+__global__ void k_rs_init(const uint32_t* gbx, uint64_t n, uint32_t ngbx, uint32_t* key, uint32_t* idx) {
+  for (uint64_t p = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; p < n;
+       p += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t k = gbx[p];
+    key[p] = k < ngbx ? k : ngbx;  // _get_count_position (sortsupers.hpp:44-48)
+    idx[p] = static_cast<uint32_t>(p);
+  }
+}
+
+// per-tile (256 keys) histogram of one 4-bit digit: cnt[digit * ntiles + tile]
+__global__ void __launch_bounds__(256) k_rs_count(const uint32_t* key, uint64_t n, int shift, uint32_t ntiles,
+                                                  uint32_t* cnt) {
+  __shared__ uint32_t wc[4][16];
+  const uint32_t t = blockIdx.x;
+  const uint64_t p = static_cast<uint64_t>(t) * 256 + threadIdx.x;
+  const bool act = p < n;
+  const uint32_t dg = act ? (key[p] >> shift) & 15u : 0u;
+  const int w = threadIdx.x >> 6, lane = lane_id();
+  for (uint32_t v = 0; v < 16; ++v) {
+    const uint64_t m = __ballot(act && dg == v);
+    if (lane == 0) wc[w][v] = static_cast<uint32_t>(__popcll(m));
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) cnt[threadIdx.x * ntiles + t] = wc[0][threadIdx.x] + wc[1][threadIdx.x] + wc[2][threadIdx.x] + wc[3][threadIdx.x];
+}
+
+__global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* key, const uint32_t* idx, uint64_t n, int shift,
+                                                    uint32_t ntiles, const uint32_t* offs, uint32_t* key2,
+                                                    uint32_t* idx2) {
+  __shared__ uint32_t wc[4][16];
+  const uint32_t t = blockIdx.x;
+  const uint64_t p = static_cast<uint64_t>(t) * 256 + threadIdx.x;
+  const bool act = p < n;
+  const uint32_t k = act ? key[p] : 0u;
+  const uint32_t dg = (k >> shift) & 15u;
+  const int w = threadIdx.x >> 6, lane = lane_id();
+  uint32_t wrank = 0;
+  for (uint32_t v = 0; v < 16; ++v) {
+    const uint64_t m = __ballot(act && dg == v);
+    if (act && dg == v) wrank = static_cast<uint32_t>(__popcll(m & lanemask_lt()));
+    if (lane == 0) wc[w][v] = static_cast<uint32_t>(__popcll(m));
+  }
+  __syncthreads();
+  if (act) {
+    uint32_t pre = 0;
+    for (int ww = 0; ww < w; ++ww) pre += wc[ww][dg];
+    const uint32_t dst = offs[dg * ntiles + t] + pre + wrank;
+    if (dst >= n) return;
+    key2[dst] = k;
+    idx2[dst] = idx[p];
+  }
+}
+
+__global__ void k_gather_soa(SoA src, SoA dst, const uint32_t* idx, uint64_t n) {
+  for (uint64_t p = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; p < n;
+       p += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t s = idx[p];
+    dst.gbx[p] = src.gbx[s];
+    dst.c3[p] = src.c3[s];
+    dst.c1[p] = src.c1[s];
+    dst.c2[p] = src.c2[s];
+    dst.xi[p] = src.xi[s];
+    dst.r[p] = src.r[s];
+    dst.msol[p] = src.msol[s];
+    dst.id[p] = src.id[s];
+  }
+}
+
+// off[b] = first position with key >= b, b = 0..ngbx (find_refs / find_domainrefs, findrefs.hpp)
+__global__ void k_bucket_bounds(const uint32_t* key, uint64_t n, uint32_t ngbx, uint32_t* off) {
+  for (uint64_t p = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; p <= n;
+       p += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const int64_t prev = p == 0 ? -1 : static_cast<int64_t>(key[p - 1]);
+    const int64_t cur = p == n ? static_cast<int64_t>(ngbx) + 1 : static_cast<int64_t>(key[p]);
+    for (int64_t b = prev + 1; b <= cur && b <= static_cast<int64_t>(ngbx); ++b) off[b] = static_cast<uint32_t>(p);
+  }
+}
+
+// ============================ scan (exclusive, u32) ===========================================
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ uint32_t load_in(const uint32_t* in, uint64_t i, uint32_t stride, uint32_t offset,
+                                            uint64_t len) {
+  return i < len ? in[i * stride + offset] : 0u;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* in, uint64_t len, uint32_t stride,
+                                                              uint32_t offset, uint32_t* partial,
+                                                              const uint32_t* flag) {
+  if (guard_off(flag)) return;
+  __shared__ uint32_t ws[4];
+  const uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * kScanTile;
+  uint32_t s = 0;
+  for (int i = 0; i < kScanItems; ++i) s += load_in(in, t0 + i * kScanThreads + threadIdx.x, stride, offset, len);
+  s = wave_sum_u32(s);
+  if (lane_id() == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_apply(const uint32_t* in, uint64_t len, uint32_t stride,
+                                                             uint32_t offset, const uint32_t* block_off,
+                                                             uint32_t* out, const uint32_t* flag) {
+  if (guard_off(flag)) return;
+  __shared__ uint32_t ws[4];
+  const uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * kScanTile;
+  // each thread owns kScanItems consecutive items
+  uint32_t v[kScanItems];
+  uint32_t s = 0;
+  for (int i = 0; i < kScanItems; ++i) {
+    v[i] = load_in(in, t0 + static_cast<uint64_t>(threadIdx.x) * kScanItems + i, stride, offset, len);
+    s += v[i];
+  }
+  // inclusive wave scan of per-thread sums
+  const int lane = lane_id();
+  uint32_t incl = s;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) ws[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  uint32_t pre = block_off ? block_off[blockIdx.x] : 0u;
+  for (int ww = 0; ww < static_cast<int>(threadIdx.x >> 6); ++ww) pre += ws[ww];
+  uint32_t run = pre + incl - s;
+  for (int i = 0; i < kScanItems; ++i) {
+    const uint64_t idx = t0 + static_cast<uint64_t>(threadIdx.x) * kScanItems + i;
+    if (idx < len) out[idx] = run;
+    run += v[i];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) out[len] = run;
+}
+
+}  // namespace cleo
+
+// ============================ host-side launchers =============================================
+namespace cleo {
+
+static inline unsigned nblocks(uint64_t n, uint64_t per) {
+  const uint64_t b = (n + per - 1) / per;
+  return static_cast<unsigned>(b == 0 ? 1 : b);
+}
+static inline unsigned stride_blocks(uint64_t n) {
+  const uint64_t b = (n + 255) / 256;
+  return static_cast<unsigned>(b == 0 ? 1 : (b > 65536 ? 65536 : b));
+}
+
+__global__ void k_set_u32(uint32_t* out, uint32_t v, const uint32_t* flag) {
+  if (guard_off(flag)) return;
+  *out = v;
+}
+
+void launch_condensation(cleo_ctx* c, const CondArgs& a) {
+  if (a.ngbx == 0) return;
+  hipLaunchKernelGGL(k_condensation, dim3(nblocks(a.ngbx, a.wpb)), dim3(64 * a.wpb), 0, c->stream, a);
+}
+void launch_qcond(cleo_ctx* c, const QcondArgs& a) {
+  if (a.ngbx == 0) return;
+  hipLaunchKernelGGL(k_qcond, dim3(nblocks(a.ngbx, a.wpb)), dim3(64 * a.wpb), 0, c->stream, a);
+}
+void launch_collisions(cleo_ctx* c, const CollArgs& a) {
+  if (a.ngbx == 0) return;
+  const size_t lds = static_cast<size_t>(a.wpb) * 2u * kCollLdsCap * sizeof(uint32_t);
+  hipLaunchKernelGGL(k_collisions, dim3(nblocks(a.ngbx, a.wpb)), dim3(64 * a.wpb), lds, c->stream, a);
+}
+void launch_motion(cleo_ctx* c, const MotionArgs& a) {
+  if (a.grid.ngbx == 0) return;
+  hipLaunchKernelGGL(k_motion, dim3(nblocks(a.grid.ngbx, a.wpb)), dim3(64 * a.wpb), 0, c->stream, a);
+}
+void launch_advance_event(cleo_ctx* c) { hipLaunchKernelGGL(k_advance_event, dim3(1), dim3(1), 0, c->stream, c->ctl); }
+void launch_inj_sizes(cleo_ctx* c, uint32_t* nj, uint32_t* np) {
+  hipLaunchKernelGGL(k_inj_sizes, dim3(nblocks(c->grid.ngbx, 256)), dim3(256), 0, c->stream, c->nact, c->grid.ngbx, nj, np);
+}
+void launch_copy_oob_region(cleo_ctx* c, int src, int dst) {
+  hipLaunchKernelGGL(k_copy_oob_region, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, c->buf[src],
+                     c->buf[dst], c->off, c->grid.ngbx, c->ntot);
+}
+
+static size_t scan_words_rec(uint64_t len) {
+  const uint64_t nb = (len + kScanTile - 1) / kScanTile;
+  if (nb <= 1) return 0;
+  return nb + (nb + 1) + scan_words_rec(nb);
+}
+size_t scan_tmp_words(uint64_t len) { return scan_words_rec(len) + 16; }
+
+static void scan_rec(cleo_ctx* c, const uint32_t* in, uint64_t len, uint32_t stride, uint32_t offset, uint32_t* out,
+                     const uint32_t* flag, uint32_t* tmp) {
+  if (len == 0) {
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, c->stream, out, 0u, flag);
+    return;
+  }
+  const uint64_t nb = (len + kScanTile - 1) / kScanTile;
+  if (nb == 1) {
+    hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(kScanThreads), 0, c->stream, in, len, stride, offset,
+                       static_cast<const uint32_t*>(nullptr), out, flag);
+    return;
+  }
+  uint32_t* partial = tmp;
+  uint32_t* pscan = tmp + nb;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, c->stream, in, len, stride,
+                     offset, partial, flag);
+  scan_rec(c, partial, nb, 1u, 0u, pscan, flag, tmp + nb + nb + 1);
+  hipLaunchKernelGGL(k_scan_apply, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, c->stream, in, len, stride,
+                     offset, static_cast<const uint32_t*>(pscan), out, flag);
+}
+void scan_u32(cleo_ctx* c, const uint32_t* in, uint64_t len, uint32_t stride, uint32_t offset, uint32_t* out,
+              const uint32_t* flag) {
+  scan_rec(c, in, len, stride, offset, out, flag, c->scan_tmp);
+}
+
+void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool from_nact) {
+  const uint32_t ngbx = c->grid.ngbx;
+  NsortArgs a{};
+  a.src = c->buf[src];
+  a.dst = c->buf[dst];
+  a.off = c->off;
+  a.base = c->off_alt;
+  a.cnt = c->cnt;
+  a.oobpre = c->oobpre;
+  a.tot = c->tot;
+  a.grid = c->grid;
+  a.wpb = c->tiles.motion_waves_per_block ? c->tiles.motion_waves_per_block : 1;
+  a.ctl = c->ctl;
+  a.flag = flag;
+  a.ntot = c->ntot;
+  if (from_nact)
+    hipLaunchKernelGGL(k_cnt_from_nact, dim3(nblocks(28ull * ngbx, 256)), dim3(256), 0, c->stream, c->off, c->nact, ngbx,
+                       c->cnt, flag);
+  hipLaunchKernelGGL(k_nsort_tot, dim3(nblocks(ngbx, 256)), dim3(256), 0, c->stream, a);
+  scan_u32(c, c->tot, ngbx, 1u, 0u, c->off_alt, flag);
+  scan_u32(c, c->cnt, ngbx, 28u, 27u, c->oobpre, flag);
+  hipLaunchKernelGGL(k_nsort_scatter, dim3(nblocks(ngbx, a.wpb)), dim3(64 * a.wpb), 0, c->stream, a);
+  hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, a, c->ntot);
+  hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, flag);
+  if (flag == nullptr) {
+    c->cur = dst;
+    hipLaunchKernelGGL(k_copy_u32, dim3(stride_blocks(ngbx + 1ull)), dim3(256), 0, c->stream, c->off_alt, c->off,
+                       ngbx + 1ull, flag);
+    hipLaunchKernelGGL(k_set_nact, dim3(nblocks(ngbx, 256)), dim3(256), 0, c->stream, c->off, ngbx, c->nact, flag);
+  } else {
+    hipLaunchKernelGGL(k_copy_soa, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, c->buf[dst], c->buf[src],
+                       0ull, c->ntot, flag);
+    hipLaunchKernelGGL(k_copy_u32, dim3(stride_blocks(ngbx + 1ull)), dim3(256), 0, c->stream, c->off_alt, c->off,
+                       ngbx + 1ull, flag);
+    hipLaunchKernelGGL(k_set_nact, dim3(nblocks(ngbx, 256)), dim3(256), 0, c->stream, c->off, ngbx, c->nact, flag);
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c->stream, c->ctl);
+  }
+}
+
+void generic_sort(cleo_ctx* c) {
+  const uint64_t n = c->ntot;
+  const uint32_t ngbx = c->grid.ngbx;
+  const int src = c->cur, dst = 1 - c->cur;
+  const uint32_t ntiles = static_cast<uint32_t>((n + 255) / 256);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_rs_init, dim3(stride_blocks(n)), dim3(256), 0, c->stream, c->buf[src].gbx, n, ngbx,
+                       c->rs_key[0], c->rs_idx[0]);
+    int passes = 1;
+    while (passes < 8 && (static_cast<uint64_t>(ngbx) >> (4 * passes)) != 0) ++passes;
+    int cur = 0;
+    uint32_t* counts = c->rs_cnt;
+    uint32_t* offs = c->rs_cnt + 16ull * ntiles;
+    for (int p = 0; p < passes; ++p) {
+      const int shift = 4 * p;
+      hipLaunchKernelGGL(k_rs_count, dim3(ntiles), dim3(256), 0, c->stream, c->rs_key[cur], n, shift, ntiles, counts);
+      scan_u32(c, counts, 16ull * ntiles, 1u, 0u, offs, nullptr);
+      hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(256), 0, c->stream, c->rs_key[cur], c->rs_idx[cur], n, shift,
+                         ntiles, static_cast<const uint32_t*>(offs), c->rs_key[1 - cur], c->rs_idx[1 - cur]);
+      cur = 1 - cur;
+    }
+    hipLaunchKernelGGL(k_gather_soa, dim3(stride_blocks(n)), dim3(256), 0, c->stream, c->buf[src], c->buf[dst],
+                       static_cast<const uint32_t*>(c->rs_idx[cur]), n);
+    hipLaunchKernelGGL(k_bucket_bounds, dim3(stride_blocks(n + 1)), dim3(256), 0, c->stream,
+                       static_cast<const uint32_t*>(c->rs_key[cur]), n, ngbx, c->off);
+    c->cur = dst;
+  } else {
+    hipLaunchKernelGGL(k_bucket_bounds, dim3(1), dim3(256), 0, c->stream, static_cast<const uint32_t*>(c->rs_key[0]),
+                       0ull, ngbx, c->off);
+  }
+  hipLaunchKernelGGL(k_set_nact, dim3(nblocks(ngbx, 256)), dim3(256), 0, c->stream, c->off, ngbx, c->nact,
+                     static_cast<const uint32_t*>(nullptr));
+  hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, static_cast<const uint32_t*>(nullptr));
+}
+
+}  // namespace cleo

@@ -1,0 +1,99 @@
+// sdm_kernels.hpp -- kernel argument blocks and host-side launchers of sdm_kernels.hip.
+#pragma once
+#include "ctx.hpp"
+
+namespace cleo {
+
+constexpr int kCollMaxWpb = 4;
+constexpr uint32_t kCollLdsCap = 512;  // drops per gridbox held in LDS; larger gridboxes use HBM scratch
+
+struct CondArgs {
+  SoA s;
+  const uint32_t* off;
+  const uint32_t* nact;
+  GState st;
+  Solute sol;
+  NewtonCfg cfg;
+  uint32_t ngbx;
+  uint32_t wpb;
+  int32_t alter;
+  int32_t fuse_qcond;
+  DevCtl* ctl;
+  uint32_t* niters;
+  const double* warm;
+  uint32_t warm_limit;
+};
+
+struct QcondArgs {
+  SoA s;
+  const uint32_t* off;
+  GState st;
+  Solute sol;
+  uint32_t ngbx;
+  uint32_t wpb;
+  DevCtl* ctl;
+};
+
+struct CollArgs {
+  SoA src, dst;
+  const uint32_t* off;
+  uint32_t* nact;
+  const double* vol;
+  Solute sol;
+  CollCfg k;
+  uint32_t ngbx;
+  uint32_t wpb;
+  DevCtl* ctl;
+  uint64_t seed;
+  const uint32_t* inj_j;
+  const double* inj_phi;
+  const uint32_t* inj_oj;
+  const uint32_t* inj_op;
+  uint32_t* big;
+};
+
+struct MotionArgs {
+  SoA s;
+  const uint32_t* off;
+  Grid grid;
+  GState st;
+  double delt;
+  int32_t tv;
+  uint32_t wpb;
+  uint32_t* cnt;
+  DevCtl* ctl;
+};
+
+struct NsortArgs {
+  SoA src, dst;
+  const uint32_t* off;     // current refs (ngbx+1)
+  const uint32_t* base;    // new refs (ngbx+1)
+  const uint32_t* cnt;     // ngbx*28
+  const uint32_t* oobpre;  // ngbx+1
+  uint32_t* tot;
+  Grid grid;
+  uint32_t wpb;
+  DevCtl* ctl;
+  const uint32_t* flag;    // non-null: skip unless *flag
+  uint64_t ntot;
+};
+
+// launchers (sdm_kernels.hip); all enqueue on ctx->stream
+void launch_condensation(cleo_ctx* c, const CondArgs& a);
+void launch_qcond(cleo_ctx* c, const QcondArgs& a);
+void launch_collisions(cleo_ctx* c, const CollArgs& a);
+void launch_motion(cleo_ctx* c, const MotionArgs& a);
+void launch_advance_event(cleo_ctx* c);
+void launch_inj_sizes(cleo_ctx* c, uint32_t* nj, uint32_t* np);
+// exclusive scan of in[i*stride+offset], i < len, into out[0..len] (out[len] = total)
+void scan_u32(cleo_ctx* c, const uint32_t* in, uint64_t len, uint32_t stride, uint32_t offset, uint32_t* out,
+              const uint32_t* flag);
+size_t scan_tmp_words(uint64_t len);
+// stable neighbour sort of buf[src] into buf[dst] after motion (flag == nullptr) or after
+// collisions (flag = &ctl->resort_flag, data ends back in buf[src] and refs in c->off)
+void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool from_nact);
+// generic stable sort of buf[cur] into buf[1-cur]; swaps cur; sets off and nact
+void generic_sort(cleo_ctx* c);
+void launch_copy_oob_region(cleo_ctx* c, int src, int dst);
+
+}  // namespace cleo
