@@ -1,0 +1,267 @@
+"""Benchmark of the full coupled super-droplet timestep on one MI355X.
+
+Workload (the metric's configuration): 128 x 128 x 128 = 2,097,152 gridboxes of 25 m cubes,
+Poisson(64) superdroplets per gridbox (~134.2M drops, SoA FP64/u64), each step = predictor-
+corrector motion under the 2-D divergence-free updraft (peak w 2 m/s) + stable sort by
+sdgbxindex + implicit-Euler condensation (FP64 Newton, rtol 1e-6, atol 1e-8, maxniters 50) +
+Long-kernel collision-coalescence/breakup (nfrags 5, coaleff 0.85), dt = 1 s, replayed as a
+hipGraph.  Inputs are synthetic and already resident in HBM when the timed region starts.
+
+Prints ONE JSON line (rank 0): superdroplet-steps/s, roofline of the dominant kernel measured
+with HIP events on the library's stream, and the CPU baseline (the OpenMP build of the
+reference-structured oracle) timed on this host on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "superdroplet-steps/sec on 1 GPU (plain vs MLP warm-start), Newton iters/drop"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+# algorithmic HBM bytes (per drop, per gridbox) of each kernel's launch
+BYTES = {
+    # read r, msol, xi; write r | read State (press, temp, qv, qc, vol), refs; write temp, qv, qc
+    "k_condensation": (32, 5 * 8 + 8 + 3 * 8),
+    # read 60-byte record; write 60-byte record in shuffled order | refs, nact, volume
+    "k_collisions": (120, 8 + 4 + 8),
+    # read coords, radius, sdgbxindex; write coords, sdgbxindex | bounds, winds, counts
+    "k_motion": (64, 6 * 8 + 6 * 8 + 28 * 4),
+    # read key + 60-byte record, write 60-byte record | counts, refs
+    "k_nsort_scatter": (124, 28 * 4 + 8 + 8),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nz", type=int, default=128, help="gridboxes along z (128 = full workload)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the learned warm-start leg")
+    ap.add_argument("--cpu-ny", type=int, default=16, help="CPU sample: 128 x 128 x ny gridboxes")
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    return ap.parse_args()
+
+
+def _ptr(t, ctype):
+    return C.cast(C.c_void_p(t.data_ptr()), C.POINTER(ctype))
+
+
+def make_params(abi):
+    cond = abi.CondParams(100, 1, 50, 1e-6, 1e-8, 0.001, 0, 10)
+    coll = abi.CollParams(100, abi.PROB_LONG, 1.0, abi.ENACT_COALBURE, 5.0, 0.85)
+    mot = abi.MotionParams(100, abi.TV_SIMMEL)
+    s = abi.SDMParams()
+    s.enable_cond = s.enable_colls = s.enable_motion = 1
+    s.cond, s.coll, s.motion = cond, coll, mot
+    return s
+
+
+def run_gpu(args):
+    import torch
+
+    from cleo_amd import abi, lib
+    from cleo_amd import populations as P
+
+    b = lib.load()
+    L = b.lib
+    dev = torch.device("cuda:0")
+    ndims = (args.nz, 128, 128)
+    t_gen = time.perf_counter()
+    arrs, st, halves = P.coupled_device(torch, dev, ndims=ndims)
+    torch.cuda.synchronize()
+    n = int(arrs["xi"].numel())
+    t_gen = time.perf_counter() - t_gen
+
+    d = abi.Domain(b, n, ndims, *halves)
+    soa = abi.SupersSoA()
+    soa.sdgbxindex = _ptr(arrs["sdgbxindex"], C.c_uint32)
+    soa.coord3, soa.coord1, soa.coord2 = (_ptr(arrs[k], C.c_double) for k in ("coord3", "coord1", "coord2"))
+    soa.xi = _ptr(arrs["xi"], C.c_uint64)
+    soa.radius, soa.msol = _ptr(arrs["radius"], C.c_double), _ptr(arrs["msol"], C.c_double)
+    soa.sdid = _ptr(arrs["sdid"], C.c_uint64)
+    L.cleo_upload_supers_device.argtypes = [C.c_void_p, C.POINTER(abi.SupersSoA), C.c_uint64]
+    d.check(L.cleo_upload_supers_device(d.h, C.byref(soa), n), "upload_supers_device")
+    d.ntot = n
+    del arrs, soa
+    torch.cuda.empty_cache()
+    d.set_state(st)
+    d.set_solute(*P.solute_dimless(P.AMMONIUM_SULPHATE))
+    sdm = make_params(abi)
+    d.set_sdm(sdm)
+    d.rng_counter(20261202)
+
+    L.cleo_event_record.argtypes = [C.c_void_p, C.c_int32]
+    L.cleo_event_elapsed.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_float)]
+    L.cleo_ndomain.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+    L.cleo_autotune.argtypes = [C.c_void_p, C.POINTER(abi.Tiles)]
+
+    def elapsed(a, z):
+        ms = C.c_float()
+        d.check(L.cleo_event_elapsed(d.h, a, z, C.byref(ms)), "event_elapsed")
+        return float(ms.value)
+
+    def ndomain():
+        v = C.c_uint64()
+        d.check(L.cleo_ndomain(d.h, C.byref(v)), "ndomain")
+        return int(v.value)
+
+    tiles = abi.Tiles(0, 0, 0)
+    if not args.no_autotune:
+        d.check(L.cleo_autotune(d.h, C.byref(tiles)), "autotune")
+    t = 0
+    span = 100
+    # warm-up (also compiles nothing: kernels are prebuilt)
+    if args.warmup:
+        d.check(L.cleo_run_steps_graph(d.h, t, span, args.warmup), "warmup")
+        t += args.warmup * span
+    d.check(L.cleo_synchronize(d.h), "sync")
+    nd0 = ndomain()
+
+    def timed_steps(t0, k):
+        d.reset_stats()
+        torch.cuda.synchronize()
+        d.check(L.cleo_synchronize(d.h), "sync")
+        w0 = time.perf_counter()
+        L.cleo_event_record(d.h, 0)
+        d.check(L.cleo_run_steps_graph(d.h, t0, span, k), "timed steps")
+        L.cleo_event_record(d.h, 1)
+        d.check(L.cleo_synchronize(d.h), "sync")
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        return elapsed(0, 1), wall, d.stats()
+
+    ms, wall, stats = timed_steps(t, args.steps)
+    t += args.steps * span
+    nd1 = ndomain()
+    drop_steps = 0.5 * (nd0 + nd1) * args.steps
+    plain = dict(ms_total=ms, wall_s=wall, stats=stats, drop_steps=drop_steps)
+
+    # per-stage eager timing on the following step (stage = the launches of one C-ABI call)
+    rec = L.cleo_event_record
+    rec(d.h, 2)
+    d.check(d.motion(sdm.motion), "motion")
+    rec(d.h, 3)
+    reps = 3
+    d.check(d.condensation(sdm.cond), "condensation")
+    for _ in range(reps - 1):
+        d.check(d.condensation(sdm.cond), "condensation")
+    rec(d.h, 4)
+    d.check(d.collisions(sdm.coll), "collisions")
+    rec(d.h, 5)
+    stage_ms = dict(motion_sort_qcond=elapsed(2, 3), condensation=elapsed(3, 4) / reps,
+                    collisions=elapsed(4, 5))
+    ngbx = int(np.prod(ndims))
+    nd = ndomain()
+
+    learned = None
+
+    d.check(L.cleo_synchronize(d.h), "final sync")
+    err = d.error()
+    d.close()
+    return dict(n=n, ngbx=ngbx, ndims=ndims, plain=plain, stage_ms=stage_ms, nd=nd, tiles=tiles,
+                gen_s=t_gen, learned=learned, err=err)
+
+
+def run_cpu(args):
+    from cleo_amd import abi
+    from cleo_amd import populations as P
+    from oracle import pyoracle
+
+    b = pyoracle.load("omp")
+    ndims = (128, 128, args.cpu_ny)
+    pop = P.coupled(ndims=ndims)
+    d = abi.Domain(b, pop.n, pop.ndims, *pop.halves)
+    d.upload(pop.supers)
+    d.set_state(pop.state)
+    d.set_solute(*P.solute_dimless(pop.solute))
+    sdm = make_params(abi)
+    d.set_sdm(sdm)
+    d.rng_counter(20261202)
+    for s in range(2):  # same warm-up as the GPU leg
+        d.run_step(s * 100, (s + 1) * 100)
+    t0 = time.perf_counter()
+    for s in range(2, 2 + args.cpu_steps):
+        rc = d.run_step(s * 100, (s + 1) * 100)
+        if rc != 0:
+            raise RuntimeError(f"oracle run_step rc={rc}")
+    dt = time.perf_counter() - t0
+    val = pop.n * args.cpu_steps / dt
+    return dict(value=val, unit="superdroplet-steps/s", cores=int(b.lib.orc_num_threads()), kind="port",
+                sample=f"coupled workload on 128x128x{args.cpu_ny} gridboxes ({pop.n} drops, Poisson(64)), "
+                       f"{args.cpu_steps} steps after 2 warm-up steps, {dt:.1f} s, OpenMP oracle "
+                       f"(gridbox-parallel, 72-byte AoS, atomic counting sort)")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is design-only "
+              "in this build.", file=sys.stderr)
+        return 2
+    if int(os.environ.get("RANK", "0")) != 0:
+        return 0
+    g = run_gpu(args)
+    p = g["plain"]
+    ms_step = p["ms_total"] / args.steps
+    value = p["drop_steps"] / (p["ms_total"] / 1e3)
+    st = p["stats"]
+    iters_plain = st["newton_iters"] / max(st["cond_drops"], 1)
+    # dominant kernel: condensation (FP64 Newton); algorithmic bytes per launch
+    nd = g["nd"]
+    bpd, bpg = BYTES["k_condensation"]
+    alg_bytes = bpd * nd + bpg * g["ngbx"]
+    t_cond = g["stage_ms"]["condensation"] / 1e3
+    achieved = alg_bytes / t_cond / 1e9
+    cpu = None if args.no_cpu else run_cpu(args)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "superdroplet-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"coupled SDM step: {g['ndims'][0]}x{g['ndims'][1]}x{g['ndims'][2]} gridboxes of 25 m, "
+                        f"Poisson(64) drops/gridbox, motion+sort+condensation+Long coal/breakup, dt=1 s, hipGraph",
+            "ndrops": g["n"],
+            "ngbx": g["ngbx"],
+            "tiles": [g["tiles"].cond_waves_per_block, g["tiles"].coll_waves_per_block,
+                      g["tiles"].motion_waves_per_block],
+        },
+        "newton_iters_per_drop": iters_plain,
+        "wall_s": p["wall_s"],
+        "stage_ms": g["stage_ms"],
+        "roofline": {"bound": "hbm", "kernel": "k_condensation", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    if g["learned"]:
+        out["learned"] = g["learned"]
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    if g["err"][0] != 0:
+        out["device_error"] = g["err"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -341,6 +341,8 @@ int cleo_destroy(cleo_ctx* c) {
                   static_cast<void*>(c->rs_cnt)})
     if (p) hipFree(p);
   if (c->h_ctl) hipHostFree(c->h_ctl);
+  for (hipEvent_t& e : c->ev)
+    if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return CLEO_OK;
@@ -376,6 +378,29 @@ int cleo_upload_supers(cleo_ctx* c, const cleo_supers_soa* h, uint64_t ntot) {
   HIP_TRY(c, hipGetLastError());
   const int rs = sync_check(c);
   // the upload's own sort is the reference's constructor sort, not a step sort
+  HIP_TRY(c, hipMemset(&c->ctl->stats[ST_SORTS], 0, 8));
+  return rs;
+}
+
+int cleo_upload_supers_device(cleo_ctx* c, const cleo_supers_soa* h, uint64_t ntot) {
+  if (!c || !h) return CLEO_E_ARG;
+  if (ntot > c->maxn) return fail(c, CLEO_E_ARG, "ntot exceeds the context capacity");
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  SoA& d = c->buf[c->cur];
+  HIP_TRY(c, hipMemcpy(d.gbx, h->sdgbxindex, ntot * 4, hipMemcpyDeviceToDevice));
+  HIP_TRY(c, hipMemcpy(d.c3, h->coord3, ntot * 8, hipMemcpyDeviceToDevice));
+  HIP_TRY(c, hipMemcpy(d.c1, h->coord1, ntot * 8, hipMemcpyDeviceToDevice));
+  HIP_TRY(c, hipMemcpy(d.c2, h->coord2, ntot * 8, hipMemcpyDeviceToDevice));
+  HIP_TRY(c, hipMemcpy(d.xi, h->xi, ntot * 8, hipMemcpyDeviceToDevice));
+  HIP_TRY(c, hipMemcpy(d.r, h->radius, ntot * 8, hipMemcpyDeviceToDevice));
+  HIP_TRY(c, hipMemcpy(d.msol, h->msol, ntot * 8, hipMemcpyDeviceToDevice));
+  HIP_TRY(c, hipMemcpy(d.id, h->sdid, ntot * 8, hipMemcpyDeviceToDevice));
+  c->ntot = ntot;
+  const int rc = ensure_radix(c, ntot);
+  if (rc != CLEO_OK) return rc;
+  generic_sort(c);
+  HIP_TRY(c, hipGetLastError());
+  const int rs = sync_check(c);
   HIP_TRY(c, hipMemset(&c->ctl->stats[ST_SORTS], 0, 8));
   return rs;
 }
@@ -638,6 +663,29 @@ int cleo_synchronize(cleo_ctx* c) {
   return sync_check(c);
 }
 
+int cleo_event_record(cleo_ctx* c, int32_t slot) {
+  if (!c || slot < 0 || slot >= 16) return CLEO_E_ARG;
+  if (!c->ev[slot]) HIP_TRY(c, hipEventCreate(&c->ev[slot]));
+  HIP_TRY(c, hipEventRecord(c->ev[slot], c->stream));
+  return CLEO_OK;
+}
+
+int cleo_event_elapsed(cleo_ctx* c, int32_t a, int32_t b, float* ms) {
+  if (!c || !ms || a < 0 || b < 0 || a >= 16 || b >= 16 || !c->ev[a] || !c->ev[b]) return CLEO_E_ARG;
+  HIP_TRY(c, hipEventSynchronize(c->ev[b]));
+  HIP_TRY(c, hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+  return CLEO_OK;
+}
+
+int cleo_ndomain(cleo_ctx* c, uint64_t* nd) {
+  if (!c || !nd) return CLEO_E_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  uint32_t v = 0;
+  HIP_TRY(c, hipMemcpy(&v, c->off + c->grid.ngbx, 4, hipMemcpyDeviceToHost));
+  *nd = v;
+  return CLEO_OK;
+}
+
 int cleo_get_error(cleo_ctx* c, cleo_error* err) {
   if (!c || !err) return CLEO_E_ARG;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -719,6 +767,116 @@ int cleo_set_tiles(cleo_ctx* c, const cleo_tiles* t) {
   for (uint32_t v : {t->cond_waves_per_block, t->coll_waves_per_block, t->motion_waves_per_block})
     if (v > 4) return fail(c, CLEO_E_ARG, "waves per block must be 0..4");
   c->tiles = *t;
+  return CLEO_OK;
+}
+
+// Times each kernel at 1, 2 and 4 waves (gridboxes) per workgroup on the resident population and
+// keeps the fastest.  Drops and State are left unchanged: the kernels run on a copy in the spare
+// buffer, State fields that condensation writes are saved and restored.
+int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
+  if (!c) return CLEO_E_ARG;
+  if (c->ntot == 0) return fail(c, CLEO_E_STATE, "no superdroplets resident");
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  const uint32_t ngbx = c->grid.ngbx;
+  const int cur = c->cur, spare = 1 - c->cur;
+  for (int s = 0; s < 2; ++s)
+    if (!c->ev[14 + s]) HIP_TRY(c, hipEventCreate(&c->ev[14 + s]));
+  auto copy_store = [&]() -> hipError_t {
+    const SoA& a = c->buf[cur];
+    const SoA& b = c->buf[spare];
+    const uint64_t n = c->ntot;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(b.gbx, a.gbx, n * 4, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(b.c3, a.c3, n * 8, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(b.c1, a.c1, n * 8, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(b.c2, a.c2, n * 8, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(b.xi, a.xi, n * 8, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(b.r, a.r, n * 8, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(b.msol, a.msol, n * 8, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
+    return hipMemcpyAsync(b.id, a.id, n * 8, hipMemcpyDeviceToDevice, c->stream);
+  };
+  // saved State fields (temp, qvap, qcond) and active counts
+  double* saved = nullptr;
+  uint32_t* saved_nact = nullptr;
+  HIP_TRY(c, dalloc(&saved, 3ull * ngbx));
+  HIP_TRY(c, dalloc(&saved_nact, ngbx));
+  HIP_TRY(c, hipMemcpyAsync(saved, c->st.temp, 3ull * ngbx * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(saved_nact, c->nact, ngbx * 4ull, hipMemcpyDeviceToDevice, c->stream));
+  const cleo_cond_params cp = c->sdm_set ? c->sdm.cond
+                                         : cleo_cond_params{100, 1, 50, 1e-6, 1e-8, 0.001, 0, 10};
+  cleo_coll_params kp = c->sdm_set ? c->sdm.coll : cleo_coll_params{100, CLEO_PROB_LONG, 1.0, CLEO_ENACT_COALBURE, 5.0, 0.85};
+  const cleo_motion_params mp = c->sdm_set ? c->sdm.motion : cleo_motion_params{100, CLEO_TV_SIMMEL};
+  const uint64_t seed = c->seed;
+  DevCtl saved_ctl;
+  HIP_TRY(c, hipMemcpy(&saved_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
+  float best[3] = {1e30f, 1e30f, 1e30f};
+  uint32_t pick[3] = {1, 1, 1};
+  int rc = CLEO_OK;
+  for (int kern = 0; kern < 3 && rc == CLEO_OK; ++kern) {
+    for (uint32_t wpb : {1u, 2u, 4u}) {
+      float tmin = 1e30f;
+      for (int rep = 0; rep < 2; ++rep) {
+        if (copy_store() != hipSuccess) { rc = fail(c, CLEO_E_HIP, "autotune copy"); break; }
+        hipEventRecord(c->ev[14], c->stream);
+        if (kern == 0) {
+          CondArgs a = cond_args(c, cp, false);
+          a.s = c->buf[spare];
+          a.wpb = wpb;
+          a.niters = nullptr;
+          launch_condensation(c, a);
+        } else if (kern == 1) {
+          CollArgs a{};
+          a.src = c->buf[cur];
+          a.dst = c->buf[spare];
+          a.off = c->off;
+          a.nact = c->nact;
+          a.vol = c->st.vol;
+          a.sol = c->solute;
+          a.k = CollCfg{kp.prob_kind, kp.prob_param, kp.enact_kind, kp.nfrags, kp.coalbure_coaleff,
+                        step2realtime(kp.interval)};
+          a.ngbx = ngbx;
+          a.wpb = wpb;
+          a.ctl = c->ctl;
+          a.seed = seed;
+          a.inj_oj = c->inj_oj;
+          a.inj_op = c->inj_op;
+          a.big = c->big_scratch;
+          a.timing_only = 1;
+          launch_collisions(c, a);
+        } else {
+          MotionArgs a{};
+          a.s = c->buf[spare];
+          a.off = c->off;
+          a.grid = c->grid;
+          a.st = c->st;
+          a.delt = step2dimlesstime(mp.interval);
+          a.tv = mp.terminalv;
+          a.wpb = wpb;
+          a.cnt = c->cnt;
+          a.ctl = c->ctl;
+          launch_motion(c, a);
+        }
+        hipEventRecord(c->ev[15], c->stream);
+        hipEventSynchronize(c->ev[15]);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, c->ev[14], c->ev[15]);
+        tmin = std::min(tmin, ms);
+        HIP_TRY(c, hipMemcpyAsync(c->st.temp, saved, 3ull * ngbx * 8, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->nact, saved_nact, ngbx * 4ull, hipMemcpyDeviceToDevice, c->stream));
+      }
+      if (tmin < best[kern]) {
+        best[kern] = tmin;
+        pick[kern] = wpb;
+      }
+    }
+  }
+  hipStreamSynchronize(c->stream);
+  hipFree(saved);
+  hipFree(saved_nact);
+  HIP_TRY(c, hipMemcpy(c->ctl, &saved_ctl, sizeof(DevCtl), hipMemcpyHostToDevice));
+  if (rc != CLEO_OK) return rc;
+  c->tiles = cleo_tiles{pick[0], pick[1], pick[2]};
+  if (chosen) *chosen = c->tiles;
   return CLEO_OK;
 }
 

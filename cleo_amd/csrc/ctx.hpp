@@ -100,5 +100,7 @@ struct cleo_ctx {
   double* guess = nullptr;
   // MLP
   cleo::MLPDev* mlp = nullptr;
+  // timing events on `stream`
+  hipEvent_t ev[16] = {};
   std::string msg;
 };

@@ -158,6 +158,19 @@ __device__ __forceinline__ void kohler_factors(double msol, double temp, const S
 }
 
 // ---- implicit Euler: impliciteuler.{hpp,cpp} -------------------------------------------------
+// x^3 with a single rounding: x*x is split exactly into hi + lo with an FMA, hi*x likewise, and
+// the small terms are added last, so the result is the correctly rounded cube except when x^3
+// lies within ~2^-100 relative of a rounding boundary.  The reference spells it pow(x, 3.0);
+// glibc's pow is accurate to 0.52 ulp, so the two differ by at most one ulp (in about 0.1% of
+// arguments).  Likewise pow(x, 0.5) is replaced by the correctly rounded sqrt(x).
+__device__ __forceinline__ double cube_rn(double x) {
+  const double hi = x * x;
+  const double lo = fma(x, x, -hi);
+  const double p = hi * x;
+  const double e = fma(hi, x, -p);
+  return p + (e + lo * x);
+}
+
 struct ODEConsts {
   double s_ratio, akoh, bkoh, ffactor_fv;
 };
@@ -165,45 +178,49 @@ struct NewtonCfg {
   double delt, minsubdelt, rtol, atol;
   uint32_t maxniters;
 };
-// g(z) (impliciteuler.cpp:270-281); pow(x, 2.0) is exactly x*x rounded, as the host compilers fold it
-__device__ __forceinline__ double ode_gfunc(const ODEConsts& c, double subdelt, double rprev,
-                                            double rsqrd) {
+// g(z) (impliciteuler.cpp:270-281) and dg/dz (impliciteuler.cpp:291-301) at the same z, sharing
+// sqrt(z), r^3 and akoh/r.  pow(x, 2.0) is exactly x*x rounded; pow(r, 3.0) is cube_rn(r).
+__device__ __forceinline__ void ode_g_dg(const ODEConsts& c, double subdelt, double rprev, double rsqrd,
+                                         double& g, double& dg) {
   const double radius = sqrt(rsqrd);
-  const double alpha = c.s_ratio - 1 - c.akoh / radius + c.bkoh / pow(radius, 3.0);
+  const double r3 = cube_rn(radius);
+  const double a_r = c.akoh / radius;
+  const double alpha = c.s_ratio - 1 - a_r + c.bkoh / r3;
   const double beta = 2.0 * subdelt / (rsqrd * c.ffactor_fv);
   const double q = rprev / radius;
   const double gamma = q * q;
-  return 1 - gamma - alpha * beta;
+  g = 1 - gamma - alpha * beta;
+  const double alpha_d = a_r - 3.0 * c.bkoh / r3;
+  const double beta_d = subdelt / (rsqrd * c.ffactor_fv);
+  dg = 1 - alpha_d * beta_d;
 }
-// dg/dz (impliciteuler.cpp:291-301)
-__device__ __forceinline__ double ode_gfuncderivative(const ODEConsts& c, double subdelt, double rsqrd) {
-  const double radius = sqrt(rsqrd);
-  const double alpha = c.akoh / radius - 3.0 * c.bkoh / pow(radius, 3.0);
-  const double beta = subdelt / (rsqrd * c.ffactor_fv);
-  return 1 - alpha * beta;
-}
-// one Newton update + convergence test (impliciteuler.cpp:240-255, impliciteuler.hpp:214-219)
+// Newton state: z with g(z) and g'(z) already evaluated
+struct NewtonState {
+  double z, g, dg;
+};
+// one Newton update + convergence test (impliciteuler.cpp:240-255, impliciteuler.hpp:214-219).
+// The reference evaluates g at the new z for the test and again as the next update's numerator;
+// both are the same expression of the same operands, so the value is carried over.
 __device__ __forceinline__ bool newton_iterate(const NewtonCfg& cfg, const ODEConsts& c, double subdelt,
-                                               double rprev, double& ziter, uint32_t& iters) {
+                                               double rprev, NewtonState& s, uint32_t& iters) {
   ++iters;
-  const double numerator = ode_gfunc(c, subdelt, rprev, ziter);
-  const double denominator = ode_gfuncderivative(c, subdelt, ziter);
-  ziter = ziter * (1 - numerator / denominator);
-  ziter = fmax(ziter, 1e-8);
-  const double newnumerator = ode_gfunc(c, subdelt, rprev, ziter);
-  const double converged = cfg.rtol * fabs(newnumerator) + cfg.atol;
-  const double current = fabs(newnumerator - numerator);
+  double z = s.z * (1 - s.g / s.dg);
+  z = fmax(z, 1e-8);
+  double g1, dg1;
+  ode_g_dg(c, subdelt, rprev, z, g1, dg1);
+  const double converged = cfg.rtol * fabs(g1) + cfg.atol;
+  const double current = fabs(g1 - s.g);
+  s = NewtonState{z, g1, dg1};
   return current < converged;
 }
 // newtonraphson_untilconverged (impliciteuler.cpp:207-227): false when `limit` is exceeded
 __device__ __forceinline__ bool newton_until(const NewtonCfg& cfg, const ODEConsts& c, uint32_t limit,
-                                             double subdelt, double rprev, double& ziter,
-                                             uint32_t& iters) {
+                                             double subdelt, double rprev, NewtonState& s, uint32_t& iters) {
   uint32_t niter = 1;
   bool conv = false;
   while (!conv) {
     if (niter > limit) return false;
-    conv = newton_iterate(cfg, c, subdelt, rprev, ziter, iters);
+    conv = newton_iterate(cfg, c, subdelt, rprev, s, iters);
     niter += 1;
   }
   return true;
@@ -212,16 +229,19 @@ __device__ __forceinline__ bool newton_until(const NewtonCfg& cfg, const ODECons
 __device__ __forceinline__ double newton_integrate(const NewtonCfg& cfg, const ODEConsts& c, double subdelt,
                                                    double rprev, double zguess, uint32_t& iters,
                                                    bool& maxed) {
-  double z = zguess;
-  bool conv = newton_iterate(cfg, c, subdelt, rprev, z, iters);
-  conv = newton_iterate(cfg, c, subdelt, rprev, z, iters);
-  if (conv) return z;
-  z = zguess;
-  if (!newton_until(cfg, c, cfg.maxniters, subdelt, rprev, z, iters)) maxed = true;
-  return z;
+  NewtonState s0;
+  s0.z = zguess;
+  ode_g_dg(c, subdelt, rprev, zguess, s0.g, s0.dg);
+  NewtonState s = s0;
+  bool conv = newton_iterate(cfg, c, subdelt, rprev, s, iters);
+  conv = newton_iterate(cfg, c, subdelt, rprev, s, iters);
+  if (conv) return s.z;
+  s = s0;
+  if (!newton_until(cfg, c, cfg.maxniters, subdelt, rprev, s, iters)) maxed = true;
+  return s.z;
 }
 __device__ __forceinline__ double initialguess(const ODEConsts& c, double rprev) {
-  const double s_act = 1 + sqrt(4.0 * pow(c.akoh, 3.0) / 27 / c.bkoh);
+  const double s_act = 1 + sqrt(4.0 * cube_rn(c.akoh) / 27 / c.bkoh);
   if (c.s_ratio > s_act) {
     constexpr double bigr(1e-3 / dimless::R0);
     const double maxradius = fmax(bigr, rprev);
@@ -234,7 +254,7 @@ __device__ __forceinline__ bool first_unique_criteria(const ODEConsts& c, double
   const bool is_ziter_unactivated = ziter < rcritsqrd;
   const bool is_unactivated = (rprev * rprev < rcritsqrd && is_ziter_unactivated);
   const double sqrdval = 4.0 * c.akoh * c.akoh * c.akoh / 27.0 / c.bkoh;
-  const bool is_subact = (c.s_ratio <= 1.0 + pow(sqrdval, 0.5));
+  const bool is_subact = (c.s_ratio <= 1.0 + sqrt(sqrdval));  // pow(x, 0.5) rounds as sqrt(x)
   return is_unactivated && is_subact;
 }
 __device__ __forceinline__ double critical_timestep(const ODEConsts& c) {
@@ -260,15 +280,18 @@ __device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, doubl
   double rsqrd;
   if (ucrit1 || ucrit2) {
     if (warm == warm) {
-      double z = warm;
-      bool conv = newton_iterate(cfg, c, cfg.delt, rprev, z, o.iters);
-      conv = newton_iterate(cfg, c, cfg.delt, rprev, z, o.iters);
+      NewtonState s0;
+      s0.z = warm;
+      ode_g_dg(c, cfg.delt, rprev, warm, s0.g, s0.dg);
+      NewtonState s = s0;
+      bool conv = newton_iterate(cfg, c, cfg.delt, rprev, s, o.iters);
+      conv = newton_iterate(cfg, c, cfg.delt, rprev, s, o.iters);
       if (conv) {
-        rsqrd = z;
+        rsqrd = s.z;
       } else {
-        z = warm;
-        if (newton_until(cfg, c, warm_limit, cfg.delt, rprev, z, o.iters)) {
-          rsqrd = z;
+        s = s0;
+        if (newton_until(cfg, c, warm_limit, cfg.delt, rprev, s, o.iters)) {
+          rsqrd = s.z;
         } else {
           o.restarted = true;
           rsqrd = newton_integrate(cfg, c, cfg.delt, rprev, ziter, o.iters, o.maxed);
@@ -286,7 +309,7 @@ __device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, doubl
     while (remdelt > 0.0) {
       const double subdelt = fmin(mindelt, remdelt);
       ziter = newton_integrate(cfg, c, subdelt, rp, ziter, o.iters, o.maxed);
-      rp = pow(ziter, 0.5);
+      rp = sqrt(ziter);  // pow(ziter, 0.5)
       remdelt -= subdelt;
     }
     rsqrd = ziter;

@@ -14,6 +14,7 @@ namespace cleo {
 // condensation.cpp:44-137).  With `fuse_qcond`, the gridbox qcond of
 // EffectOnHydrometeorStatesFunctor (movesupersindomain.hpp:95-136) is recomputed first, as
 // happens between motion and condensation in the reference step.
+template <bool kFuseQcond>
 __global__ void __launch_bounds__(256) k_condensation(CondArgs a) {
   const int lane = lane_id();
   const uint32_t g = blockIdx.x * a.wpb + (threadIdx.x >> 6);
@@ -28,7 +29,7 @@ __global__ void __launch_bounds__(256) k_condensation(CondArgs a) {
   const double volume = a.st.vol[g];
   bool bad_cm = false;
 
-  if (a.fuse_qcond) {
+  if (kFuseQcond) {
     double tot = 0.0;
     for (uint32_t c = 0; c < nfull; c += kWave) {
       const uint32_t k = base + c + lane;
@@ -96,7 +97,7 @@ __global__ void __launch_bounds__(256) k_condensation(CondArgs a) {
       a.st.temp[g] = tnew;
       a.st.qv[g] = qv;
     }
-    if (a.alter || a.fuse_qcond) a.st.qc[g] = qc;
+    if (a.alter || kFuseQcond) a.st.qc[g] = qc;
     add_stat(a.ctl, ST_COND_DROPS, n);
     add_stat(a.ctl, ST_NEWTON_ITERS, iters);
     add_stat(a.ctl, ST_SUBSTEP_DROPS, subs);
@@ -255,7 +256,7 @@ __global__ void __launch_bounds__(256) k_collisions(CollArgs a) {
     a.dst.msol[kd] = a.src.msol[ks];
     a.dst.id[kd] = a.src.id[ks];
   }
-  if (nulls > 0) {
+  if (nulls > 0 && !a.timing_only) {
     // remove_null_supers (collisions.hpp:240-248): the reference's sort_team is restated as a
     // stable partition, through the now unused `src` range of this gridbox.
     __threadfence();
@@ -810,7 +811,10 @@ __global__ void k_set_u32(uint32_t* out, uint32_t v, const uint32_t* flag) {
 
 void launch_condensation(cleo_ctx* c, const CondArgs& a) {
   if (a.ngbx == 0) return;
-  hipLaunchKernelGGL(k_condensation, dim3(nblocks(a.ngbx, a.wpb)), dim3(64 * a.wpb), 0, c->stream, a);
+  if (a.fuse_qcond)
+    hipLaunchKernelGGL(k_condensation<true>, dim3(nblocks(a.ngbx, a.wpb)), dim3(64 * a.wpb), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL(k_condensation<false>, dim3(nblocks(a.ngbx, a.wpb)), dim3(64 * a.wpb), 0, c->stream, a);
 }
 void launch_qcond(cleo_ctx* c, const QcondArgs& a) {
   if (a.ngbx == 0) return;

@@ -50,6 +50,7 @@ struct CollArgs {
   const uint32_t* inj_oj;
   const uint32_t* inj_op;
   uint32_t* big;
+  int32_t timing_only;  // autotuner runs: no null partition, `src` and `nact` stay untouched
 };
 
 struct MotionArgs {

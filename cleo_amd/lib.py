@@ -25,6 +25,19 @@ def load():
         lib.cleo_synchronize.restype = C.c_int
         lib.cleo_record_guess.argtypes = [C.c_void_p, C.c_int32]
         lib.cleo_download_guess.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_uint64]
+        fp = C.POINTER(C.c_float)
+        lib.cleo_mlp_set_weights.argtypes = [C.c_void_p, fp]
+        lib.cleo_mlp_get_weights.argtypes = [C.c_void_p, fp]
+        lib.cleo_mlp_make_samples.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                              C.POINTER(C.c_uint64)]
+        lib.cleo_mlp_train.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, fp]
+        lib.cleo_mlp_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]
+        lib.cleo_mlp_loss_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, fp, fp]
+        lib.cleo_upload_supers_device.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
+        lib.cleo_event_record.argtypes = [C.c_void_p, C.c_int32]
+        lib.cleo_event_elapsed.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_float)]
+        lib.cleo_ndomain.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+        lib.cleo_autotune.argtypes = [C.c_void_p, C.c_void_p]
         _backend = b
     return _backend
 

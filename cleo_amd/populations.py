@@ -234,3 +234,50 @@ def coupled(seed=20261202, ndims=(128, 128, 128), spacing_m=25.0, mean=64.0, sol
     st.update(divfree_winds(ndims, pop.halves, wmax_mps))
     pop.state = st
     return pop
+
+
+def coupled_device(torch, device, seed=20261202, ndims=(128, 128, 128), spacing_m=25.0, mean=64.0,
+                   solute=AMMONIUM_SULPHATE, wmax_mps=2.0):
+    """The `coupled` workload drawn on the GPU with torch (same distributions, different
+    random stream): returns (arrays dict of device tensors, state dict of numpy arrays, halves).
+    sdgbxindex is int32 and xi, sdid int64 tensors; their bits are the u32 / u64 the C ABI reads."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    ngbx = int(np.prod(ndims))
+    nz, nx, ny = ndims
+    halves = uniform_grid(ndims, spacing_m)
+    counts = torch.poisson(torch.full((ngbx,), float(mean), device=device, dtype=torch.float32),
+                           generator=gen).to(torch.int64)
+    gidx = torch.repeat_interleave(torch.arange(ngbx, device=device, dtype=torch.int64), counts)
+    n = int(gidx.numel())
+    iz, ix, iy = gidx % nz, (gidx // nz) % nx, gidx // (nz * nx)
+    dzc = spacing_m / COORD0
+    out = {}
+    for name, i in (("coord3", iz), ("coord1", ix), ("coord2", iy)):
+        u = torch.rand(n, device=device, dtype=torch.float64, generator=gen)
+        lo = i.to(torch.float64) * dzc
+        c = lo + u * dzc
+        out[name] = torch.minimum(c, torch.nextafter(lo + dzc, torch.zeros_like(lo)))
+        del u, lo, c
+    del iz, ix, iy
+    out["sdgbxindex"] = gidx.to(torch.int32)
+    del gidx
+    rho_sol, _, _ = solute_dimless(solute)
+    lmin, lmax = np.log(1e-18), np.log(1e-15)
+    msol = torch.exp(lmin + (lmax - lmin) * torch.rand(n, device=device, dtype=torch.float64, generator=gen)) / MASS0
+    pick = torch.rand(n, device=device, dtype=torch.float64, generator=gen) < 0.5
+    r1 = torch.exp(np.log(0.05e-6) + np.log(1.4) * torch.randn(n, device=device, dtype=torch.float64, generator=gen))
+    r2 = torch.exp(np.log(15e-6) + np.log(1.6) * torch.randn(n, device=device, dtype=torch.float64, generator=gen))
+    radius = torch.where(pick, r1, r2) / R0
+    del r1, r2, pick
+    rdry = torch.pow(3.0 / (4.0 * np.pi) * msol / rho_sol, 1.0 / 3.0)
+    out["radius"] = torch.maximum(radius, rdry)
+    out["msol"] = msol
+    del radius, rdry
+    out["xi"] = torch.randint(1_000_000, 100_000_001, (n,), device=device, dtype=torch.int64, generator=gen)
+    out["sdid"] = torch.arange(n, device=device, dtype=torch.int64)
+    rng = np.random.default_rng(seed + 1)
+    temp_K, press_Pa, s_ratio = _thermo(rng, ngbx)
+    st = _state(ngbx, spacing_m ** 3 / VOL0, temp_K, press_Pa, s_ratio)
+    st.update(divfree_winds(ndims, halves, wmax_mps))
+    return out, st, halves

@@ -198,6 +198,8 @@ int cleo_upload_supers(cleo_ctx* ctx, const cleo_supers_soa* host, uint64_t ntot
 /* Same from the reference's 72-byte Superdrop layout (superdrop.hpp:36-60). */
 int cleo_upload_supers_aos(cleo_ctx* ctx, const void* aos72, uint64_t ntot);
 /* Whole store in reference order: in-domain drops sorted by gridbox, then out-of-domain. */
+/* Same from device-resident SoA arrays (any device pointers on the context's GPU). */
+int cleo_upload_supers_device(cleo_ctx* ctx, const cleo_supers_soa* dev, uint64_t ntot);
 int cleo_download_supers(cleo_ctx* ctx, cleo_supers_soa* host, uint64_t* ntot, uint64_t* ndomain);
 int cleo_download_supers_aos(cleo_ctx* ctx, void* aos72, uint64_t* ntot, uint64_t* ndomain);
 /* refs: ngbx+1 entries, gridbox g holds [refs[g], refs[g+1]) (supersingbx.hpp:32-102). */
@@ -235,6 +237,11 @@ int cleo_run_step(cleo_ctx* ctx, uint32_t t_mdl, uint32_t t_mdl_next);
 int cleo_run_steps_graph(cleo_ctx* ctx, uint32_t t_mdl, uint32_t span, uint32_t nsteps);
 
 int cleo_synchronize(cleo_ctx* ctx);
+/* HIP events on the context's stream (slots 0..15), for timing sequences of calls. */
+int cleo_event_record(cleo_ctx* ctx, int32_t slot);
+int cleo_event_elapsed(cleo_ctx* ctx, int32_t slot_begin, int32_t slot_end, float* ms);
+/* number of in-domain superdroplets (refs[ngbx]) */
+int cleo_ndomain(cleo_ctx* ctx, uint64_t* ndomain);
 int cleo_get_error(cleo_ctx* ctx, cleo_error* err);
 int cleo_clear_error(cleo_ctx* ctx);
 int cleo_get_stats(cleo_ctx* ctx, cleo_stats* stats);
