@@ -83,56 +83,40 @@ def run_gpu(args):
     torch.cuda.synchronize()
     n = int(arrs["xi"].numel())
     t_gen = time.perf_counter() - t_gen
-
-    d = abi.Domain(b, n, ndims, *halves)
     soa = abi.SupersSoA()
     soa.sdgbxindex = _ptr(arrs["sdgbxindex"], C.c_uint32)
     soa.coord3, soa.coord1, soa.coord2 = (_ptr(arrs[k], C.c_double) for k in ("coord3", "coord1", "coord2"))
     soa.xi = _ptr(arrs["xi"], C.c_uint64)
     soa.radius, soa.msol = _ptr(arrs["radius"], C.c_double), _ptr(arrs["msol"], C.c_double)
     soa.sdid = _ptr(arrs["sdid"], C.c_uint64)
-    L.cleo_upload_supers_device.argtypes = [C.c_void_p, C.POINTER(abi.SupersSoA), C.c_uint64]
-    d.check(L.cleo_upload_supers_device(d.h, C.byref(soa), n), "upload_supers_device")
-    d.ntot = n
-    del arrs, soa
-    torch.cuda.empty_cache()
-    d.set_state(st)
-    d.set_solute(*P.solute_dimless(P.AMMONIUM_SULPHATE))
     sdm = make_params(abi)
-    d.set_sdm(sdm)
-    d.rng_counter(20261202)
+    span = 100
 
-    L.cleo_event_record.argtypes = [C.c_void_p, C.c_int32]
-    L.cleo_event_elapsed.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_float)]
-    L.cleo_ndomain.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
-    L.cleo_autotune.argtypes = [C.c_void_p, C.POINTER(abi.Tiles)]
+    def new_domain():
+        d = abi.Domain(b, n, ndims, *halves)
+        d.check(L.cleo_upload_supers_device(d.h, C.byref(soa), n), "upload_supers_device")
+        d.ntot = n
+        d.set_state(st)
+        d.set_solute(*P.solute_dimless(P.AMMONIUM_SULPHATE))
+        d.set_sdm(sdm)
+        d.rng_counter(20261202)
+        return d
 
-    def elapsed(a, z):
+    def elapsed(d, a, z):
         ms = C.c_float()
         d.check(L.cleo_event_elapsed(d.h, a, z, C.byref(ms)), "event_elapsed")
         return float(ms.value)
 
-    def ndomain():
+    def ndomain(d):
         v = C.c_uint64()
         d.check(L.cleo_ndomain(d.h, C.byref(v)), "ndomain")
         return int(v.value)
 
-    tiles = abi.Tiles(0, 0, 0)
-    if not args.no_autotune:
-        d.check(L.cleo_autotune(d.h, C.byref(tiles)), "autotune")
-    t = 0
-    span = 100
-    # warm-up (also compiles nothing: kernels are prebuilt)
-    if args.warmup:
-        d.check(L.cleo_run_steps_graph(d.h, t, span, args.warmup), "warmup")
-        t += args.warmup * span
-    d.check(L.cleo_synchronize(d.h), "sync")
-    nd0 = ndomain()
-
-    def timed_steps(t0, k):
+    def timed_steps(d, t0, k):
         d.reset_stats()
         torch.cuda.synchronize()
         d.check(L.cleo_synchronize(d.h), "sync")
+        nd0 = ndomain(d)
         w0 = time.perf_counter()
         L.cleo_event_record(d.h, 0)
         d.check(L.cleo_run_steps_graph(d.h, t0, span, k), "timed steps")
@@ -140,13 +124,37 @@ def run_gpu(args):
         d.check(L.cleo_synchronize(d.h), "sync")
         torch.cuda.synchronize()
         wall = time.perf_counter() - w0
-        return elapsed(0, 1), wall, d.stats()
+        ms = elapsed(d, 0, 1)
+        nd1 = ndomain(d)
+        stats = d.stats()
+        return dict(ms_total=ms, wall_s=wall, stats=stats, drop_steps=0.5 * (nd0 + nd1) * k,
+                    iters_per_drop=stats["newton_iters"] / max(stats["cond_drops"], 1))
 
-    ms, wall, stats = timed_steps(t, args.steps)
+    d = new_domain()
+    tiles = abi.Tiles(0, 0, 0)
+    if not args.no_autotune:
+        d.check(L.cleo_autotune(d.h, C.byref(tiles)), "autotune")
+    t = 0
+    if args.warmup:
+        d.check(L.cleo_run_steps_graph(d.h, t, span, args.warmup), "warmup")
+        t += args.warmup * span
+    dl = None
+    if not args.no_mlp:
+        # identical twin for the learned leg: same upload, same warm-up, same random streams
+        dl = new_domain()
+        L.cleo_set_tiles(dl.h, C.byref(tiles))
+        if args.warmup:
+            dl.check(L.cleo_run_steps_graph(dl.h, 0, span, args.warmup), "warmup learned")
+    del arrs, soa
+    torch.cuda.empty_cache()
+
+    plain = timed_steps(d, t, args.steps)
+
+    learned = None
+    if dl is not None:
+        learned = learned_leg(args, b, dl, sdm, t, timed_steps, torch)
+        dl.close()
     t += args.steps * span
-    nd1 = ndomain()
-    drop_steps = 0.5 * (nd0 + nd1) * args.steps
-    plain = dict(ms_total=ms, wall_s=wall, stats=stats, drop_steps=drop_steps)
 
     # per-stage eager timing on the following step (stage = the launches of one C-ABI call)
     rec = L.cleo_event_record
@@ -154,24 +162,52 @@ def run_gpu(args):
     d.check(d.motion(sdm.motion), "motion")
     rec(d.h, 3)
     reps = 3
-    d.check(d.condensation(sdm.cond), "condensation")
-    for _ in range(reps - 1):
+    for _ in range(reps):
         d.check(d.condensation(sdm.cond), "condensation")
     rec(d.h, 4)
     d.check(d.collisions(sdm.coll), "collisions")
     rec(d.h, 5)
-    stage_ms = dict(motion_sort_qcond=elapsed(2, 3), condensation=elapsed(3, 4) / reps,
-                    collisions=elapsed(4, 5))
+    stage_ms = dict(motion_sort_qcond=elapsed(d, 2, 3), condensation=elapsed(d, 3, 4) / reps,
+                    collisions=elapsed(d, 4, 5))
     ngbx = int(np.prod(ndims))
-    nd = ndomain()
-
-    learned = None
-
+    nd = ndomain(d)
     d.check(L.cleo_synchronize(d.h), "final sync")
     err = d.error()
     d.close()
     return dict(n=n, ngbx=ngbx, ndims=ndims, plain=plain, stage_ms=stage_ms, nd=nd, tiles=tiles,
                 gen_s=t_gen, learned=learned, err=err)
+
+
+def learned_leg(args, b, d, sdm, t, timed_steps, torch):
+    """Train the warm-start MLP on samples the plain solver produces from the current population,
+    then time the same steps with the learned guess."""
+    from cleo_amd import abi
+    L = b.lib
+    cap = 10_000_000
+    xd = torch.zeros((cap, 8), dtype=torch.float32, device="cuda")
+    yd = torch.zeros(cap, dtype=torch.float32, device="cuda")
+    nw = C.c_uint64()
+    t0 = time.perf_counter()
+    d.check(L.cleo_mlp_make_samples(d.h, C.byref(sdm.cond), xd.data_ptr(), yd.data_ptr(), cap, C.byref(nw)),
+            "make_samples")
+    t_samples = time.perf_counter() - t0
+    ns = int(nw.value)
+    tp = abi.MLPTrainParams(20, 65536, 1e-3, 0.9, 0.999, 1e-8, 20261202)
+    loss = C.c_float()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d.check(L.cleo_mlp_train(d.h, xd.data_ptr(), yd.data_ptr(), ns, C.byref(tp), C.byref(loss)), "mlp_train")
+    t_train = time.perf_counter() - t0
+    del xd, yd
+    torch.cuda.empty_cache()
+    sdm_l = make_params(abi)
+    sdm_l.cond.warm_start = 1
+    sdm_l.cond.warm_restart_iters = 10
+    d.set_sdm(sdm_l)
+    res = timed_steps(d, t, args.steps)
+    res.update(samples=ns, train_s=t_train, samples_s=t_samples, final_loss=float(loss.value),
+               warm_restarts=res["stats"]["warm_restarts"])
+    return res
 
 
 def run_cpu(args):
@@ -216,8 +252,7 @@ def main():
     p = g["plain"]
     ms_step = p["ms_total"] / args.steps
     value = p["drop_steps"] / (p["ms_total"] / 1e3)
-    st = p["stats"]
-    iters_plain = st["newton_iters"] / max(st["cond_drops"], 1)
+    iters_plain = p["iters_per_drop"]
     # dominant kernel: condensation (FP64 Newton); algorithmic bytes per launch
     nd = g["nd"]
     bpd, bpg = BYTES["k_condensation"]
@@ -248,13 +283,23 @@ def main():
         },
         "newton_iters_per_drop": iters_plain,
         "wall_s": p["wall_s"],
+        "gen_s": g["gen_s"],
         "stage_ms": g["stage_ms"],
         "roofline": {"bound": "hbm", "kernel": "k_condensation", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,
     }
     if g["learned"]:
-        out["learned"] = g["learned"]
+        lr = g["learned"]
+        out["learned"] = {
+            "value": lr["drop_steps"] / (lr["ms_total"] / 1e3),
+            "ms_per_step": lr["ms_total"] / args.steps,
+            "newton_iters_per_drop": lr["iters_per_drop"],
+            "iters_cut_vs_plain": 1.0 - lr["iters_per_drop"] / iters_plain,
+            "warm_restarts": lr["warm_restarts"],
+            "train": {"samples": lr["samples"], "epochs": 20, "batch": 65536, "lr": 1e-3, "seconds": lr["train_s"],
+                      "final_mse_normalised": lr["final_loss"]},
+        }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
     if g["err"][0] != 0:

@@ -132,7 +132,13 @@ int enqueue_condensation(cleo_ctx* c, const cleo_cond_params& p, bool fuse_qcond
     if (c->rec_guess && c->guess)
       HIP_TRY(c, hipMemcpyAsync(c->guess, a.warm, c->ntot * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   }
-  launch_condensation(c, a);
+  if (c->tiles.cond_flat) {
+    CondFlatArgs f{a.s, a.off, a.nact, a.st, a.sol, a.cfg, a.ngbx, a.alter, a.fuse_qcond, a.ctl, a.niters,
+                   a.warm, a.warm_limit, c->gs, c->dm};
+    launch_condensation_flat(c, f);
+  } else {
+    launch_condensation(c, a);
+  }
   return launch_check(c);
 }
 
@@ -199,7 +205,10 @@ int enqueue_motion(cleo_ctx* c, const cleo_motion_params& p, bool qcond) {
   a.wpb = wpb_or1(c->tiles.motion_waves_per_block);
   a.cnt = c->cnt;
   a.ctl = c->ctl;
-  launch_motion(c, a);
+  if (c->tiles.motion_flat)
+    launch_motion_flat(c, a);
+  else
+    launch_motion(c, a);
   neighbour_sort(c, c->cur, 1 - c->cur, nullptr, false);
   if (qcond) {
     QcondArgs q{c->buf[c->cur], c->off, c->st, c->solute, c->grid.ngbx, wpb_or1(c->tiles.cond_waves_per_block), c->ctl};
@@ -311,6 +320,8 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   c->scan_tmp_n = scan_tmp_words(28 * ngbx + 1);
   HIP_TRY(c, dalloc(&c->scan_tmp, c->scan_tmp_n));
   HIP_TRY(c, dalloc(&c->big_scratch, 2 * maxn));
+  HIP_TRY(c, dalloc(&c->gs, 2 * ngbx));
+  HIP_TRY(c, dalloc(&c->dm, maxn));
   c->big_scratch_n = 2 * maxn;
   HIP_TRY(c, dalloc(&c->ctl, 1));
   HIP_TRY(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_ctl), sizeof(DevCtl), hipHostMallocDefault));
@@ -338,7 +349,7 @@ int cleo_destroy(cleo_ctx* c) {
                   static_cast<void*>(c->ctl), static_cast<void*>(c->inj_j), static_cast<void*>(c->inj_phi),
                   static_cast<void*>(c->niters), static_cast<void*>(c->guess), static_cast<void*>(c->rs_key[0]),
                   static_cast<void*>(c->rs_key[1]), static_cast<void*>(c->rs_idx[0]), static_cast<void*>(c->rs_idx[1]),
-                  static_cast<void*>(c->rs_cnt)})
+                  static_cast<void*>(c->rs_cnt), static_cast<void*>(c->gs), static_cast<void*>(c->dm)})
     if (p) hipFree(p);
   if (c->h_ctl) hipHostFree(c->h_ctl);
   for (hipEvent_t& e : c->ev)
@@ -766,6 +777,7 @@ int cleo_set_tiles(cleo_ctx* c, const cleo_tiles* t) {
   if (!c || !t) return CLEO_E_ARG;
   for (uint32_t v : {t->cond_waves_per_block, t->coll_waves_per_block, t->motion_waves_per_block})
     if (v > 4) return fail(c, CLEO_E_ARG, "waves per block must be 0..4");
+  if (t->cond_flat > 1 || t->motion_flat > 1) return fail(c, CLEO_E_ARG, "variant flags must be 0 or 1");
   c->tiles = *t;
   return CLEO_OK;
 }
@@ -811,9 +823,15 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
   HIP_TRY(c, hipMemcpy(&saved_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
   float best[3] = {1e30f, 1e30f, 1e30f};
   uint32_t pick[3] = {1, 1, 1};
+  uint32_t pick_flat[3] = {0, 0, 0};
   int rc = CLEO_OK;
+  struct Cand {
+    uint32_t wpb, flat;
+  };
+  const Cand cands[4] = {{1, 0}, {2, 0}, {4, 0}, {1, 1}};
   for (int kern = 0; kern < 3 && rc == CLEO_OK; ++kern) {
-    for (uint32_t wpb : {1u, 2u, 4u}) {
+    for (const Cand& cd : cands) {
+      if (kern == 1 && cd.flat) continue;  // collisions are gridbox-shaped only
       float tmin = 1e30f;
       for (int rep = 0; rep < 2; ++rep) {
         if (copy_store() != hipSuccess) { rc = fail(c, CLEO_E_HIP, "autotune copy"); break; }
@@ -821,9 +839,15 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
         if (kern == 0) {
           CondArgs a = cond_args(c, cp, false);
           a.s = c->buf[spare];
-          a.wpb = wpb;
+          a.wpb = cd.wpb;
           a.niters = nullptr;
-          launch_condensation(c, a);
+          if (cd.flat) {
+            CondFlatArgs f{a.s, a.off, a.nact, a.st, a.sol, a.cfg, a.ngbx, a.alter, a.fuse_qcond, a.ctl, nullptr,
+                           nullptr, a.warm_limit, c->gs, c->dm};
+            launch_condensation_flat(c, f);
+          } else {
+            launch_condensation(c, a);
+          }
         } else if (kern == 1) {
           CollArgs a{};
           a.src = c->buf[cur];
@@ -835,7 +859,7 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
           a.k = CollCfg{kp.prob_kind, kp.prob_param, kp.enact_kind, kp.nfrags, kp.coalbure_coaleff,
                         step2realtime(kp.interval)};
           a.ngbx = ngbx;
-          a.wpb = wpb;
+          a.wpb = cd.wpb;
           a.ctl = c->ctl;
           a.seed = seed;
           a.inj_oj = c->inj_oj;
@@ -851,10 +875,13 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
           a.st = c->st;
           a.delt = step2dimlesstime(mp.interval);
           a.tv = mp.terminalv;
-          a.wpb = wpb;
+          a.wpb = cd.wpb;
           a.cnt = c->cnt;
           a.ctl = c->ctl;
-          launch_motion(c, a);
+          if (cd.flat)
+            launch_motion_flat(c, a);
+          else
+            launch_motion(c, a);
         }
         hipEventRecord(c->ev[15], c->stream);
         hipEventSynchronize(c->ev[15]);
@@ -866,7 +893,8 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
       }
       if (tmin < best[kern]) {
         best[kern] = tmin;
-        pick[kern] = wpb;
+        pick[kern] = cd.wpb;
+        pick_flat[kern] = cd.flat;
       }
     }
   }
@@ -875,7 +903,7 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
   hipFree(saved_nact);
   HIP_TRY(c, hipMemcpy(c->ctl, &saved_ctl, sizeof(DevCtl), hipMemcpyHostToDevice));
   if (rc != CLEO_OK) return rc;
-  c->tiles = cleo_tiles{pick[0], pick[1], pick[2]};
+  c->tiles = cleo_tiles{pick[0], pick[1], pick[2], pick_flat[0], pick_flat[2]};
   if (chosen) *chosen = c->tiles;
   return CLEO_OK;
 }

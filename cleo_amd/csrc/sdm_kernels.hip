@@ -945,3 +945,201 @@ void generic_sort(cleo_ctx* c) {
 }
 
 }  // namespace cleo
+
+// ============================ flat (thread-per-drop) variants =================================
+namespace cleo {
+
+// per gridbox: optional qcond recompute (ordered, before any radius changes), then the
+// saturation ratio and diffusion factor condensation uses (condensation.cpp:44-56)
+__global__ void __launch_bounds__(256) k_cond_prep(CondFlatArgs a) {
+  const int lane = lane_id();
+  const uint32_t g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (g >= a.ngbx) return;
+  const double temp = a.st.temp[g], press = a.st.press[g], qv = a.st.qv[g];
+  if (a.fuse_qcond) {
+    const uint32_t base = a.off[g];
+    const uint32_t nfull = a.off[g + 1] - base;
+    bool bad = false;
+    double tot = 0.0;
+    for (uint32_t c = 0; c < nfull; c += kWave) {
+      const uint32_t k = base + c + lane;
+      double v = 0.0;
+      if (c + lane < nfull) v = condensate_mass(a.s.r[k], a.s.msol[k], a.sol, &bad) * static_cast<double>(a.s.xi[k]);
+      tot = ordered_add(tot, v, static_cast<int>(min(static_cast<uint32_t>(kWave), nfull - c)));
+    }
+    if (bad) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, g, base);
+    if (lane == 0) {
+      const double rho_cond = tot / a.st.vol[g] * dimless::R0cubed_VOL0;
+      a.st.qc[g] = rho_cond / dry_air_density(press, temp, qv);
+    }
+  }
+  if (lane == 0) {
+    if (temp <= 0) set_error(a.ctl, CLEO_ERR_TEMP_NONPOSITIVE, g, a.off[g]);
+    const double psat = saturation_pressure(temp);
+    a.gs[2 * g] = supersaturation_ratio(press, qv, psat);
+    a.gs[2 * g + 1] = diffusion_factor(press, temp, psat);
+  }
+}
+
+// per active drop: Koehler factors, Newton solve, radius update, mass change * xi into dm[k]
+__global__ void __launch_bounds__(256) k_cond_drops(CondFlatArgs a) {
+  const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint32_t ndom = a.off[a.ngbx];
+  uint32_t iters = 0, subs = 0, restarts = 0, solved = 0;
+  if (k < ndom) {
+    const uint32_t g = a.s.gbx[k];
+    if (g < a.ngbx && k < a.off[g] + a.nact[g]) {
+      const double s_ratio = a.gs[2 * g], ffactor = a.gs[2 * g + 1];
+      const double temp = a.st.temp[g];
+      const double r = a.s.r[k], msol = a.s.msol[k];
+      const uint64_t xi = a.s.xi[k];
+      bool bad = false;
+      const double old_m = condensate_mass(r, msol, a.sol, &bad);
+      double akoh, bkoh;
+      kohler_factors(msol, temp, a.sol, akoh, bkoh);
+      const double warm = a.warm ? a.warm[k] : __longlong_as_double(0x7FF8000000000000ll);
+      SolveOut o{0, false, false, false};
+      const double newr = solve_condensation(a.cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
+      const double rnew = fmax(newr, dryradius(msol, a.sol));
+      a.s.r[k] = rnew;
+      a.dm[k] = (condensate_mass(rnew, msol, a.sol, &bad) - old_m) * static_cast<double>(xi);
+      if (bad) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, g, k);
+      if (o.maxed) set_error(a.ctl, CLEO_ERR_NEWTON_MAXITERS, g, k);
+      if (a.niters) a.niters[k] = o.iters;
+      iters = o.iters;
+      subs = o.substep ? 1u : 0u;
+      restarts = o.restarted ? 1u : 0u;
+      solved = 1;
+    }
+  }
+  iters = wave_sum_u32(iters);
+  subs = wave_sum_u32(subs);
+  restarts = wave_sum_u32(restarts);
+  solved = wave_sum_u32(solved);
+  if (lane_id() == 0) {
+    add_stat(a.ctl, ST_COND_DROPS, solved);
+    add_stat(a.ctl, ST_NEWTON_ITERS, iters);
+    add_stat(a.ctl, ST_SUBSTEP_DROPS, subs);
+    add_stat(a.ctl, ST_WARM_RESTARTS, restarts);
+  }
+}
+
+// per gridbox: Σ dm in drop order and the thermodynamic feedback (condensation.cpp:126-137)
+__global__ void __launch_bounds__(256) k_cond_state(CondFlatArgs a) {
+  const int lane = lane_id();
+  const uint32_t g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (g >= a.ngbx) return;
+  const uint32_t base = a.off[g];
+  const uint32_t n = a.nact[g];
+  double tot = 0.0;
+  for (uint32_t c = 0; c < n; c += kWave) {
+    const double v = (c + lane < n) ? a.dm[base + c + lane] : 0.0;
+    tot = ordered_add(tot, v, static_cast<int>(min(static_cast<uint32_t>(kWave), n - c)));
+  }
+  if (lane == 0 && a.alter) {
+    const double temp = a.st.temp[g], press = a.st.press[g];
+    double qv = a.st.qv[g], qc = a.st.qc[g];
+    const double totrho = tot / a.st.vol[g] * dimless::R0cubed_VOL0;
+    const double dq = totrho / dry_air_density(press, temp, qv);
+    qv -= dq;
+    qc += dq;
+    a.st.temp[g] = temp + (dimless::Latent_v / moist_specific_heat(qv, qc)) * dq;
+    a.st.qv[g] = qv;
+    a.st.qc[g] = qc;
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+// MoveSupersInGridboxesFunctor per drop; direction counts aggregated per wave, then atomics
+__global__ void __launch_bounds__(256) k_motion_flat(MotionArgs a) {
+  const int lane = lane_id();
+  const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint32_t ndom = a.off[a.grid.ngbx];
+  const bool act = k < ndom;
+  uint32_t key = 0;  // g*28 + direction
+  if (act) {
+    const uint32_t g = a.s.gbx[k];
+    const uint32_t nz = a.grid.nz, nx = a.grid.nx, ny = a.grid.ny;
+    const uint32_t iz = g % nz, ix = (g / nz) % nx, iy = g / (nz * nx);
+    const double zlo = a.grid.zh[iz], zhi = a.grid.zh[iz + 1];
+    const double xlo = a.grid.xh[ix], xhi = a.grid.xh[ix + 1];
+    const double ylo = a.grid.yh[iy], yhi = a.grid.yh[iy + 1];
+    const double wlo = a.st.wlo[g], whi = a.st.whi[g];
+    const double ulo = a.st.ulo[g], uhi = a.st.uhi[g];
+    const double vlo = a.st.vlo[g], vhi = a.st.vhi[g];
+    const double delt = a.delt;
+    const double c3_0 = a.s.c3[k], c1_0 = a.s.c1[k], c2_0 = a.s.c2[k];
+    double coord3 = c3_0;
+    const double terminal = terminal_velocity(a.tv, a.s.r[k]);
+    double vel3 = interpolation(zlo, zhi, wlo, whi, coord3);
+    vel3 -= terminal;
+    coord3 += vel3 * delt;
+    double corrvel3 = interpolation(zlo, zhi, wlo, whi, coord3);
+    corrvel3 -= terminal;
+    const double delta3 = (vel3 + corrvel3) * (delt / 2);
+    double coord1 = c1_0;
+    const double vel1 = interpolation(xlo, xhi, ulo, uhi, coord1);
+    coord1 += vel1 * delt;
+    const double corrvel1 = interpolation(xlo, xhi, ulo, uhi, coord1);
+    const double delta1 = (vel1 + corrvel1) * (delt / 2);
+    double coord2 = c2_0;
+    const double vel2 = interpolation(ylo, yhi, vlo, vhi, coord2);
+    coord2 += vel2 * delt;
+    const double corrvel2 = interpolation(ylo, yhi, vlo, vhi, coord2);
+    const double delta2 = (vel2 + corrvel2) * (delt / 2);
+    bool cfl = (fabs(delta3) <= fabs(zhi - zlo));
+    cfl = (fabs(delta1) <= fabs(xhi - xlo)) && cfl;
+    cfl = (fabs(delta2) <= fabs(yhi - ylo)) && cfl;
+    if (!cfl) set_error(a.ctl, CLEO_ERR_CFL, g, k);
+    double c3 = c3_0 + delta3, c1 = c1_0 + delta1, c2 = c2_0 + delta2;
+    uint32_t idx = g;
+    const int dz = axis_step(idx, c3, iz, nz, 1u, a.grid.per[0] != 0, a.grid.zh);
+    int dx = 0, dy = 0;
+    if (idx != kOOB) dx = axis_step(idx, c1, ix, nx, nz, a.grid.per[1] != 0, a.grid.xh);
+    if (idx != kOOB) dy = axis_step(idx, c2, iy, ny, nz * nx, a.grid.per[2] != 0, a.grid.yh);
+    a.s.c3[k] = c3;
+    a.s.c1[k] = c1;
+    a.s.c2[k] = c2;
+    a.s.gbx[k] = idx;
+    int dir = 27;
+    if (idx != kOOB) {
+      const uint32_t jz = idx % nz, jx = (idx / nz) % nx, jy = idx / (nz * nx);
+      const bool bad = (c3 < a.grid.zh[jz]) | (c3 >= a.grid.zh[jz + 1]) | (c1 < a.grid.xh[jx]) |
+                       (c1 >= a.grid.xh[jx + 1]) | (c2 < a.grid.yh[jy]) | (c2 >= a.grid.yh[jy + 1]);
+      if (bad) set_error(a.ctl, CLEO_ERR_BOUNDS, g, k);
+      dir = (dz + 1) + 3 * (dx + 1) + 9 * (dy + 1);
+    }
+    key = 28u * g + static_cast<uint32_t>(dir);
+  }
+  uint64_t mask = __ballot(act);
+  const uint32_t nmoved = static_cast<uint32_t>(__popcll(mask));
+  while (mask) {
+    const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+    const uint32_t v = readlane_u32(key, leader);
+    const uint64_t m = __ballot(act && key == v);
+    if (lane == leader) atomicAdd(&a.cnt[v], static_cast<uint32_t>(__popcll(m)));
+    mask &= ~m;
+  }
+  if (lane == 0) add_stat(a.ctl, ST_MOTION_DROPS, nmoved);
+}
+
+void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& a) {
+  if (a.ngbx == 0) return;
+  hipLaunchKernelGGL(k_cond_prep, dim3(nblocks(a.ngbx, 4)), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(k_cond_drops, dim3(nblocks(c->ntot, 256)), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(k_cond_state, dim3(nblocks(a.ngbx, 4)), dim3(256), 0, c->stream, a);
+}
+
+void launch_motion_flat(cleo_ctx* c, const MotionArgs& a) {
+  if (a.grid.ngbx == 0) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(stride_blocks(28ull * a.grid.ngbx)), dim3(256), 0, c->stream, a.cnt,
+                     28ull * a.grid.ngbx, 0u);
+  hipLaunchKernelGGL(k_motion_flat, dim3(nblocks(c->ntot, 256)), dim3(256), 0, c->stream, a);
+}
+
+}  // namespace cleo

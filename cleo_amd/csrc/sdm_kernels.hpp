@@ -79,7 +79,28 @@ struct NsortArgs {
   uint64_t ntot;
 };
 
+// thread-per-drop condensation: prep (per gridbox) -> drops -> state (per gridbox)
+struct CondFlatArgs {
+  SoA s;
+  const uint32_t* off;
+  const uint32_t* nact;
+  GState st;
+  Solute sol;
+  NewtonCfg cfg;
+  uint32_t ngbx;
+  int32_t alter;
+  int32_t fuse_qcond;
+  DevCtl* ctl;
+  uint32_t* niters;
+  const double* warm;
+  uint32_t warm_limit;
+  double* gs;  // 2 per gridbox: saturation ratio, diffusion factor
+  double* dm;  // per drop: condensed mass * xi
+};
+
 // launchers (sdm_kernels.hip); all enqueue on ctx->stream
+void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& a);
+void launch_motion_flat(cleo_ctx* c, const MotionArgs& a);
 void launch_condensation(cleo_ctx* c, const CondArgs& a);
 void launch_qcond(cleo_ctx* c, const QcondArgs& a);
 void launch_collisions(cleo_ctx* c, const CollArgs& a);

@@ -179,13 +179,13 @@ def test_graph_replay_collisions_only(gpu):
 
 
 def test_tiles_do_not_change_results(gpu):
-    _say("case: waves-per-block tilings give identical results")
+    _say("case: waves-per-block tilings and thread-per-drop variants give identical results")
     pop = P.coupled(SEED, ndims=(8, 6, 4), mean=64)
     s = sdm_params(cond=cond_params(), coll=coll_params(), motion=motion_params())
     outs = []
-    for wpb in (1, 2, 4):
+    for wpb, flat in ((1, 0), (2, 0), (4, 0), (1, 1)):
         d = make_domain(gpu, pop)
-        t = abi.Tiles(wpb, wpb, wpb)
+        t = abi.Tiles(wpb, wpb, wpb, flat, flat)
         assert gpu.lib.cleo_set_tiles(d.h, abi.ctypes_byref(t)) == 0
         d.set_sdm(s)
         d.rng_counter(11)
