@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--cpu-ny", type=int, default=16, help="CPU sample: 128 x 128 x ny gridboxes")
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="copy one 1 GiB device buffer first (reference point for byte counters)")
     return ap.parse_args()
 
 
@@ -77,6 +79,11 @@ def run_gpu(args):
     b = lib.load()
     L = b.lib
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        dst = src.clone()
+        torch.cuda.synchronize()
+        del src, dst
     ndims = (args.nz, 128, 128)
     t_gen = time.perf_counter()
     arrs, st, halves = P.coupled_device(torch, dev, ndims=ndims)

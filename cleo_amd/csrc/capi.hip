@@ -63,7 +63,7 @@ void free_soa(SoA& s) {
 // synchronise, then report a recorded device error
 int sync_check(cleo_ctx* c) {
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  HIP_TRY(c, hipMemcpy(c->h_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, c->h_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
   if (c->h_ctl->err_code != 0) {
     c->msg = "device error " + std::to_string(c->h_ctl->err_code) + " at gridbox " +
              std::to_string(c->h_ctl->err_gbx) + " drop " + std::to_string(c->h_ctl->err_drop);
@@ -296,9 +296,9 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   for (int i = 0; i < 3; ++i) c->grid.per[i] = dom->periodic[i] ? 1 : 0;
   const uint64_t nh = nz + nx + ny + 3;
   HIP_TRY(c, dalloc(&c->d_halves, nh));
-  HIP_TRY(c, hipMemcpy(c->d_halves, dom->zhalf, (nz + 1) * 8, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->d_halves + nz + 1, dom->xhalf, (nx + 1) * 8, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(c->d_halves + nz + nx + 2, dom->yhalf, (ny + 1) * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, ctx_copy(c, c->d_halves, dom->zhalf, (nz + 1) * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, ctx_copy(c, c->d_halves + nz + 1, dom->xhalf, (nx + 1) * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, ctx_copy(c, c->d_halves + nz + nx + 2, dom->yhalf, (ny + 1) * 8, hipMemcpyHostToDevice));
   c->grid.zh = c->d_halves;
   c->grid.xh = c->d_halves + nz + 1;
   c->grid.yh = c->d_halves + nz + nx + 2;
@@ -325,11 +325,11 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   c->big_scratch_n = 2 * maxn;
   HIP_TRY(c, dalloc(&c->ctl, 1));
   HIP_TRY(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_ctl), sizeof(DevCtl), hipHostMallocDefault));
-  HIP_TRY(c, hipMemset(c->ctl, 0, sizeof(DevCtl)));
-  HIP_TRY(c, hipMemset(c->off, 0, (ngbx + 1) * 4));
-  HIP_TRY(c, hipMemset(c->off_alt, 0, (ngbx + 1) * 4));
-  HIP_TRY(c, hipMemset(c->nact, 0, ngbx * 4));
-  HIP_TRY(c, hipMemset(c->d_state, 0, 11 * ngbx * 8));
+  HIP_TRY(c, ctx_fill(c, c->ctl, 0, sizeof(DevCtl)));
+  HIP_TRY(c, ctx_fill(c, c->off, 0, (ngbx + 1) * 4));
+  HIP_TRY(c, ctx_fill(c, c->off_alt, 0, (ngbx + 1) * 4));
+  HIP_TRY(c, ctx_fill(c, c->nact, 0, ngbx * 4));
+  HIP_TRY(c, ctx_fill(c, c->d_state, 0, 11 * ngbx * 8));
   c->solute = Solute{2016.5 / dimless::RHO0, 0.05844277 / dimless::MR0, 2.0};
   HIP_TRY(c, mlp_create(c));
   *out = c;
@@ -374,14 +374,14 @@ int cleo_upload_supers(cleo_ctx* c, const cleo_supers_soa* h, uint64_t ntot) {
   if (ntot > c->maxn) return fail(c, CLEO_E_ARG, "ntot exceeds the context capacity");
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   SoA& d = c->buf[c->cur];
-  HIP_TRY(c, hipMemcpy(d.gbx, h->sdgbxindex, ntot * 4, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(d.c3, h->coord3, ntot * 8, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(d.c1, h->coord1, ntot * 8, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(d.c2, h->coord2, ntot * 8, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(d.xi, h->xi, ntot * 8, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(d.r, h->radius, ntot * 8, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(d.msol, h->msol, ntot * 8, hipMemcpyHostToDevice));
-  HIP_TRY(c, hipMemcpy(d.id, h->sdid, ntot * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, ctx_copy(c, d.gbx, h->sdgbxindex, ntot * 4, hipMemcpyHostToDevice));
+  HIP_TRY(c, ctx_copy(c, d.c3, h->coord3, ntot * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, ctx_copy(c, d.c1, h->coord1, ntot * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, ctx_copy(c, d.c2, h->coord2, ntot * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, ctx_copy(c, d.xi, h->xi, ntot * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, ctx_copy(c, d.r, h->radius, ntot * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, ctx_copy(c, d.msol, h->msol, ntot * 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, ctx_copy(c, d.id, h->sdid, ntot * 8, hipMemcpyHostToDevice));
   c->ntot = ntot;
   const int rc = ensure_radix(c, ntot);
   if (rc != CLEO_OK) return rc;
@@ -389,7 +389,7 @@ int cleo_upload_supers(cleo_ctx* c, const cleo_supers_soa* h, uint64_t ntot) {
   HIP_TRY(c, hipGetLastError());
   const int rs = sync_check(c);
   // the upload's own sort is the reference's constructor sort, not a step sort
-  HIP_TRY(c, hipMemset(&c->ctl->stats[ST_SORTS], 0, 8));
+  HIP_TRY(c, ctx_fill(c, &c->ctl->stats[ST_SORTS], 0, 8));
   return rs;
 }
 
@@ -398,21 +398,21 @@ int cleo_upload_supers_device(cleo_ctx* c, const cleo_supers_soa* h, uint64_t nt
   if (ntot > c->maxn) return fail(c, CLEO_E_ARG, "ntot exceeds the context capacity");
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   SoA& d = c->buf[c->cur];
-  HIP_TRY(c, hipMemcpy(d.gbx, h->sdgbxindex, ntot * 4, hipMemcpyDeviceToDevice));
-  HIP_TRY(c, hipMemcpy(d.c3, h->coord3, ntot * 8, hipMemcpyDeviceToDevice));
-  HIP_TRY(c, hipMemcpy(d.c1, h->coord1, ntot * 8, hipMemcpyDeviceToDevice));
-  HIP_TRY(c, hipMemcpy(d.c2, h->coord2, ntot * 8, hipMemcpyDeviceToDevice));
-  HIP_TRY(c, hipMemcpy(d.xi, h->xi, ntot * 8, hipMemcpyDeviceToDevice));
-  HIP_TRY(c, hipMemcpy(d.r, h->radius, ntot * 8, hipMemcpyDeviceToDevice));
-  HIP_TRY(c, hipMemcpy(d.msol, h->msol, ntot * 8, hipMemcpyDeviceToDevice));
-  HIP_TRY(c, hipMemcpy(d.id, h->sdid, ntot * 8, hipMemcpyDeviceToDevice));
+  HIP_TRY(c, ctx_copy(c, d.gbx, h->sdgbxindex, ntot * 4, hipMemcpyDeviceToDevice));
+  HIP_TRY(c, ctx_copy(c, d.c3, h->coord3, ntot * 8, hipMemcpyDeviceToDevice));
+  HIP_TRY(c, ctx_copy(c, d.c1, h->coord1, ntot * 8, hipMemcpyDeviceToDevice));
+  HIP_TRY(c, ctx_copy(c, d.c2, h->coord2, ntot * 8, hipMemcpyDeviceToDevice));
+  HIP_TRY(c, ctx_copy(c, d.xi, h->xi, ntot * 8, hipMemcpyDeviceToDevice));
+  HIP_TRY(c, ctx_copy(c, d.r, h->radius, ntot * 8, hipMemcpyDeviceToDevice));
+  HIP_TRY(c, ctx_copy(c, d.msol, h->msol, ntot * 8, hipMemcpyDeviceToDevice));
+  HIP_TRY(c, ctx_copy(c, d.id, h->sdid, ntot * 8, hipMemcpyDeviceToDevice));
   c->ntot = ntot;
   const int rc = ensure_radix(c, ntot);
   if (rc != CLEO_OK) return rc;
   generic_sort(c);
   HIP_TRY(c, hipGetLastError());
   const int rs = sync_check(c);
-  HIP_TRY(c, hipMemset(&c->ctl->stats[ST_SORTS], 0, 8));
+  HIP_TRY(c, ctx_fill(c, &c->ctl->stats[ST_SORTS], 0, 8));
   return rs;
 }
 
@@ -443,16 +443,16 @@ int cleo_download_supers(cleo_ctx* c, cleo_supers_soa* h, uint64_t* ntot, uint64
   const int rc = sync_check(c);
   const uint64_t n = c->ntot;
   const SoA& d = c->buf[c->cur];
-  HIP_TRY(c, hipMemcpy(h->sdgbxindex, d.gbx, n * 4, hipMemcpyDeviceToHost));
-  HIP_TRY(c, hipMemcpy(h->coord3, d.c3, n * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(c, hipMemcpy(h->coord1, d.c1, n * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(c, hipMemcpy(h->coord2, d.c2, n * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(c, hipMemcpy(h->xi, d.xi, n * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(c, hipMemcpy(h->radius, d.r, n * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(c, hipMemcpy(h->msol, d.msol, n * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(c, hipMemcpy(h->sdid, d.id, n * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, h->sdgbxindex, d.gbx, n * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, h->coord3, d.c3, n * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, h->coord1, d.c1, n * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, h->coord2, d.c2, n * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, h->xi, d.xi, n * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, h->radius, d.r, n * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, h->msol, d.msol, n * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, h->sdid, d.id, n * 8, hipMemcpyDeviceToHost));
   uint32_t nd = 0;
-  HIP_TRY(c, hipMemcpy(&nd, c->off + c->grid.ngbx, 4, hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, &nd, c->off + c->grid.ngbx, 4, hipMemcpyDeviceToHost));
   if (ntot) *ntot = n;
   if (ndomain) *ndomain = nd;
   return rc;
@@ -484,7 +484,7 @@ int cleo_download_supers_aos(cleo_ctx* c, void* aos, uint64_t* ntot, uint64_t* n
 int cleo_download_refs(cleo_ctx* c, uint32_t* refs) {
   if (!c || !refs) return CLEO_E_ARG;
   const int rc = sync_check(c);
-  HIP_TRY(c, hipMemcpy(refs, c->off, (c->grid.ngbx + 1ull) * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, refs, c->off, (c->grid.ngbx + 1ull) * 4, hipMemcpyDeviceToHost));
   return rc;
 }
 
@@ -504,7 +504,7 @@ int cleo_upload_gbx_state(cleo_ctx* c, const cleo_gbx_state* h) {
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   for (int i = 0; i < 11; ++i) {
     if (!src[i]) return fail(c, CLEO_E_ARG, "null state field");
-    HIP_TRY(c, hipMemcpy(f[i], src[i], c->grid.ngbx * 8ull, hipMemcpyHostToDevice));
+    HIP_TRY(c, ctx_copy(c, f[i], src[i], c->grid.ngbx * 8ull, hipMemcpyHostToDevice));
   }
   return CLEO_OK;
 }
@@ -517,7 +517,7 @@ int cleo_download_gbx_state(cleo_ctx* c, cleo_gbx_state* h) {
   double* dst[11] = {h->volume, h->press, h->temp, h->qvap, h->qcond, h->wvel_lo,
                      h->wvel_hi, h->uvel_lo, h->uvel_hi, h->vvel_lo, h->vvel_hi};
   for (int i = 0; i < 11; ++i)
-    if (dst[i]) HIP_TRY(c, hipMemcpy(dst[i], f[i], c->grid.ngbx * 8ull, hipMemcpyDeviceToHost));
+    if (dst[i]) HIP_TRY(c, ctx_copy(c, dst[i], f[i], c->grid.ngbx * 8ull, hipMemcpyDeviceToHost));
   return rc;
 }
 
@@ -532,7 +532,7 @@ int cleo_set_rng_counter(cleo_ctx* c, uint64_t seed, uint64_t first_event) {
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->rng_mode = 1;
   c->seed = seed;
-  HIP_TRY(c, hipMemcpy(&c->ctl->event, &first_event, 8, hipMemcpyHostToDevice));
+  HIP_TRY(c, ctx_copy(c, &c->ctl->event, &first_event, 8, hipMemcpyHostToDevice));
   return CLEO_OK;
 }
 
@@ -551,8 +551,8 @@ int cleo_set_rng_injected(cleo_ctx* c, const uint32_t* j, uint64_t nj, const dou
   }
   if (!c->inj_j) HIP_TRY(c, dalloc(&c->inj_j, 1));
   if (!c->inj_phi) HIP_TRY(c, dalloc(&c->inj_phi, 1));
-  if (nj) HIP_TRY(c, hipMemcpy(c->inj_j, j, nj * 4, hipMemcpyHostToDevice));
-  if (nphi) HIP_TRY(c, hipMemcpy(c->inj_phi, phi, nphi * 8, hipMemcpyHostToDevice));
+  if (nj) HIP_TRY(c, ctx_copy(c, c->inj_j, j, nj * 4, hipMemcpyHostToDevice));
+  if (nphi) HIP_TRY(c, ctx_copy(c, c->inj_phi, phi, nphi * 8, hipMemcpyHostToDevice));
   c->inj_nj = nj;
   c->inj_nphi = nphi;
   c->rng_mode = 2;
@@ -584,10 +584,10 @@ int cleo_motion(cleo_ctx* c, const cleo_motion_params* p) {
 
 static int set_substep_error(cleo_ctx* c) {
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  HIP_TRY(c, hipMemcpy(c->h_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, c->h_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
   if (c->h_ctl->err_code == 0) {
     int32_t code = CLEO_ERR_SUBSTEP;
-    HIP_TRY(c, hipMemcpy(&c->ctl->err_code, &code, 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, ctx_copy(c, &c->ctl->err_code, &code, 4, hipMemcpyHostToDevice));
   }
   c->msg = "timestep must be at least as large as the minimum sub-timestep";
   return CLEO_E_DEVICE;
@@ -692,7 +692,7 @@ int cleo_ndomain(cleo_ctx* c, uint64_t* nd) {
   if (!c || !nd) return CLEO_E_ARG;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   uint32_t v = 0;
-  HIP_TRY(c, hipMemcpy(&v, c->off + c->grid.ngbx, 4, hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, &v, c->off + c->grid.ngbx, 4, hipMemcpyDeviceToHost));
   *nd = v;
   return CLEO_OK;
 }
@@ -700,7 +700,7 @@ int cleo_ndomain(cleo_ctx* c, uint64_t* nd) {
 int cleo_get_error(cleo_ctx* c, cleo_error* err) {
   if (!c || !err) return CLEO_E_ARG;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  HIP_TRY(c, hipMemcpy(c->h_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, c->h_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
   err->code = c->h_ctl->err_code;
   err->gbxindex = c->h_ctl->err_gbx;
   err->drop = c->h_ctl->err_drop;
@@ -710,14 +710,14 @@ int cleo_get_error(cleo_ctx* c, cleo_error* err) {
 int cleo_clear_error(cleo_ctx* c) {
   if (!c) return CLEO_E_ARG;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  HIP_TRY(c, hipMemset(c->ctl, 0, 16));
+  HIP_TRY(c, ctx_fill(c, c->ctl, 0, 16));
   return CLEO_OK;
 }
 
 int cleo_get_stats(cleo_ctx* c, cleo_stats* s) {
   if (!c || !s) return CLEO_E_ARG;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  HIP_TRY(c, hipMemcpy(c->h_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, c->h_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
   const unsigned long long* v = c->h_ctl->stats;
   s->cond_drops = v[ST_COND_DROPS];
   s->newton_iters = v[ST_NEWTON_ITERS];
@@ -734,7 +734,7 @@ int cleo_get_stats(cleo_ctx* c, cleo_stats* s) {
 int cleo_reset_stats(cleo_ctx* c) {
   if (!c) return CLEO_E_ARG;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  HIP_TRY(c, hipMemset(c->ctl->stats, 0, sizeof(c->ctl->stats)));
+  HIP_TRY(c, ctx_fill(c, c->ctl->stats, 0, sizeof(c->ctl->stats)));
   return CLEO_OK;
 }
 
@@ -745,7 +745,7 @@ int cleo_record_niters(cleo_ctx* c, int32_t enable) {
   if (c->rec_niters && !c->niters) {
     HIP_TRY(c, dalloc(&c->niters, c->maxn));
   }
-  if (c->rec_niters) HIP_TRY(c, hipMemset(c->niters, 0, c->maxn * 4));
+  if (c->rec_niters) HIP_TRY(c, ctx_fill(c, c->niters, 0, c->maxn * 4));
   return CLEO_OK;
 }
 
@@ -753,7 +753,7 @@ int cleo_download_niters(cleo_ctx* c, uint32_t* out, uint64_t n) {
   if (!c || !out) return CLEO_E_ARG;
   if (!c->niters) return fail(c, CLEO_E_STATE, "niters recording not enabled");
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  HIP_TRY(c, hipMemcpy(out, c->niters, std::min<uint64_t>(n, c->maxn) * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, out, c->niters, std::min<uint64_t>(n, c->maxn) * 4, hipMemcpyDeviceToHost));
   return CLEO_OK;
 }
 
@@ -769,7 +769,7 @@ int cleo_download_guess(cleo_ctx* c, double* out, uint64_t n) {
   if (!c || !out) return CLEO_E_ARG;
   if (!c->guess) return fail(c, CLEO_E_STATE, "guess recording not enabled");
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  HIP_TRY(c, hipMemcpy(out, c->guess, std::min<uint64_t>(n, c->maxn) * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, out, c->guess, std::min<uint64_t>(n, c->maxn) * 8, hipMemcpyDeviceToHost));
   return CLEO_OK;
 }
 
@@ -820,7 +820,7 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
   const cleo_motion_params mp = c->sdm_set ? c->sdm.motion : cleo_motion_params{100, CLEO_TV_SIMMEL};
   const uint64_t seed = c->seed;
   DevCtl saved_ctl;
-  HIP_TRY(c, hipMemcpy(&saved_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, &saved_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
   float best[3] = {1e30f, 1e30f, 1e30f};
   uint32_t pick[3] = {1, 1, 1};
   uint32_t pick_flat[3] = {0, 0, 0};
@@ -901,7 +901,7 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
   hipStreamSynchronize(c->stream);
   hipFree(saved);
   hipFree(saved_nact);
-  HIP_TRY(c, hipMemcpy(c->ctl, &saved_ctl, sizeof(DevCtl), hipMemcpyHostToDevice));
+  HIP_TRY(c, ctx_copy(c, c->ctl, &saved_ctl, sizeof(DevCtl), hipMemcpyHostToDevice));
   if (rc != CLEO_OK) return rc;
   c->tiles = cleo_tiles{pick[0], pick[1], pick[2], pick_flat[0], pick_flat[2]};
   if (chosen) *chosen = c->tiles;

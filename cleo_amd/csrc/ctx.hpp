@@ -106,3 +106,16 @@ struct cleo_ctx {
   hipEvent_t ev[16] = {};
   std::string msg;
 };
+// Copies and fills of a context run in order on its stream and have completed when they return.
+// The stream is created non-blocking, so null-stream work (a plain hipMemcpy or hipMemset) would
+// not be ordered with the context's kernels, and a device-to-device hipMemcpy or a device
+// hipMemset may return before it has completed.
+inline hipError_t ctx_copy(cleo_ctx* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+  return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
+}
+inline hipError_t ctx_fill(cleo_ctx* c, void* dst, int value, size_t bytes) {
+  const hipError_t e = hipMemsetAsync(dst, value, bytes, c->stream);
+  return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
+}
+

@@ -579,7 +579,7 @@ int cleo_mlp_set_weights(cleo_ctx* c, const float* blob) {
     if (!(blob[P_XSTD + j] > 0.0f)) return CLEO_E_ARG;
   if (!(blob[P_YSTD] > 0.0f)) return CLEO_E_ARG;
   MLP_TRY(c, hipStreamSynchronize(c->stream));
-  MLP_TRY(c, hipMemcpy(c->mlp->master, blob, NBLOB * 4, hipMemcpyHostToDevice));
+  MLP_TRY(c, ctx_copy(c, c->mlp->master, blob, NBLOB * 4, hipMemcpyHostToDevice));
   launch_pack(c);
   MLP_TRY(c, hipGetLastError());
   MLP_TRY(c, hipStreamSynchronize(c->stream));
@@ -590,7 +590,7 @@ int cleo_mlp_set_weights(cleo_ctx* c, const float* blob) {
 int cleo_mlp_get_weights(cleo_ctx* c, float* blob) {
   if (!c || !blob) return CLEO_E_ARG;
   MLP_TRY(c, hipStreamSynchronize(c->stream));
-  MLP_TRY(c, hipMemcpy(blob, c->mlp->master, NBLOB * 4, hipMemcpyDeviceToHost));
+  MLP_TRY(c, ctx_copy(c, blob, c->mlp->master, NBLOB * 4, hipMemcpyDeviceToHost));
   return CLEO_OK;
 }
 
@@ -669,9 +669,9 @@ int cleo_mlp_loss_grad(cleo_ctx* c, const float* x, const float* y, uint64_t n, 
   MLP_TRY(c, hipGetLastError());
   MLP_TRY(c, hipStreamSynchronize(c->stream));
   float l = 0.f;
-  MLP_TRY(c, hipMemcpy(&l, m->loss, 4, hipMemcpyDeviceToHost));
+  MLP_TRY(c, ctx_copy(c, &l, m->loss, 4, hipMemcpyDeviceToHost));
   if (loss) *loss = l / static_cast<float>(n);
-  if (grad_host) MLP_TRY(c, hipMemcpy(grad_host, m->grad, NPARAM * 4, hipMemcpyDeviceToHost));
+  if (grad_host) MLP_TRY(c, ctx_copy(c, grad_host, m->grad, NPARAM * 4, hipMemcpyDeviceToHost));
   return CLEO_OK;
 }
 
@@ -707,7 +707,7 @@ int cleo_mlp_train(cleo_ctx* c, const float* x, const float* y, uint64_t n, cons
     for (int j = 0; j < 14; ++j) mom[j] += hp[b * 14 + j];
   std::vector<float> blob(NBLOB, 0.0f);
   const bool fresh = !(m->ready && tp->seed == 0);
-  if (!fresh) MLP_TRY(c, hipMemcpy(blob.data(), m->master, NBLOB * 4, hipMemcpyDeviceToHost));
+  if (!fresh) MLP_TRY(c, ctx_copy(c, blob.data(), m->master, NBLOB * 4, hipMemcpyDeviceToHost));
   if (fresh) {
     for (int j = 0; j < kIn; ++j) {
       const double mu = mom[j] / n;
@@ -735,9 +735,9 @@ int cleo_mlp_train(cleo_ctx* c, const float* x, const float* y, uint64_t n, cons
     fill(P_W2, kHid * kHid, kHid);
     fill(P_W3, kHid, kHid);
   }
-  MLP_TRY(c, hipMemcpy(m->master, blob.data(), NBLOB * 4, hipMemcpyHostToDevice));
-  MLP_TRY(c, hipMemset(m->adam_m, 0, NBLOB * 4));
-  MLP_TRY(c, hipMemset(m->adam_v, 0, NBLOB * 4));
+  MLP_TRY(c, ctx_copy(c, m->master, blob.data(), NBLOB * 4, hipMemcpyHostToDevice));
+  MLP_TRY(c, ctx_fill(c, m->adam_m, 0, NBLOB * 4));
+  MLP_TRY(c, ctx_fill(c, m->adam_v, 0, NBLOB * 4));
   launch_pack(c);
   m->ready = true;
   // sample order: one seeded permutation, batches visited in a seeded order each epoch
@@ -753,7 +753,7 @@ int cleo_mlp_train(cleo_ctx* c, const float* x, const float* y, uint64_t n, cons
   for (uint64_t i = n - 1; i > 0; --i) std::swap(perm[i], perm[next() % (i + 1)]);
   uint32_t* dperm = nullptr;
   MLP_TRY(c, hipMalloc(reinterpret_cast<void**>(&dperm), n * 4));
-  MLP_TRY(c, hipMemcpy(dperm, perm.data(), n * 4, hipMemcpyHostToDevice));
+  MLP_TRY(c, ctx_copy(c, dperm, perm.data(), n * 4, hipMemcpyHostToDevice));
   const uint64_t nbatch = n / tp->batch ? n / tp->batch : 1;
   const uint32_t bsz = static_cast<uint32_t>(std::min<uint64_t>(tp->batch, n));
   const uint32_t waves = (bsz + 63) / 64;
@@ -782,7 +782,7 @@ int cleo_mlp_train(cleo_ctx* c, const float* x, const float* y, uint64_t n, cons
   if (rc != CLEO_OK) return rc;
   if (final_loss) {
     float l = 0.0f;
-    MLP_TRY(c, hipMemcpy(&l, m->loss, 4, hipMemcpyDeviceToHost));
+    MLP_TRY(c, ctx_copy(c, &l, m->loss, 4, hipMemcpyDeviceToHost));
     *final_loss = l / static_cast<float>(bsz);
   }
   return CLEO_OK;

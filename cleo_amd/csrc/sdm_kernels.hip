@@ -133,26 +133,12 @@ __global__ void __launch_bounds__(256) k_qcond(QcondArgs a) {
 // Fisher-Yates on a u32 index permutation in LDS (one lane walks the n-1 swaps, all lanes draw
 // the indices first), pairs (2jj, 2jj+1) one per lane, enactment, then the drops are written to
 // `dst` in shuffled order with nulls moved behind the active drops (stable partition).
-__global__ void __launch_bounds__(256) k_collisions(CollArgs a) {
-  extern __shared__ uint32_t lds_dyn[];
-  const int lane = lane_id();
-  const int w = static_cast<int>(threadIdx.x >> 6);
-  const uint32_t g = blockIdx.x * a.wpb + w;
-  if (g >= a.ngbx) return;
-  const uint32_t base = a.off[g];
-  const uint32_t nfull = a.off[g + 1] - base;
-  const uint32_t n = a.nact[g];
+// The permutation `perm` and the drawn indices `jb` live in LDS (kLds) or, for gridboxes above
+// kCollLdsCap drops, in HBM scratch; the two instantiations keep LDS accesses as ds_* instructions.
+template <bool kLds>
+__device__ __forceinline__ void coll_gbx(const CollArgs& a, int lane, uint32_t g, uint32_t base, uint32_t nfull,
+                                         uint32_t n, uint32_t* perm, uint32_t* jb) {
   const uint64_t event = a.ctl->event;
-
-  uint32_t* perm;
-  uint32_t* jb;
-  if (n <= kCollLdsCap) {
-    perm = lds_dyn + static_cast<uint32_t>(w) * 2u * kCollLdsCap;
-    jb = perm + kCollLdsCap;
-  } else {
-    perm = a.big + 2ull * base;
-    jb = perm + n;
-  }
   for (uint32_t k = lane; k < n; k += kWave) perm[k] = k;
   if (n >= 2) {
     for (uint32_t t = lane; t < n - 1; t += kWave) {
@@ -171,7 +157,7 @@ __global__ void __launch_bounds__(256) k_collisions(CollArgs a) {
       }
       jb[t] = j;
     }
-    if (n > kCollLdsCap) __threadfence_block();
+    if (!kLds) __threadfence_block();
     wave_sync();
     if (lane == 0) {  // fisher_yates_shuffle (shuffle.hpp:80-87)
       for (uint32_t t = 0; t + 1 < n; ++t) {
@@ -182,7 +168,7 @@ __global__ void __launch_bounds__(256) k_collisions(CollArgs a) {
         perm[j] = pi;
       }
     }
-    if (n > kCollLdsCap) __threadfence_block();
+    if (!kLds) __threadfence_block();
     wave_sync();
   }
 
@@ -308,6 +294,24 @@ __global__ void __launch_bounds__(256) k_collisions(CollArgs a) {
     add_stat(a.ctl, ST_COLL_PAIRS, npairs);
     add_stat(a.ctl, ST_COLL_EVENTS, events);
     add_stat(a.ctl, ST_NULLS, nulls);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_collisions(CollArgs a) {
+  extern __shared__ uint32_t lds_dyn[];
+  const int lane = lane_id();
+  const int w = static_cast<int>(threadIdx.x >> 6);
+  const uint32_t g = blockIdx.x * a.wpb + w;
+  if (g >= a.ngbx) return;
+  const uint32_t base = a.off[g];
+  const uint32_t nfull = a.off[g + 1] - base;
+  const uint32_t n = a.nact[g];
+  if (n <= kCollLdsCap) {
+    uint32_t* perm = lds_dyn + static_cast<uint32_t>(w) * 2u * kCollLdsCap;
+    coll_gbx<true>(a, lane, g, base, nfull, n, perm, perm + kCollLdsCap);
+  } else {
+    uint32_t* perm = a.big + 2ull * base;
+    coll_gbx<false>(a, lane, g, base, nfull, n, perm, perm + n);
   }
 }
 

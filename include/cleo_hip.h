@@ -197,9 +197,10 @@ int cleo_device_info(char* buf, size_t len);
 int cleo_upload_supers(cleo_ctx* ctx, const cleo_supers_soa* host, uint64_t ntot);
 /* Same from the reference's 72-byte Superdrop layout (superdrop.hpp:36-60). */
 int cleo_upload_supers_aos(cleo_ctx* ctx, const void* aos72, uint64_t ntot);
-/* Whole store in reference order: in-domain drops sorted by gridbox, then out-of-domain. */
-/* Same from device-resident SoA arrays (any device pointers on the context's GPU). */
+/* Same from device-resident SoA arrays (any device pointers on the context's GPU). The
+   caller finishes the work that produced them (e.g. synchronises its own stream) first. */
 int cleo_upload_supers_device(cleo_ctx* ctx, const cleo_supers_soa* dev, uint64_t ntot);
+/* Whole store in reference order: in-domain drops sorted by gridbox, then out-of-domain. */
 int cleo_download_supers(cleo_ctx* ctx, cleo_supers_soa* host, uint64_t* ntot, uint64_t* ndomain);
 int cleo_download_supers_aos(cleo_ctx* ctx, void* aos72, uint64_t* ntot, uint64_t* ndomain);
 /* refs: ngbx+1 entries, gridbox g holds [refs[g], refs[g+1]) (supersingbx.hpp:32-102). */

@@ -194,3 +194,19 @@ def test_tiles_do_not_change_results(gpu):
         outs.append(d.download())
     for o in outs[1:]:
         assert_supers_match(outs[0], o, exact_fp=True, what="tiles")
+
+
+def test_multiblock_scans_coupled(gpu, oracle):
+    _say("case: 4096 gridboxes (multi-block scans), 2 coupled steps")
+    pop = P.coupled(SEED, ndims=(16, 16, 16), mean=64)
+    g, o = _both(gpu, oracle, pop)
+    s = sdm_params(cond=cond_params(), coll=coll_params(), motion=motion_params())
+    for d in (g, o):
+        d.set_sdm(s)
+        d.rng_counter(31)
+    for step in range(2):
+        assert g.run_step(step * 100, (step + 1) * 100) == 0
+        assert o.run_step(step * 100, (step + 1) * 100) == 0
+        assert g.error()[0] == 0 and o.error()[0] == 0, (g.error(), o.error())
+    assert_supers_match(g.download(), o.download(), what="multiblock")
+    assert np.array_equal(g.refs(), o.refs())
