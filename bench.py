@@ -138,7 +138,7 @@ def run_gpu(args):
                     iters_per_drop=stats["newton_iters"] / max(stats["cond_drops"], 1))
 
     d = new_domain()
-    tiles = abi.Tiles(0, 0, 0)
+    tiles = abi.Tiles()
     if not args.no_autotune:
         d.check(L.cleo_autotune(d.h, C.byref(tiles)), "autotune")
     t = 0
@@ -285,8 +285,7 @@ def main():
                         f"Poisson(64) drops/gridbox, motion+sort+condensation+Long coal/breakup, dt=1 s, hipGraph",
             "ndrops": g["n"],
             "ngbx": g["ngbx"],
-            "tiles": [g["tiles"].cond_waves_per_block, g["tiles"].coll_waves_per_block,
-                      g["tiles"].motion_waves_per_block],
+            "tiles": {f: int(getattr(g["tiles"], f)) for f, _ in g["tiles"]._fields_},
         },
         "newton_iters_per_drop": iters_plain,
         "wall_s": p["wall_s"],

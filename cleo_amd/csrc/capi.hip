@@ -322,6 +322,9 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   HIP_TRY(c, dalloc(&c->big_scratch, 2 * maxn));
   HIP_TRY(c, dalloc(&c->gs, 2 * ngbx));
   HIP_TRY(c, dalloc(&c->dm, maxn));
+  // a gridbox launch has at most ngbx + 3 waves, a drop launch 4 per 256 drops
+  c->sp_slots = std::max<uint64_t>(ngbx + 4, 4 * ((maxn + 255) / 256) + 4);
+  HIP_TRY(c, dalloc(&c->sp, kStatSlotsMax * c->sp_slots));
   c->big_scratch_n = 2 * maxn;
   HIP_TRY(c, dalloc(&c->ctl, 1));
   HIP_TRY(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_ctl), sizeof(DevCtl), hipHostMallocDefault));
@@ -349,7 +352,8 @@ int cleo_destroy(cleo_ctx* c) {
                   static_cast<void*>(c->ctl), static_cast<void*>(c->inj_j), static_cast<void*>(c->inj_phi),
                   static_cast<void*>(c->niters), static_cast<void*>(c->guess), static_cast<void*>(c->rs_key[0]),
                   static_cast<void*>(c->rs_key[1]), static_cast<void*>(c->rs_idx[0]), static_cast<void*>(c->rs_idx[1]),
-                  static_cast<void*>(c->rs_cnt), static_cast<void*>(c->gs), static_cast<void*>(c->dm)})
+                  static_cast<void*>(c->rs_cnt), static_cast<void*>(c->gs), static_cast<void*>(c->dm),
+                  static_cast<void*>(c->sp)})
     if (p) hipFree(p);
   if (c->h_ctl) hipHostFree(c->h_ctl);
   for (hipEvent_t& e : c->ev)
@@ -782,8 +786,9 @@ int cleo_set_tiles(cleo_ctx* c, const cleo_tiles* t) {
   return CLEO_OK;
 }
 
-// Times each kernel at 1, 2 and 4 waves (gridboxes) per workgroup on the resident population and
-// keeps the fastest.  Drops and State are left unchanged: the kernels run on a copy in the spare
+// Times each kernel at 1, 2 and 4 waves (gridboxes) per workgroup, the thread-per-drop variant,
+// and grids of one wave per gridbox or of 8192 / 32768 looping waves, on the resident population,
+// and keeps the fastest.  Drops and State are left unchanged: the kernels run on a copy in the spare
 // buffer, State fields that condensation writes are saved and restored.
 int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
   if (!c) return CLEO_E_ARG;
@@ -824,14 +829,20 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
   float best[3] = {1e30f, 1e30f, 1e30f};
   uint32_t pick[3] = {1, 1, 1};
   uint32_t pick_flat[3] = {0, 0, 0};
+  uint32_t pick_gw[3] = {0, 0, 0};
+  const cleo_tiles tiles0 = c->tiles;
   int rc = CLEO_OK;
   struct Cand {
     uint32_t wpb, flat;
   };
   const Cand cands[4] = {{1, 0}, {2, 0}, {4, 0}, {1, 1}};
+  const uint32_t grid_waves[3] = {0, 8192, 32768};
   for (int kern = 0; kern < 3 && rc == CLEO_OK; ++kern) {
-    for (const Cand& cd : cands) {
+    for (int gi = 0; gi < 3 * 4; ++gi) {
+      const Cand& cd = cands[gi % 4];
+      const uint32_t gw = grid_waves[gi / 4];
       if (kern == 1 && cd.flat) continue;  // collisions are gridbox-shaped only
+      c->tiles.cond_grid_waves = c->tiles.coll_grid_waves = c->tiles.motion_grid_waves = gw;
       float tmin = 1e30f;
       for (int rep = 0; rep < 2; ++rep) {
         if (copy_store() != hipSuccess) { rc = fail(c, CLEO_E_HIP, "autotune copy"); break; }
@@ -895,15 +906,17 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
         best[kern] = tmin;
         pick[kern] = cd.wpb;
         pick_flat[kern] = cd.flat;
+        pick_gw[kern] = gw;
       }
     }
   }
+  c->tiles = tiles0;
   hipStreamSynchronize(c->stream);
   hipFree(saved);
   hipFree(saved_nact);
   HIP_TRY(c, ctx_copy(c, c->ctl, &saved_ctl, sizeof(DevCtl), hipMemcpyHostToDevice));
   if (rc != CLEO_OK) return rc;
-  c->tiles = cleo_tiles{pick[0], pick[1], pick[2], pick_flat[0], pick_flat[2]};
+  c->tiles = cleo_tiles{pick[0], pick[1], pick[2], pick_flat[0], pick_flat[2], pick_gw[0], pick_gw[1], pick_gw[2]};
   if (chosen) *chosen = c->tiles;
   return CLEO_OK;
 }

@@ -44,6 +44,12 @@ struct GState {
   double* vhi;
 };
 
+// Per-wave statistics slots: counter j of the wave in slot i is sp[j * nslot + i].
+struct StatSlots {
+  uint32_t* sp;
+  uint32_t nslot;
+};
+
 struct MLPDev;  // mlp.hip
 
 }  // namespace cleo
@@ -91,9 +97,12 @@ struct cleo_ctx {
   // configuration
   cleo_sdm_params sdm{};
   bool sdm_set = false;
-  cleo_tiles tiles{1, 1, 1, 0, 0};
+  cleo_tiles tiles{1, 1, 1, 0, 0, 0, 0, 0};
   double* gs = nullptr;  // 2*ngbx: per-gridbox saturation ratio and diffusion factor
   double* dm = nullptr;  // maxn: per-drop condensed mass (flat condensation)
+  // per-wave statistics: up to kStatSlotsMax counters per wave, reduced into DevCtl::stats
+  uint32_t* sp = nullptr;
+  uint64_t sp_slots = 0;  // slots per counter: max(ngbx, waves of a thread-per-drop launch)
   // per-drop Newton iteration record
   bool rec_niters = false;
   uint32_t* niters = nullptr;

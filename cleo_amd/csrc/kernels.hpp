@@ -10,6 +10,10 @@ constexpr int kWave = 64;
 
 __device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & (kWave - 1)); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+// waves of a launch: gridbox kernels visit g = wave_index(), + wave_count(), ... so that a grid
+// smaller than one wave per gridbox amortises the dispatch of short waves
+__device__ __forceinline__ uint32_t wave_index() { return blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); }
+__device__ __forceinline__ uint32_t wave_count() { return gridDim.x * (blockDim.x >> 6); }
 
 // LDS / memory ordering between lanes of one wavefront (no workgroup barrier needed)
 __device__ __forceinline__ void wave_sync() {
@@ -45,8 +49,11 @@ __device__ __forceinline__ void set_error(DevCtl* ctl, int code, uint32_t gbx, u
   if (code) record_error(ctl, code, gbx, drop);
 }
 
-__device__ __forceinline__ void add_stat(DevCtl* ctl, int idx, unsigned long long v) {
-  if (v) atomicAdd(&ctl->stats[idx], v);
+// Statistics of a wave go to a slot of their own, sp[j * nslot + slot] for the kernel's j-th
+// counter; k_stats_reduce adds the slots into DevCtl::stats after the launch.  Same-address
+// device-scope atomics from every wave would serialise across the XCDs.
+__device__ __forceinline__ void put_stat(const StatSlots& s, int j, uint32_t slot, uint32_t v) {
+  s.sp[static_cast<uint64_t>(j) * s.nslot + slot] = v;
 }
 
 }  // namespace cleo

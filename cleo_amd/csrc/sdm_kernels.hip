@@ -1,9 +1,11 @@
 // sdm_kernels.hip -- gfx950 kernels of the super-droplet step.
 //
-// Launch shape: one wave64 per gridbox; `wpb` gridboxes (waves) per workgroup.  Drops of a
-// gridbox are contiguous in the SoA store (sorted by sdgbxindex), so every per-drop load and
-// store of a wave is a coalesced run.  No kernel uses atomics on drop data: sums keep the
+// Launch shape: one wave64 per gridbox, `wpb` waves per workgroup; with a capped grid each wave
+// loops over gridboxes g, g + (waves in the grid), ...  Drops of a gridbox are contiguous in the
+// SoA store (sorted by sdgbxindex), so every per-drop load and store of a wave is a coalesced run.  No kernel uses atomics on drop data: sums keep the
 // reference's sequential order and the sort is stable by construction.
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "sdm_kernels.hpp"
 
@@ -15,10 +17,7 @@ namespace cleo {
 // EffectOnHydrometeorStatesFunctor (movesupersindomain.hpp:95-136) is recomputed first, as
 // happens between motion and condensation in the reference step.
 template <bool kFuseQcond>
-__global__ void __launch_bounds__(256) k_condensation(CondArgs a) {
-  const int lane = lane_id();
-  const uint32_t g = blockIdx.x * a.wpb + (threadIdx.x >> 6);
-  if (g >= a.ngbx) return;
+__device__ __forceinline__ void cond_gbx(const CondArgs& a, uint32_t g, int lane, uint32_t (&acc)[4]) {
   const uint32_t base = a.off[g];
   const uint32_t nfull = a.off[g + 1] - base;
   const uint32_t n = a.nact[g];
@@ -98,18 +97,23 @@ __global__ void __launch_bounds__(256) k_condensation(CondArgs a) {
       a.st.qv[g] = qv;
     }
     if (a.alter || kFuseQcond) a.st.qc[g] = qc;
-    add_stat(a.ctl, ST_COND_DROPS, n);
-    add_stat(a.ctl, ST_NEWTON_ITERS, iters);
-    add_stat(a.ctl, ST_SUBSTEP_DROPS, subs);
-    add_stat(a.ctl, ST_WARM_RESTARTS, restarts);
   }
+  acc[0] += n;
+  acc[1] += iters;
+  acc[2] += subs;
+  acc[3] += restarts;
+}
+template <bool kFuseQcond>
+__global__ void __launch_bounds__(256) k_condensation(CondArgs a) {
+  const int lane = lane_id();
+  uint32_t acc[4] = {0, 0, 0, 0};
+  for (uint32_t g = wave_index(); g < a.ngbx; g += wave_count()) cond_gbx<kFuseQcond>(a, g, lane, acc);
+  if (lane == 0)
+    for (int j = 0; j < 4; ++j) put_stat(a.stat, j, wave_index(), acc[j]);
 }
 
 // EffectOnHydrometeorStatesFunctor (movesupersindomain.hpp:95-136), standalone
-__global__ void __launch_bounds__(256) k_qcond(QcondArgs a) {
-  const int lane = lane_id();
-  const uint32_t g = blockIdx.x * a.wpb + (threadIdx.x >> 6);
-  if (g >= a.ngbx) return;
+__device__ __forceinline__ void qcond_gbx(const QcondArgs& a, uint32_t g, int lane) {
   const uint32_t base = a.off[g];
   const uint32_t nfull = a.off[g + 1] - base;
   bool bad = false;
@@ -127,6 +131,10 @@ __global__ void __launch_bounds__(256) k_qcond(QcondArgs a) {
     a.st.qc[g] = rho_cond / rho_dry;
   }
 }
+__global__ void __launch_bounds__(256) k_qcond(QcondArgs a) {
+  const int lane = lane_id();
+  for (uint32_t g = wave_index(); g < a.ngbx; g += wave_count()) qcond_gbx(a, g, lane);
+}
 
 // ============================ collisions ======================================================
 // DoCollisions::do_collisions for one gridbox per wave (collisions.hpp:140-278):
@@ -137,7 +145,8 @@ __global__ void __launch_bounds__(256) k_qcond(QcondArgs a) {
 // kCollLdsCap drops, in HBM scratch; the two instantiations keep LDS accesses as ds_* instructions.
 template <bool kLds>
 __device__ __forceinline__ void coll_gbx(const CollArgs& a, int lane, uint32_t g, uint32_t base, uint32_t nfull,
-                                         uint32_t n, uint32_t* perm, uint32_t* jb) {
+                                         uint32_t n, uint32_t* perm, uint32_t* jb, uint32_t (&acc)[3]) {
+  wave_sync();  // the previous gridbox of this wave is done with perm / jb
   const uint64_t event = a.ctl->event;
   for (uint32_t k = lane; k < n; k += kWave) perm[k] = k;
   if (n >= 2) {
@@ -244,8 +253,10 @@ __device__ __forceinline__ void coll_gbx(const CollArgs& a, int lane, uint32_t g
   }
   if (nulls > 0 && !a.timing_only) {
     // remove_null_supers (collisions.hpp:240-248): the reference's sort_team is restated as a
-    // stable partition, through the now unused `src` range of this gridbox.
-    __threadfence();
+    // stable partition, through the now unused `src` range of this gridbox.  Only this wave
+    // reads back what it stored, so a workgroup-scope fence orders it (an agent-scope fence
+    // would write back the XCD's whole L2).
+    __threadfence_block();
     wave_sync();
     const uint32_t nn = n - nulls;
     uint32_t run_nn = 0, run_null = 0;
@@ -271,7 +282,7 @@ __device__ __forceinline__ void coll_gbx(const CollArgs& a, int lane, uint32_t g
       run_null += __popcll(mnull);
       run_nn += __popcll(mnn);
     }
-    __threadfence();
+    __threadfence_block();
     wave_sync();
     for (uint32_t k = lane; k < n; k += kWave) {
       const uint32_t kk = base + k;
@@ -289,30 +300,30 @@ __device__ __forceinline__ void coll_gbx(const CollArgs& a, int lane, uint32_t g
       a.ctl->resort_flag = 1u;
     }
   }
-  events = wave_sum_u32(events);
-  if (lane == 0) {
-    add_stat(a.ctl, ST_COLL_PAIRS, npairs);
-    add_stat(a.ctl, ST_COLL_EVENTS, events);
-    add_stat(a.ctl, ST_NULLS, nulls);
-  }
+  acc[0] += npairs;
+  acc[1] += wave_sum_u32(events);
+  acc[2] += nulls;
 }
 
 __global__ void __launch_bounds__(256) k_collisions(CollArgs a) {
   extern __shared__ uint32_t lds_dyn[];
   const int lane = lane_id();
   const int w = static_cast<int>(threadIdx.x >> 6);
-  const uint32_t g = blockIdx.x * a.wpb + w;
-  if (g >= a.ngbx) return;
-  const uint32_t base = a.off[g];
-  const uint32_t nfull = a.off[g + 1] - base;
-  const uint32_t n = a.nact[g];
-  if (n <= kCollLdsCap) {
-    uint32_t* perm = lds_dyn + static_cast<uint32_t>(w) * 2u * kCollLdsCap;
-    coll_gbx<true>(a, lane, g, base, nfull, n, perm, perm + kCollLdsCap);
-  } else {
-    uint32_t* perm = a.big + 2ull * base;
-    coll_gbx<false>(a, lane, g, base, nfull, n, perm, perm + n);
+  uint32_t acc[3] = {0, 0, 0};
+  for (uint32_t g = wave_index(); g < a.ngbx; g += wave_count()) {
+    const uint32_t base = a.off[g];
+    const uint32_t nfull = a.off[g + 1] - base;
+    const uint32_t n = a.nact[g];
+    if (n <= kCollLdsCap) {
+      uint32_t* perm = lds_dyn + static_cast<uint32_t>(w) * 2u * kCollLdsCap;
+      coll_gbx<true>(a, lane, g, base, nfull, n, perm, perm + kCollLdsCap, acc);
+    } else {
+      uint32_t* perm = a.big + 2ull * base;
+      coll_gbx<false>(a, lane, g, base, nfull, n, perm, perm + n, acc);
+    }
   }
+  if (lane == 0)
+    for (int j = 0; j < 3; ++j) put_stat(a.stat, j, wave_index(), acc[j]);
 }
 
 // per-gridbox sizes of the injected stream: n-1 indices and 2*floor(n/2) phis
@@ -325,6 +336,22 @@ __global__ void k_inj_sizes(const uint32_t* nact, uint32_t ngbx, uint32_t* nj, u
 }
 
 __global__ void k_advance_event(DevCtl* ctl) { ctl->event += 1; }
+
+// DevCtl::stats[first + blockIdx.y] += sum of the nslot slots of counter blockIdx.y; each block
+// sums a strided share and adds it with one atomic (a few hundred atomics per launch).
+__global__ void __launch_bounds__(256) k_stats_reduce(const uint32_t* sp, uint32_t nslot, int first, DevCtl* ctl) {
+  __shared__ unsigned long long part[4];
+  const uint32_t* col = sp + static_cast<uint64_t>(blockIdx.y) * nslot;
+  unsigned long long v = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nslot; i += gridDim.x * blockDim.x) v += col[i];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if (lane_id() == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+    if (t) atomicAdd(&ctl->stats[first + static_cast<int>(blockIdx.y)], t);
+  }
+}
 
 // ============================ motion ==========================================================
 __device__ __forceinline__ double interpolation(double blo, double bhi, double vlo, double vhi, double sd) {
@@ -373,10 +400,7 @@ __device__ __forceinline__ int axis_step(uint32_t& idx, double& coord, uint32_t 
 // MoveSupersInGridboxesFunctor (movesupersindomain.hpp:47-93) with PredCorrMotion over
 // CartesianMaps; also counts, per gridbox, how many drops go to each of the 27 neighbour
 // directions (and out of the domain) for the stable neighbour sort.
-__global__ void __launch_bounds__(256) k_motion(MotionArgs a) {
-  const int lane = lane_id();
-  const uint32_t g = blockIdx.x * a.wpb + (threadIdx.x >> 6);
-  if (g >= a.grid.ngbx) return;
+__device__ __forceinline__ uint32_t motion_gbx(const MotionArgs& a, uint32_t g, int lane) {
   const uint32_t base = a.off[g];
   const uint32_t nfull = a.off[g + 1] - base;
   const uint32_t nz = a.grid.nz, nx = a.grid.nx, ny = a.grid.ny;
@@ -452,7 +476,13 @@ __global__ void __launch_bounds__(256) k_motion(MotionArgs a) {
     }
   }
   if (lane < 28) a.cnt[28ull * g + lane] = mycount;
-  if (lane == 0) add_stat(a.ctl, ST_MOTION_DROPS, nfull);
+  return nfull;
+}
+__global__ void __launch_bounds__(256) k_motion(MotionArgs a) {
+  const int lane = lane_id();
+  uint32_t moved = 0;
+  for (uint32_t g = wave_index(); g < a.grid.ngbx; g += wave_count()) moved += motion_gbx(a, g, lane);
+  if (lane == 0) put_stat(a.stat, 0, wave_index(), moved);
 }
 
 // ============================ neighbour sort ==================================================
@@ -496,11 +526,7 @@ __global__ void k_nsort_tot(NsortArgs a) {
 
 // Stable scatter (sortsupers.hpp:63-76,167-193 under Serial): drop p of source gridbox g with
 // key d lands at  base[d] + (drops reaching d from gridboxes < g) + (earlier drops of g with key d).
-__global__ void __launch_bounds__(256) k_nsort_scatter(NsortArgs a) {
-  if (guard_off(a.flag)) return;
-  const int lane = lane_id();
-  const uint32_t g = blockIdx.x * a.wpb + (threadIdx.x >> 6);
-  if (g >= a.grid.ngbx) return;
+__device__ __forceinline__ void nsort_scatter_gbx(const NsortArgs& a, uint32_t g, int lane) {
   const uint32_t sbase = a.off[g];
   const uint32_t nfull = a.off[g + 1] - sbase;
   const uint32_t ngbx = a.grid.ngbx;
@@ -571,6 +597,11 @@ __global__ void __launch_bounds__(256) k_nsort_scatter(NsortArgs a) {
       a.dst.id[dest] = a.src.id[p];
     }
   }
+}
+__global__ void __launch_bounds__(256) k_nsort_scatter(NsortArgs a) {
+  if (guard_off(a.flag)) return;
+  const int lane = lane_id();
+  for (uint32_t g = wave_index(); g < a.grid.ngbx; g += wave_count()) nsort_scatter_gbx(a, g, lane);
 }
 
 // drops already out of the domain stay behind the newly out-of-domain ones, in order
@@ -803,6 +834,16 @@ static inline unsigned nblocks(uint64_t n, uint64_t per) {
   const uint64_t b = (n + per - 1) / per;
   return static_cast<unsigned>(b == 0 ? 1 : b);
 }
+// blocks of a gridbox loop: one wave per gridbox, capped at `grid_waves` waves when non-zero
+static inline unsigned gbx_blocks(uint64_t ngbx, uint32_t wpb, uint32_t grid_waves) {
+  const unsigned b = nblocks(ngbx, wpb);
+  return grid_waves ? std::min<unsigned>(b, std::max<unsigned>(1u, grid_waves / wpb)) : b;
+}
+// blocks of 256 threads of a drop loop: one thread per drop, capped at `grid_waves` waves
+static inline unsigned drop_blocks(uint64_t n, uint32_t grid_waves) {
+  const unsigned b = nblocks(n, 256);
+  return grid_waves ? std::min<unsigned>(b, std::max<unsigned>(1u, grid_waves / 4)) : b;
+}
 static inline unsigned stride_blocks(uint64_t n) {
   const uint64_t b = (n + 255) / 256;
   return static_cast<unsigned>(b == 0 ? 1 : (b > 65536 ? 65536 : b));
@@ -813,26 +854,46 @@ __global__ void k_set_u32(uint32_t* out, uint32_t v, const uint32_t* flag) {
   *out = v;
 }
 
-void launch_condensation(cleo_ctx* c, const CondArgs& a) {
-  if (a.ngbx == 0) return;
+void launch_condensation(cleo_ctx* c, const CondArgs& args) {
+  if (args.ngbx == 0) return;
+  CondArgs a = args;
+  const unsigned nb = gbx_blocks(a.ngbx, a.wpb, c->tiles.cond_grid_waves);
+  a.stat = stat_slots(c, static_cast<uint64_t>(nb) * a.wpb);
   if (a.fuse_qcond)
-    hipLaunchKernelGGL(k_condensation<true>, dim3(nblocks(a.ngbx, a.wpb)), dim3(64 * a.wpb), 0, c->stream, a);
+    hipLaunchKernelGGL(k_condensation<true>, dim3(nb), dim3(64 * a.wpb), 0, c->stream, a);
   else
-    hipLaunchKernelGGL(k_condensation<false>, dim3(nblocks(a.ngbx, a.wpb)), dim3(64 * a.wpb), 0, c->stream, a);
+    hipLaunchKernelGGL(k_condensation<false>, dim3(nb), dim3(64 * a.wpb), 0, c->stream, a);
+  launch_stats_reduce(c, nb * a.wpb, ST_COND_DROPS, 4);
 }
 void launch_qcond(cleo_ctx* c, const QcondArgs& a) {
   if (a.ngbx == 0) return;
-  hipLaunchKernelGGL(k_qcond, dim3(nblocks(a.ngbx, a.wpb)), dim3(64 * a.wpb), 0, c->stream, a);
+  hipLaunchKernelGGL(k_qcond, dim3(gbx_blocks(a.ngbx, a.wpb, c->tiles.cond_grid_waves)), dim3(64 * a.wpb), 0,
+                     c->stream, a);
 }
-void launch_collisions(cleo_ctx* c, const CollArgs& a) {
-  if (a.ngbx == 0) return;
+void launch_collisions(cleo_ctx* c, const CollArgs& args) {
+  if (args.ngbx == 0) return;
+  CollArgs a = args;
+  const unsigned nb = gbx_blocks(a.ngbx, a.wpb, c->tiles.coll_grid_waves);
+  a.stat = stat_slots(c, static_cast<uint64_t>(nb) * a.wpb);
   const size_t lds = static_cast<size_t>(a.wpb) * 2u * kCollLdsCap * sizeof(uint32_t);
-  hipLaunchKernelGGL(k_collisions, dim3(nblocks(a.ngbx, a.wpb)), dim3(64 * a.wpb), lds, c->stream, a);
+  hipLaunchKernelGGL(k_collisions, dim3(nb), dim3(64 * a.wpb), lds, c->stream, a);
+  launch_stats_reduce(c, nb * a.wpb, ST_COLL_PAIRS, 3);
 }
-void launch_motion(cleo_ctx* c, const MotionArgs& a) {
-  if (a.grid.ngbx == 0) return;
-  hipLaunchKernelGGL(k_motion, dim3(nblocks(a.grid.ngbx, a.wpb)), dim3(64 * a.wpb), 0, c->stream, a);
+void launch_motion(cleo_ctx* c, const MotionArgs& args) {
+  if (args.grid.ngbx == 0) return;
+  MotionArgs a = args;
+  const unsigned nb = gbx_blocks(a.grid.ngbx, a.wpb, c->tiles.motion_grid_waves);
+  a.stat = stat_slots(c, static_cast<uint64_t>(nb) * a.wpb);
+  hipLaunchKernelGGL(k_motion, dim3(nb), dim3(64 * a.wpb), 0, c->stream, a);
+  launch_stats_reduce(c, nb * a.wpb, ST_MOTION_DROPS, 1);
 }
+void launch_stats_reduce(cleo_ctx* c, uint32_t nslot, int first, int ncount) {
+  if (nslot == 0) return;
+  const unsigned nb = static_cast<unsigned>(std::min<uint64_t>(128, (nslot + 1023) / 1024));
+  hipLaunchKernelGGL(k_stats_reduce, dim3(nb, static_cast<unsigned>(ncount)), dim3(256), 0, c->stream,
+                     static_cast<const uint32_t*>(c->sp), nslot, first, c->ctl);
+}
+StatSlots stat_slots(cleo_ctx* c, uint64_t nslot) { return StatSlots{c->sp, static_cast<uint32_t>(nslot)}; }
 void launch_advance_event(cleo_ctx* c) { hipLaunchKernelGGL(k_advance_event, dim3(1), dim3(1), 0, c->stream, c->ctl); }
 void launch_inj_sizes(cleo_ctx* c, uint32_t* nj, uint32_t* np) {
   hipLaunchKernelGGL(k_inj_sizes, dim3(nblocks(c->grid.ngbx, 256)), dim3(256), 0, c->stream, c->nact, c->grid.ngbx, nj, np);
@@ -895,7 +956,8 @@ void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool fr
   hipLaunchKernelGGL(k_nsort_tot, dim3(nblocks(ngbx, 256)), dim3(256), 0, c->stream, a);
   scan_u32(c, c->tot, ngbx, 1u, 0u, c->off_alt, flag);
   scan_u32(c, c->cnt, ngbx, 28u, 27u, c->oobpre, flag);
-  hipLaunchKernelGGL(k_nsort_scatter, dim3(nblocks(ngbx, a.wpb)), dim3(64 * a.wpb), 0, c->stream, a);
+  hipLaunchKernelGGL(k_nsort_scatter, dim3(gbx_blocks(ngbx, a.wpb, c->tiles.motion_grid_waves)), dim3(64 * a.wpb), 0,
+                     c->stream, a);
   hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, a, c->ntot);
   hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, flag);
   if (flag == nullptr) {
@@ -955,10 +1017,7 @@ namespace cleo {
 
 // per gridbox: optional qcond recompute (ordered, before any radius changes), then the
 // saturation ratio and diffusion factor condensation uses (condensation.cpp:44-56)
-__global__ void __launch_bounds__(256) k_cond_prep(CondFlatArgs a) {
-  const int lane = lane_id();
-  const uint32_t g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (g >= a.ngbx) return;
+__device__ __forceinline__ void cond_prep_gbx(const CondFlatArgs& a, uint32_t g, int lane) {
   const double temp = a.st.temp[g], press = a.st.press[g], qv = a.st.qv[g];
   if (a.fuse_qcond) {
     const uint32_t base = a.off[g];
@@ -984,13 +1043,17 @@ __global__ void __launch_bounds__(256) k_cond_prep(CondFlatArgs a) {
     a.gs[2 * g + 1] = diffusion_factor(press, temp, psat);
   }
 }
+__global__ void __launch_bounds__(256) k_cond_prep(CondFlatArgs a) {
+  const int lane = lane_id();
+  for (uint32_t g = wave_index(); g < a.ngbx; g += wave_count()) cond_prep_gbx(a, g, lane);
+}
 
 // per active drop: Koehler factors, Newton solve, radius update, mass change * xi into dm[k]
 __global__ void __launch_bounds__(256) k_cond_drops(CondFlatArgs a) {
-  const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint32_t ndom = a.off[a.ngbx];
   uint32_t iters = 0, subs = 0, restarts = 0, solved = 0;
-  if (k < ndom) {
+  for (uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < ndom;
+       k += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     const uint32_t g = a.s.gbx[k];
     if (g < a.ngbx && k < a.off[g] + a.nact[g]) {
       const double s_ratio = a.gs[2 * g], ffactor = a.gs[2 * g + 1];
@@ -1010,10 +1073,10 @@ __global__ void __launch_bounds__(256) k_cond_drops(CondFlatArgs a) {
       if (bad) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, g, k);
       if (o.maxed) set_error(a.ctl, CLEO_ERR_NEWTON_MAXITERS, g, k);
       if (a.niters) a.niters[k] = o.iters;
-      iters = o.iters;
-      subs = o.substep ? 1u : 0u;
-      restarts = o.restarted ? 1u : 0u;
-      solved = 1;
+      iters += o.iters;
+      subs += o.substep ? 1u : 0u;
+      restarts += o.restarted ? 1u : 0u;
+      solved += 1;
     }
   }
   iters = wave_sum_u32(iters);
@@ -1021,18 +1084,16 @@ __global__ void __launch_bounds__(256) k_cond_drops(CondFlatArgs a) {
   restarts = wave_sum_u32(restarts);
   solved = wave_sum_u32(solved);
   if (lane_id() == 0) {
-    add_stat(a.ctl, ST_COND_DROPS, solved);
-    add_stat(a.ctl, ST_NEWTON_ITERS, iters);
-    add_stat(a.ctl, ST_SUBSTEP_DROPS, subs);
-    add_stat(a.ctl, ST_WARM_RESTARTS, restarts);
+    const uint32_t slot = wave_index();
+    put_stat(a.stat, 0, slot, solved);
+    put_stat(a.stat, 1, slot, iters);
+    put_stat(a.stat, 2, slot, subs);
+    put_stat(a.stat, 3, slot, restarts);
   }
 }
 
 // per gridbox: Σ dm in drop order and the thermodynamic feedback (condensation.cpp:126-137)
-__global__ void __launch_bounds__(256) k_cond_state(CondFlatArgs a) {
-  const int lane = lane_id();
-  const uint32_t g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (g >= a.ngbx) return;
+__device__ __forceinline__ void cond_state_gbx(const CondFlatArgs& a, uint32_t g, int lane) {
   const uint32_t base = a.off[g];
   const uint32_t n = a.nact[g];
   double tot = 0.0;
@@ -1052,6 +1113,10 @@ __global__ void __launch_bounds__(256) k_cond_state(CondFlatArgs a) {
     a.st.qc[g] = qc;
   }
 }
+__global__ void __launch_bounds__(256) k_cond_state(CondFlatArgs a) {
+  const int lane = lane_id();
+  for (uint32_t g = wave_index(); g < a.ngbx; g += wave_count()) cond_state_gbx(a, g, lane);
+}
 
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
   for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
@@ -1060,13 +1125,10 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
 }
 
 // MoveSupersInGridboxesFunctor per drop; direction counts aggregated per wave, then atomics
-__global__ void __launch_bounds__(256) k_motion_flat(MotionArgs a) {
-  const int lane = lane_id();
-  const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const uint32_t ndom = a.off[a.grid.ngbx];
-  const bool act = k < ndom;
-  uint32_t key = 0;  // g*28 + direction
-  if (act) {
+// one drop: move it and return its direction key g*28 + direction
+__device__ __forceinline__ uint32_t motion_drop(const MotionArgs& a, uint64_t k) {
+  uint32_t key;
+  {
     const uint32_t g = a.s.gbx[k];
     const uint32_t nz = a.grid.nz, nx = a.grid.nx, ny = a.grid.ny;
     const uint32_t iz = g % nz, ix = (g / nz) % nx, iy = g / (nz * nx);
@@ -1120,30 +1182,52 @@ __global__ void __launch_bounds__(256) k_motion_flat(MotionArgs a) {
     }
     key = 28u * g + static_cast<uint32_t>(dir);
   }
-  uint64_t mask = __ballot(act);
-  const uint32_t nmoved = static_cast<uint32_t>(__popcll(mask));
-  while (mask) {
-    const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-    const uint32_t v = readlane_u32(key, leader);
-    const uint64_t m = __ballot(act && key == v);
-    if (lane == leader) atomicAdd(&a.cnt[v], static_cast<uint32_t>(__popcll(m)));
-    mask &= ~m;
+  return key;
+}
+__global__ void __launch_bounds__(256) k_motion_flat(MotionArgs a) {
+  const int lane = lane_id();
+  const uint32_t ndom = a.off[a.grid.ngbx];
+  uint32_t nmoved = 0;
+  // block-uniform chunks keep every wave in the loop together for the ballots below
+  for (uint64_t k0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x; k0 < ndom;
+       k0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t k = k0 + threadIdx.x;
+    const bool act = k < ndom;
+    const uint32_t key = act ? motion_drop(a, k) : 0u;
+    uint64_t mask = __ballot(act);
+    nmoved += static_cast<uint32_t>(__popcll(mask));
+    while (mask) {
+      const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+      const uint32_t v = readlane_u32(key, leader);
+      const uint64_t m = __ballot(act && key == v);
+      if (lane == leader) atomicAdd(&a.cnt[v], static_cast<uint32_t>(__popcll(m)));
+      mask &= ~m;
+    }
   }
-  if (lane == 0) add_stat(a.ctl, ST_MOTION_DROPS, nmoved);
+  if (lane == 0) put_stat(a.stat, 0, wave_index(), nmoved);
 }
 
-void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& a) {
-  if (a.ngbx == 0) return;
-  hipLaunchKernelGGL(k_cond_prep, dim3(nblocks(a.ngbx, 4)), dim3(256), 0, c->stream, a);
-  hipLaunchKernelGGL(k_cond_drops, dim3(nblocks(c->ntot, 256)), dim3(256), 0, c->stream, a);
-  hipLaunchKernelGGL(k_cond_state, dim3(nblocks(a.ngbx, 4)), dim3(256), 0, c->stream, a);
+void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& args) {
+  if (args.ngbx == 0) return;
+  CondFlatArgs a = args;
+  const uint32_t gw = c->tiles.cond_grid_waves;
+  const unsigned nb = drop_blocks(c->ntot, gw);
+  a.stat = stat_slots(c, 4ull * nb);  // every wave of k_cond_drops writes its slot
+  hipLaunchKernelGGL(k_cond_prep, dim3(gbx_blocks(a.ngbx, 4, gw)), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(k_cond_drops, dim3(nb), dim3(256), 0, c->stream, a);
+  launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
+  hipLaunchKernelGGL(k_cond_state, dim3(gbx_blocks(a.ngbx, 4, gw)), dim3(256), 0, c->stream, a);
 }
 
-void launch_motion_flat(cleo_ctx* c, const MotionArgs& a) {
-  if (a.grid.ngbx == 0) return;
+void launch_motion_flat(cleo_ctx* c, const MotionArgs& args) {
+  if (args.grid.ngbx == 0) return;
+  MotionArgs a = args;
+  const unsigned nb = drop_blocks(c->ntot, c->tiles.motion_grid_waves);
+  a.stat = stat_slots(c, 4ull * nb);  // every wave of k_motion_flat writes its slot
   hipLaunchKernelGGL(k_fill_u32, dim3(stride_blocks(28ull * a.grid.ngbx)), dim3(256), 0, c->stream, a.cnt,
                      28ull * a.grid.ngbx, 0u);
-  hipLaunchKernelGGL(k_motion_flat, dim3(nblocks(c->ntot, 256)), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(k_motion_flat, dim3(nb), dim3(256), 0, c->stream, a);
+  launch_stats_reduce(c, 4u * nb, ST_MOTION_DROPS, 1);
 }
 
 }  // namespace cleo

@@ -6,6 +6,7 @@ namespace cleo {
 
 constexpr int kCollMaxWpb = 4;
 constexpr uint32_t kCollLdsCap = 512;  // drops per gridbox held in LDS; larger gridboxes use HBM scratch
+constexpr int kStatSlotsMax = 4;       // counters per wave of one kernel
 
 struct CondArgs {
   SoA s;
@@ -22,6 +23,7 @@ struct CondArgs {
   uint32_t* niters;
   const double* warm;
   uint32_t warm_limit;
+  StatSlots stat;  // ST_COND_DROPS .. ST_WARM_RESTARTS
 };
 
 struct QcondArgs {
@@ -51,6 +53,7 @@ struct CollArgs {
   const uint32_t* inj_op;
   uint32_t* big;
   int32_t timing_only;  // autotuner runs: no null partition, `src` and `nact` stay untouched
+  StatSlots stat;       // ST_COLL_PAIRS .. ST_NULLS
 };
 
 struct MotionArgs {
@@ -63,6 +66,7 @@ struct MotionArgs {
   uint32_t wpb;
   uint32_t* cnt;
   DevCtl* ctl;
+  StatSlots stat;  // ST_MOTION_DROPS
 };
 
 struct NsortArgs {
@@ -96,6 +100,7 @@ struct CondFlatArgs {
   uint32_t warm_limit;
   double* gs;  // 2 per gridbox: saturation ratio, diffusion factor
   double* dm;  // per drop: condensed mass * xi
+  StatSlots stat;  // ST_COND_DROPS .. ST_WARM_RESTARTS
 };
 
 // launchers (sdm_kernels.hip); all enqueue on ctx->stream
@@ -106,6 +111,10 @@ void launch_qcond(cleo_ctx* c, const QcondArgs& a);
 void launch_collisions(cleo_ctx* c, const CollArgs& a);
 void launch_motion(cleo_ctx* c, const MotionArgs& a);
 void launch_advance_event(cleo_ctx* c);
+// DevCtl::stats[first + j] += sum over the nslot slots of counter j, j < ncount
+void launch_stats_reduce(cleo_ctx* c, uint32_t nslot, int first, int ncount);
+// slot block of the context for a launch with `nslot` waves
+StatSlots stat_slots(cleo_ctx* c, uint64_t nslot);
 void launch_inj_sizes(cleo_ctx* c, uint32_t* nj, uint32_t* np);
 // exclusive scan of in[i*stride+offset], i < len, into out[0..len] (out[len] = total)
 void scan_u32(cleo_ctx* c, const uint32_t* in, uint64_t len, uint32_t stride, uint32_t offset, uint32_t* out,

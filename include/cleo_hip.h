@@ -256,13 +256,16 @@ int cleo_download_niters(cleo_ctx* ctx, uint32_t* niters, uint64_t n);
 int cleo_record_guess(cleo_ctx* ctx, int32_t enable);
 int cleo_download_guess(cleo_ctx* ctx, double* z, uint64_t n);
 
-/* Launch tiling: drops per workgroup chunk etc. chosen by the autotuner; values 0 = default. */
+/* Launch tiling, chosen by the autotuner; values 0 = default. */
 typedef struct {
   uint32_t cond_waves_per_block;   /* gridboxes (waves) per workgroup, 1..4 */
   uint32_t coll_waves_per_block;
   uint32_t motion_waves_per_block;
   uint32_t cond_flat;              /* 1: one thread per drop + per-gridbox prep/sum kernels */
   uint32_t motion_flat;            /* 1: one thread per drop, direction counts by atomics */
+  uint32_t cond_grid_waves;        /* waves per launch; each loops over gridboxes (or drops) */
+  uint32_t coll_grid_waves;        /*   0: one wave per gridbox (one thread per drop) */
+  uint32_t motion_grid_waves;
 } cleo_tiles;
 int cleo_set_tiles(cleo_ctx* ctx, const cleo_tiles* tiles);
 int cleo_get_tiles(cleo_ctx* ctx, cleo_tiles* tiles);

@@ -179,21 +179,25 @@ def test_graph_replay_collisions_only(gpu):
 
 
 def test_tiles_do_not_change_results(gpu):
-    _say("case: waves-per-block tilings and thread-per-drop variants give identical results")
+    _say("case: waves-per-block tilings, looping grids and thread-per-drop variants give identical results")
     pop = P.coupled(SEED, ndims=(8, 6, 4), mean=64)
     s = sdm_params(cond=cond_params(), coll=coll_params(), motion=motion_params())
-    outs = []
-    for wpb, flat in ((1, 0), (2, 0), (4, 0), (1, 1)):
+    outs, stats = [], []
+    # the small grids make every wave loop over several gridboxes (or drop chunks)
+    for wpb, flat, gw in ((1, 0, 0), (2, 0, 0), (4, 0, 0), (1, 1, 0), (1, 0, 64), (2, 1, 64), (4, 0, 8), (1, 1, 4)):
         d = make_domain(gpu, pop)
-        t = abi.Tiles(wpb, wpb, wpb, flat, flat)
+        t = abi.Tiles(wpb, wpb, wpb, flat, flat, gw, gw, gw)
         assert gpu.lib.cleo_set_tiles(d.h, abi.ctypes_byref(t)) == 0
         d.set_sdm(s)
         d.rng_counter(11)
         for step in range(2):
             assert d.run_step(step * 100, (step + 1) * 100) == 0
         outs.append(d.download())
+        stats.append(d.stats())
     for o in outs[1:]:
         assert_supers_match(outs[0], o, exact_fp=True, what="tiles")
+    for st in stats[1:]:
+        assert st == stats[0]
 
 
 def test_multiblock_scans_coupled(gpu, oracle):
