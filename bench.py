@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--no-mlp", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--cpu-ny", type=int, default=16, help="CPU sample: 128 x 128 x ny gridboxes")
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--tiles", type=str, default=None,
+                    help="launch tiling as 8 comma-separated cleo_tiles fields (skips the autotuner)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy one 1 GiB device buffer first (reference point for byte counters)")
     return ap.parse_args()
@@ -139,7 +141,10 @@ def run_gpu(args):
 
     d = new_domain()
     tiles = abi.Tiles()
-    if not args.no_autotune:
+    if args.tiles:
+        tiles = abi.Tiles(*[int(v) for v in args.tiles.split(",")])
+        d.check(L.cleo_set_tiles(d.h, C.byref(tiles)), "set_tiles")
+    elif not args.no_autotune:
         d.check(L.cleo_autotune(d.h, C.byref(tiles)), "autotune")
     t = 0
     if args.warmup:

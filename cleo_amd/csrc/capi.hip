@@ -781,7 +781,8 @@ int cleo_set_tiles(cleo_ctx* c, const cleo_tiles* t) {
   if (!c || !t) return CLEO_E_ARG;
   for (uint32_t v : {t->cond_waves_per_block, t->coll_waves_per_block, t->motion_waves_per_block})
     if (v > 4) return fail(c, CLEO_E_ARG, "waves per block must be 0..4");
-  if (t->cond_flat > 1 || t->motion_flat > 1) return fail(c, CLEO_E_ARG, "variant flags must be 0 or 1");
+  if (t->cond_flat > 1 || t->motion_flat > 1 || t->coll_flat > 1)
+    return fail(c, CLEO_E_ARG, "variant flags must be 0 or 1");
   c->tiles = *t;
   return CLEO_OK;
 }
@@ -835,14 +836,16 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
   struct Cand {
     uint32_t wpb, flat;
   };
-  const Cand cands[4] = {{1, 0}, {2, 0}, {4, 0}, {1, 1}};
+  constexpr int kNC = 5;
+  const Cand cands[kNC] = {{1, 0}, {2, 0}, {4, 0}, {1, 1}, {4, 1}};
   const uint32_t grid_waves[3] = {0, 8192, 32768};
   for (int kern = 0; kern < 3 && rc == CLEO_OK; ++kern) {
-    for (int gi = 0; gi < 3 * 4; ++gi) {
-      const Cand& cd = cands[gi % 4];
-      const uint32_t gw = grid_waves[gi / 4];
-      if (kern == 1 && cd.flat) continue;  // collisions are gridbox-shaped only
+    for (int gi = 0; gi < 3 * kNC; ++gi) {
+      const Cand& cd = cands[gi % kNC];
+      const uint32_t gw = grid_waves[gi / kNC];
+      if (kern != 1 && cd.flat && cd.wpb != 1) continue;  // thread-per-drop kernels ignore wpb
       c->tiles.cond_grid_waves = c->tiles.coll_grid_waves = c->tiles.motion_grid_waves = gw;
+      c->tiles.coll_flat = cd.flat;
       float tmin = 1e30f;
       for (int rep = 0; rep < 2; ++rep) {
         if (copy_store() != hipSuccess) { rc = fail(c, CLEO_E_HIP, "autotune copy"); break; }
@@ -916,7 +919,8 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
   hipFree(saved_nact);
   HIP_TRY(c, ctx_copy(c, c->ctl, &saved_ctl, sizeof(DevCtl), hipMemcpyHostToDevice));
   if (rc != CLEO_OK) return rc;
-  c->tiles = cleo_tiles{pick[0], pick[1], pick[2], pick_flat[0], pick_flat[2], pick_gw[0], pick_gw[1], pick_gw[2]};
+  c->tiles = cleo_tiles{pick[0], pick[1], pick[2], pick_flat[0], pick_flat[2], pick_gw[0], pick_gw[1], pick_gw[2],
+                        pick_flat[1]};
   if (chosen) *chosen = c->tiles;
   return CLEO_OK;
 }

@@ -25,6 +25,13 @@ __device__ __forceinline__ void wave_sync() {
 __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int l) {
   return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), l));
 }
+// occupancy floor of a kernel: at least W waves per SIMD, so at most 512/W VGPRs per lane
+#define CLEO_OCCUPANCY(W) __attribute__((amdgpu_waves_per_eu(W)))
+
+// v with lane l replaced by the wave-uniform value x
+__device__ __forceinline__ uint32_t writelane_u32(uint32_t v, uint32_t x, uint32_t l) {
+  return static_cast<uint32_t>(lane_id()) == l ? x : v;
+}
 __device__ __forceinline__ double readlane_f64(double v, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);

@@ -92,7 +92,7 @@ __device__ __forceinline__ double rcubed(double r) { return r * r * r; }
 __device__ __forceinline__ double dryradius(double msol, const Solute& s) {
   constexpr double vconst = 3.0 / (4.0 * kPi);
   const double dryrcubed = vconst * msol / s.rho_sol;
-  return pow(dryrcubed, 1.0 / 3.0);
+  return cbrt(dryrcubed);  // pow(x, 1.0 / 3.0): the exponent is 1/3 rounded; cbrt is within 1 ulp of it
 }
 __device__ __forceinline__ double drop_mass(double r, double msol, const Solute& s) {
   constexpr double massconst(4.0 / 3.0 * kPi * dimless::Rho_l);
@@ -258,7 +258,8 @@ __device__ __forceinline__ bool first_unique_criteria(const ODEConsts& c, double
   return is_unactivated && is_subact;
 }
 __device__ __forceinline__ double critical_timestep(const ODEConsts& c) {
-  const double cuberoot = pow(5.0 * c.bkoh / c.akoh, 1.5);
+  const double x = 5.0 * c.bkoh / c.akoh;
+  const double cuberoot = x * sqrt(x);  // pow(x, 1.5) to within 1 ulp
   return 2.5 * c.ffactor_fv / c.akoh * cuberoot;
 }
 
@@ -330,9 +331,11 @@ __device__ __forceinline__ double tv_simmel(double radius) {
   if (radius >= r3) return a4;
   constexpr double massconst(4.0 / 3.0 * kPi * dimless::Rho_l);
   const double MASS = (massconst * radius * radius * radius) * dimless::MASS0grams;
-  if (radius >= r2) return a3 * pow(MASS, 1.0 / 6.0);
-  if (radius >= r1) return a2 * pow(MASS, 1.0 / 3.0);
-  return a1 * pow(MASS, 2.0 / 3.0);
+  // MASS^(1/6), ^(1/3), ^(2/3) from one cube root (pow in the reference; within 2 ulp of it)
+  const double cr = cbrt(MASS);
+  if (radius >= r2) return a3 * sqrt(cr);
+  if (radius >= r1) return a2 * cr;
+  return a1 * (cr * cr);
 }
 __device__ __forceinline__ double tv_rogersyau(double radius) {
   constexpr double r1 = 3e-5 / dimless::R0;
@@ -344,7 +347,7 @@ __device__ __forceinline__ double tv_rogersyau(double radius) {
   constexpr double k4 = 9 / dimless::W0;
   if (radius < r1) return k1 * (radius * radius);
   if (radius < r2) return k2 * radius;
-  if (radius < r3) return k3 * pow((radius * dimless::R0), 0.5);
+  if (radius < r3) return k3 * sqrt(radius * dimless::R0);  // pow(x, 0.5) rounds as sqrt(x)
   return k4;
 }
 __device__ __forceinline__ double tv_rogersgk(double radius) {
@@ -412,6 +415,7 @@ __device__ __forceinline__ double pair_probability(const CollCfg& k, double r1, 
 }
 
 // ---- enactment on (xi, r, msol) registers -----------------------------------------------------
+// New radii are cbrt(x) where the reference writes pow(x, 1.0 / 3.0) (within 1 ulp of it).
 struct DropAttrs {
   uint64_t xi;
   double r, msol;
@@ -433,7 +437,7 @@ __device__ __forceinline__ int coalesce(uint64_t gamma, DropAttrs& d1, DropAttrs
     if (new_xi >= xi1) err = CLEO_ERR_COAL_UNDEFINED;
     d1.xi = new_xi;
     const double new_rcubed = rcubed(d2.r) + static_cast<double>(gamma) * rcubed(d1.r);
-    const double nr = pow(new_rcubed, (1.0 / 3.0));
+    const double nr = cbrt(new_rcubed);
     if (radius_below_dry(nr, d2.msol, s)) err = err ? err : CLEO_ERR_RADIUS_BELOW_DRY;
     d2.r = nr;
     d2.msol = d2.msol + static_cast<double>(gamma) * d1.msol;
@@ -444,7 +448,7 @@ __device__ __forceinline__ int coalesce(uint64_t gamma, DropAttrs& d1, DropAttrs
     const uint64_t new_xi = old_xi / 2;
     if (new_xi >= old_xi) err = CLEO_ERR_COAL_UNDEFINED;
     const double new_rcubed = rcubed(d2.r) + static_cast<double>(gamma) * rcubed(d1.r);
-    const double new_r = pow(new_rcubed, (1.0 / 3.0));
+    const double new_r = cbrt(new_rcubed);
     const double new_msol = d2.msol + static_cast<double>(gamma) * d1.msol;
     if (new_xi == 0) {
       d1.xi = 0;
@@ -478,7 +482,7 @@ __device__ __forceinline__ int breakup(double nfrags, DropAttrs& d1, DropAttrs& 
     if (new_xitot <= (old_xi * 2)) err = err ? err : CLEO_ERR_BREAKUP_NFRAGS;
     const double sum_rcubed = rcubed(d1.r) + rcubed(d2.r);
     const double new_rcubed = sum_rcubed * static_cast<double>(old_xi) / static_cast<double>(new_xitot);
-    const double new_r = pow(new_rcubed, (1.0 / 3.0));
+    const double new_r = cbrt(new_rcubed);
     const double sum_msol = d1.msol + d2.msol;
     const double new_msol = sum_msol * static_cast<double>(old_xi) / static_cast<double>(new_xitot);
     if (new_xi1 == 0 || new_xi2 == 0) err = err ? err : CLEO_ERR_XI_ZERO;
@@ -503,7 +507,7 @@ __device__ __forceinline__ int breakup(double nfrags, DropAttrs& d1, DropAttrs& 
   if ((new_xi1 + new_xi2) <= (old_xi1 + old_xi2)) err = err ? err : CLEO_ERR_BREAKUP_NFRAGS;
   const double sum_rcubed = rcubed(d1.r) + rcubed(d2.r);
   const double new_rcubed = sum_rcubed * static_cast<double>(old_xi2) / static_cast<double>(new_xi2);
-  const double new_r = pow(new_rcubed, (1.0 / 3.0));
+  const double new_r = cbrt(new_rcubed);
   const double sum_msol = d1.msol + d2.msol;
   const double new_msol = sum_msol * static_cast<double>(old_xi2) / static_cast<double>(new_xi2);
   if (new_xi2 == 0) err = err ? err : CLEO_ERR_XI_ZERO;

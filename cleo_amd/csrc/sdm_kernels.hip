@@ -104,7 +104,7 @@ __device__ __forceinline__ void cond_gbx(const CondArgs& a, uint32_t g, int lane
   acc[3] += restarts;
 }
 template <bool kFuseQcond>
-__global__ void __launch_bounds__(256) k_condensation(CondArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_condensation(CondArgs a) {
   const int lane = lane_id();
   uint32_t acc[4] = {0, 0, 0, 0};
   for (uint32_t g = wave_index(); g < a.ngbx; g += wave_count()) cond_gbx<kFuseQcond>(a, g, lane, acc);
@@ -131,16 +131,153 @@ __device__ __forceinline__ void qcond_gbx(const QcondArgs& a, uint32_t g, int la
     a.st.qc[g] = rho_cond / rho_dry;
   }
 }
-__global__ void __launch_bounds__(256) k_qcond(QcondArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(8) k_qcond(QcondArgs a) {
   const int lane = lane_id();
   for (uint32_t g = wave_index(); g < a.ngbx; g += wave_count()) qcond_gbx(a, g, lane);
 }
 
 // ============================ collisions ======================================================
 // DoCollisions::do_collisions for one gridbox per wave (collisions.hpp:140-278):
-// Fisher-Yates on a u32 index permutation in LDS (one lane walks the n-1 swaps, all lanes draw
-// the indices first), pairs (2jj, 2jj+1) one per lane, enactment, then the drops are written to
+// Fisher-Yates on a u32 index permutation (all lanes draw the indices first; the n-1 swaps run
+// in registers for n <= 128, else one lane walks them in LDS / HBM scratch), pairs (2jj, 2jj+1) one per lane, enactment, then the drops are written to
 // `dst` in shuffled order with nulls moved behind the active drops (stable partition).
+// j_t of Fisher-Yates swap t < n-1 of gridbox g: uniform in [0, iter], iter = n-1-t
+// (shuffle.hpp:80-87), from the counter stream or the injected indices
+__device__ __forceinline__ uint32_t fy_index(const CollArgs& a, uint32_t g, uint32_t base, uint32_t n, uint32_t t,
+                                             uint64_t event) {
+  const uint32_t iter = n - 1 - t;
+  uint32_t j;
+  if (a.inj_j) {
+    j = a.inj_j[a.inj_oj[g] + t];
+    if (j > iter) {  // an injected index outside [0, iter] is an argument error
+      set_error(a.ctl, CLEO_ERR_INJECTED_INDEX, g, base + iter);
+      j = iter;
+    }
+  } else {
+    uint32_t r4[4];
+    rng_block(a.seed, event, 0u, g, iter, r4);
+    j = static_cast<uint32_t>((static_cast<uint64_t>(r4[0]) * (iter + 1ull)) >> 32);
+  }
+  return j;
+}
+
+// One collision pair (2jj, 2jj+1) of gridbox g (n active drops) from the drops at ka, kb of `src`
+// (collisions.hpp:180-230): probability, enactment, and both drops written to their shuffled
+// positions base+2jj, base+2jj+1 of `dst` (updated attributes, other fields carried).  Returns
+// true when one of the two became null; `events` counts enactments.
+__device__ __forceinline__ bool coll_pair(const CollArgs& a, uint32_t g, uint32_t base, uint32_t n, uint32_t jj,
+                                          uint32_t ka, uint32_t kb, uint64_t event, uint32_t& events) {
+  const uint32_t npairs = n / 2;
+  const double scale_p = n * (n - 1.0) / (2.0 * npairs);
+  const double VOLUME = a.vol[g] * dimless::VOL0;
+  DropAttrs A{a.src.xi[ka], a.src.r[ka], a.src.msol[ka]};
+  DropAttrs B{a.src.xi[kb], a.src.r[kb], a.src.msol[kb]};
+  const bool a_first = !(A.xi < B.xi);  // assign_drops (collisions.hpp:94-101)
+  DropAttrs& d1 = a_first ? A : B;
+  DropAttrs& d2 = a_first ? B : A;
+  const double prob_jk = pair_probability(a.k, d1.r, d2.r, VOLUME);
+  const double prob = scale_p * static_cast<double>(d1.xi) * prob_jk;
+  double phi_coll, phi_out;
+  if (a.inj_phi) {
+    const uint64_t o = a.inj_op[g] + 2ull * jj;
+    phi_coll = a.inj_phi[o];
+    phi_out = a.inj_phi[o + 1];
+  } else {
+    uint32_t r4[4];
+    rng_block(a.seed, event, 1u, g, jj, r4);
+    phi_coll = u53(r4[0], r4[1]);
+    phi_out = u53(r4[2], r4[3]);
+  }
+  int err = 0;
+  const uint32_t res = enact_pair(a.k, d1, d2, prob, phi_coll, phi_out, a.sol, err);
+  if (err) set_error(a.ctl, err, g, base + 2 * jj);
+  events += (res >> 1) & 1u;
+  const bool nullA = (res & 1u) && a_first;
+  const bool nullB = (res & 1u) && !a_first;
+  const uint32_t oa = base + 2 * jj, ob = oa + 1;
+  a.dst.gbx[oa] = nullA ? kOOB : a.src.gbx[ka];
+  a.dst.c3[oa] = a.src.c3[ka];
+  a.dst.c1[oa] = a.src.c1[ka];
+  a.dst.c2[oa] = a.src.c2[ka];
+  a.dst.xi[oa] = A.xi;
+  a.dst.r[oa] = A.r;
+  a.dst.msol[oa] = A.msol;
+  a.dst.id[oa] = a.src.id[ka];
+  a.dst.gbx[ob] = nullB ? kOOB : a.src.gbx[kb];
+  a.dst.c3[ob] = a.src.c3[kb];
+  a.dst.c1[ob] = a.src.c1[kb];
+  a.dst.c2[ob] = a.src.c2[kb];
+  a.dst.xi[ob] = B.xi;
+  a.dst.r[ob] = B.r;
+  a.dst.msol[ob] = B.msol;
+  a.dst.id[ob] = a.src.id[kb];
+  return res & 1u;
+}
+// drop k of a gridbox that takes no part in a pair (the unpaired last drop and the inactive
+// tail) is carried unchanged from src[ks] to dst[kd]
+__device__ __forceinline__ void coll_carry(const CollArgs& a, uint32_t ks, uint32_t kd) {
+  a.dst.gbx[kd] = a.src.gbx[ks];
+  a.dst.c3[kd] = a.src.c3[ks];
+  a.dst.c1[kd] = a.src.c1[ks];
+  a.dst.c2[kd] = a.src.c2[ks];
+  a.dst.xi[kd] = a.src.xi[ks];
+  a.dst.r[kd] = a.src.r[ks];
+  a.dst.msol[kd] = a.src.msol[ks];
+  a.dst.id[kd] = a.src.id[ks];
+}
+
+// remove_null_supers (collisions.hpp:240-248) for gridbox g after its pairs were written to
+// `dst`: the reference's sort_team is restated as a stable partition (active drops first, nulls
+// behind, each in order), through the now unused `src` range of the gridbox.  Only this wave
+// reads back what it stored, so a workgroup-scope fence orders it (an agent-scope fence would
+// write back the XCD's whole L2).
+__device__ __forceinline__ void coll_remove_nulls(const CollArgs& a, uint32_t g, uint32_t base, uint32_t n,
+                                                  uint32_t nulls, int lane) {
+  __threadfence_block();
+  wave_sync();
+  const uint32_t nn = n - nulls;
+  uint32_t run_nn = 0, run_null = 0;
+  for (uint32_t c = 0; c < n; c += kWave) {
+    const uint32_t k = c + lane;
+    const bool act = k < n;
+    const bool isnull = act && a.dst.xi[base + k] == 0;
+    const uint64_t mnull = __ballot(isnull);
+    const uint64_t mnn = __ballot(act && !isnull);
+    if (act) {
+      const uint32_t pos = isnull ? nn + run_null + __popcll(mnull & lanemask_lt())
+                                  : run_nn + __popcll(mnn & lanemask_lt());
+      const uint32_t ks = base + k, kd = base + pos;
+      a.src.gbx[kd] = a.dst.gbx[ks];
+      a.src.c3[kd] = a.dst.c3[ks];
+      a.src.c1[kd] = a.dst.c1[ks];
+      a.src.c2[kd] = a.dst.c2[ks];
+      a.src.xi[kd] = a.dst.xi[ks];
+      a.src.r[kd] = a.dst.r[ks];
+      a.src.msol[kd] = a.dst.msol[ks];
+      a.src.id[kd] = a.dst.id[ks];
+    }
+    run_null += __popcll(mnull);
+    run_nn += __popcll(mnn);
+  }
+  __threadfence_block();
+  wave_sync();
+  for (uint32_t k = lane; k < n; k += kWave) {
+    const uint32_t kk = base + k;
+    a.dst.gbx[kk] = a.src.gbx[kk];
+    a.dst.c3[kk] = a.src.c3[kk];
+    a.dst.c1[kk] = a.src.c1[kk];
+    a.dst.c2[kk] = a.src.c2[kk];
+    a.dst.xi[kk] = a.src.xi[kk];
+    a.dst.r[kk] = a.src.r[kk];
+    a.dst.msol[kk] = a.src.msol[kk];
+    a.dst.id[kk] = a.src.id[kk];
+  }
+  if (lane == 0) {
+    a.nact[g] = nn;
+    a.ctl->resort_flag = 1u;
+  }
+}
+
 // The permutation `perm` and the drawn indices `jb` live in LDS (kLds) or, for gridboxes above
 // kCollLdsCap drops, in HBM scratch; the two instantiations keep LDS accesses as ds_* instructions.
 template <bool kLds>
@@ -148,33 +285,44 @@ __device__ __forceinline__ void coll_gbx(const CollArgs& a, int lane, uint32_t g
                                          uint32_t n, uint32_t* perm, uint32_t* jb, uint32_t (&acc)[3]) {
   wave_sync();  // the previous gridbox of this wave is done with perm / jb
   const uint64_t event = a.ctl->event;
-  for (uint32_t k = lane; k < n; k += kWave) perm[k] = k;
-  if (n >= 2) {
-    for (uint32_t t = lane; t < n - 1; t += kWave) {
+  auto draw = [&](uint32_t t) -> uint32_t { return fy_index(a, g, base, n, t, event); };
+  if (kLds && n <= 2 * kWave) {
+    // Fisher-Yates in registers: element e of the permutation is lane e % 64 of P[e / 64]; the
+    // n-1 swaps are wave-uniform readlane / writelane pairs instead of a dependent LDS chain.
+    uint32_t P0 = static_cast<uint32_t>(lane), P1 = static_cast<uint32_t>(lane) + kWave;
+    const uint32_t J0 = (n >= 2 && static_cast<uint32_t>(lane) < n - 1) ? draw(lane) : 0u;
+    const uint32_t J1 = (n >= 2 && static_cast<uint32_t>(lane) + kWave < n - 1) ? draw(lane + kWave) : 0u;
+    for (uint32_t t = 0; t + 1 < n; ++t) {
       const uint32_t iter = n - 1 - t;
-      uint32_t j;
-      if (a.inj_j) {
-        j = a.inj_j[a.inj_oj[g] + t];
-        if (j > iter) {  // an injected index outside [0, iter] is an argument error
-          set_error(a.ctl, CLEO_ERR_INJECTED_INDEX, g, base + iter);
-          j = iter;
-        }
-      } else {
-        uint32_t r4[4];
-        rng_block(a.seed, event, 0u, g, iter, r4);
-        j = static_cast<uint32_t>((static_cast<uint64_t>(r4[0]) * (iter + 1ull)) >> 32);
-      }
-      jb[t] = j;
+      const uint32_t j = t < kWave ? readlane_u32(J0, t) : readlane_u32(J1, t - kWave);
+      const uint32_t vi = iter < kWave ? readlane_u32(P0, iter) : readlane_u32(P1, iter - kWave);
+      const uint32_t vj = j < kWave ? readlane_u32(P0, j) : readlane_u32(P1, j - kWave);
+      if (iter < kWave)
+        P0 = writelane_u32(P0, vj, iter);
+      else
+        P1 = writelane_u32(P1, vj, iter - kWave);
+      if (j < kWave)
+        P0 = writelane_u32(P0, vi, j);
+      else
+        P1 = writelane_u32(P1, vi, j - kWave);
     }
-    if (!kLds) __threadfence_block();
+    perm[lane] = P0;
+    perm[lane + kWave] = P1;
     wave_sync();
-    if (lane == 0) {  // fisher_yates_shuffle (shuffle.hpp:80-87)
-      for (uint32_t t = 0; t + 1 < n; ++t) {
-        const uint32_t iter = n - 1 - t;
-        const uint32_t j = jb[t];
-        const uint32_t pi = perm[iter];
-        perm[iter] = perm[j];
-        perm[j] = pi;
+  } else {
+    for (uint32_t k = lane; k < n; k += kWave) perm[k] = k;
+    if (n >= 2) {
+      for (uint32_t t = lane; t < n - 1; t += kWave) jb[t] = draw(t);
+      if (!kLds) __threadfence_block();
+      wave_sync();
+      if (lane == 0) {  // fisher_yates_shuffle (shuffle.hpp:80-87)
+        for (uint32_t t = 0; t + 1 < n; ++t) {
+          const uint32_t iter = n - 1 - t;
+          const uint32_t j = jb[t];
+          const uint32_t pi = perm[iter];
+          perm[iter] = perm[j];
+          perm[j] = pi;
+        }
       }
     }
     if (!kLds) __threadfence_block();
@@ -182,130 +330,22 @@ __device__ __forceinline__ void coll_gbx(const CollArgs& a, int lane, uint32_t g
   }
 
   const uint32_t npairs = n / 2;
-  const double scale_p = n * (n - 1.0) / (2.0 * npairs);
-  const double VOLUME = a.vol[g] * dimless::VOL0;
   uint32_t nulls = 0, events = 0;
   for (uint32_t c = 0; c < npairs; c += kWave) {
     const uint32_t jj = c + lane;
-    bool nullA = false, nullB = false;
-    if (jj < npairs) {
-      const uint32_t ka = base + perm[2 * jj];
-      const uint32_t kb = base + perm[2 * jj + 1];
-      DropAttrs A{a.src.xi[ka], a.src.r[ka], a.src.msol[ka]};
-      DropAttrs B{a.src.xi[kb], a.src.r[kb], a.src.msol[kb]};
-      const bool a_first = !(A.xi < B.xi);  // assign_drops (collisions.hpp:94-101)
-      DropAttrs& d1 = a_first ? A : B;
-      DropAttrs& d2 = a_first ? B : A;
-      const double prob_jk = pair_probability(a.k, d1.r, d2.r, VOLUME);
-      const double prob = scale_p * static_cast<double>(d1.xi) * prob_jk;
-      double phi_coll, phi_out;
-      if (a.inj_phi) {
-        const uint64_t o = a.inj_op[g] + 2ull * jj;
-        phi_coll = a.inj_phi[o];
-        phi_out = a.inj_phi[o + 1];
-      } else {
-        uint32_t r4[4];
-        rng_block(a.seed, event, 1u, g, jj, r4);
-        phi_coll = u53(r4[0], r4[1]);
-        phi_out = u53(r4[2], r4[3]);
-      }
-      int err = 0;
-      const uint32_t res = enact_pair(a.k, d1, d2, prob, phi_coll, phi_out, a.sol, err);
-      if (err) set_error(a.ctl, err, g, base + 2 * jj);
-      events += (res >> 1) & 1u;
-      if (res & 1u) {
-        nullA = a_first;
-        nullB = !a_first;
-      }
-      // write the pair at its shuffled positions (updated attributes, other fields carried)
-      const uint32_t oa = base + 2 * jj, ob = oa + 1;
-      a.dst.gbx[oa] = nullA ? kOOB : a.src.gbx[ka];
-      a.dst.c3[oa] = a.src.c3[ka];
-      a.dst.c1[oa] = a.src.c1[ka];
-      a.dst.c2[oa] = a.src.c2[ka];
-      a.dst.xi[oa] = A.xi;
-      a.dst.r[oa] = A.r;
-      a.dst.msol[oa] = A.msol;
-      a.dst.id[oa] = a.src.id[ka];
-      a.dst.gbx[ob] = nullB ? kOOB : a.src.gbx[kb];
-      a.dst.c3[ob] = a.src.c3[kb];
-      a.dst.c1[ob] = a.src.c1[kb];
-      a.dst.c2[ob] = a.src.c2[kb];
-      a.dst.xi[ob] = B.xi;
-      a.dst.r[ob] = B.r;
-      a.dst.msol[ob] = B.msol;
-      a.dst.id[ob] = a.src.id[kb];
-    }
-    nulls += static_cast<uint32_t>(__popcll(__ballot(nullA || nullB)));
+    bool nul = false;
+    if (jj < npairs) nul = coll_pair(a, g, base, n, jj, base + perm[2 * jj], base + perm[2 * jj + 1], event, events);
+    nulls += static_cast<uint32_t>(__popcll(__ballot(nul)));
   }
   // unpaired last drop and the inactive tail (nulls of earlier sub-steps) are carried unchanged
-  for (uint32_t k = 2 * npairs + lane; k < nfull; k += kWave) {
-    const uint32_t ks = base + (k < n ? perm[k] : k);
-    const uint32_t kd = base + k;
-    a.dst.gbx[kd] = a.src.gbx[ks];
-    a.dst.c3[kd] = a.src.c3[ks];
-    a.dst.c1[kd] = a.src.c1[ks];
-    a.dst.c2[kd] = a.src.c2[ks];
-    a.dst.xi[kd] = a.src.xi[ks];
-    a.dst.r[kd] = a.src.r[ks];
-    a.dst.msol[kd] = a.src.msol[ks];
-    a.dst.id[kd] = a.src.id[ks];
-  }
-  if (nulls > 0 && !a.timing_only) {
-    // remove_null_supers (collisions.hpp:240-248): the reference's sort_team is restated as a
-    // stable partition, through the now unused `src` range of this gridbox.  Only this wave
-    // reads back what it stored, so a workgroup-scope fence orders it (an agent-scope fence
-    // would write back the XCD's whole L2).
-    __threadfence_block();
-    wave_sync();
-    const uint32_t nn = n - nulls;
-    uint32_t run_nn = 0, run_null = 0;
-    for (uint32_t c = 0; c < n; c += kWave) {
-      const uint32_t k = c + lane;
-      const bool act = k < n;
-      const bool isnull = act && a.dst.xi[base + k] == 0;
-      const uint64_t mnull = __ballot(isnull);
-      const uint64_t mnn = __ballot(act && !isnull);
-      if (act) {
-        const uint32_t pos = isnull ? nn + run_null + __popcll(mnull & lanemask_lt())
-                                    : run_nn + __popcll(mnn & lanemask_lt());
-        const uint32_t ks = base + k, kd = base + pos;
-        a.src.gbx[kd] = a.dst.gbx[ks];
-        a.src.c3[kd] = a.dst.c3[ks];
-        a.src.c1[kd] = a.dst.c1[ks];
-        a.src.c2[kd] = a.dst.c2[ks];
-        a.src.xi[kd] = a.dst.xi[ks];
-        a.src.r[kd] = a.dst.r[ks];
-        a.src.msol[kd] = a.dst.msol[ks];
-        a.src.id[kd] = a.dst.id[ks];
-      }
-      run_null += __popcll(mnull);
-      run_nn += __popcll(mnn);
-    }
-    __threadfence_block();
-    wave_sync();
-    for (uint32_t k = lane; k < n; k += kWave) {
-      const uint32_t kk = base + k;
-      a.dst.gbx[kk] = a.src.gbx[kk];
-      a.dst.c3[kk] = a.src.c3[kk];
-      a.dst.c1[kk] = a.src.c1[kk];
-      a.dst.c2[kk] = a.src.c2[kk];
-      a.dst.xi[kk] = a.src.xi[kk];
-      a.dst.r[kk] = a.src.r[kk];
-      a.dst.msol[kk] = a.src.msol[kk];
-      a.dst.id[kk] = a.src.id[kk];
-    }
-    if (lane == 0) {
-      a.nact[g] = nn;
-      a.ctl->resort_flag = 1u;
-    }
-  }
+  for (uint32_t k = 2 * npairs + lane; k < nfull; k += kWave) coll_carry(a, base + (k < n ? perm[k] : k), base + k);
+  if (nulls > 0 && !a.timing_only) coll_remove_nulls(a, g, base, n, nulls, lane);
   acc[0] += npairs;
   acc[1] += wave_sum_u32(events);
   acc[2] += nulls;
 }
 
-__global__ void __launch_bounds__(256) k_collisions(CollArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_collisions(CollArgs a) {
   extern __shared__ uint32_t lds_dyn[];
   const int lane = lane_id();
   const int w = static_cast<int>(threadIdx.x >> 6);
@@ -322,6 +362,142 @@ __global__ void __launch_bounds__(256) k_collisions(CollArgs a) {
       coll_gbx<false>(a, lane, g, base, nfull, n, perm, perm + n, acc);
     }
   }
+  if (lane == 0)
+    for (int j = 0; j < 3; ++j) put_stat(a.stat, j, wave_index(), acc[j]);
+}
+
+// ---- tiled collisions: 64 gridboxes per wave ---------------------------------------------------
+// Lane i shuffles gridbox 64*tile + i: its permutation (one byte per drop) lives in the wave's LDS
+// tile and the swaps of all 64 gridboxes run side by side, with the same index sequence as the
+// per-gridbox path.  The tile's pairs and carried drops are then flattened over the lanes (a lane
+// finds its gridbox by binary search over the prefix of per-gridbox counts), so every lane works
+// whatever the gridbox sizes.  Gridboxes above kTileMaxN drops, or beyond the tile's LDS, take the
+// per-gridbox path through HBM scratch.
+struct CollTileLds {
+  uint32_t perm32[kTilePermBytes / 4];  // byte permutations, gridbox i from byte loff[i]
+  uint32_t pre[kWave + 1];              // pairs of the gridboxes before i
+  uint32_t tpre[kWave + 1];             // carried drops of the gridboxes before i
+  uint32_t base[kWave];
+  uint32_t loff[kWave];
+  uint32_t n[kWave];
+  uint32_t nulls[kWave];
+};
+static_assert(sizeof(CollTileLds) == kCollTileLdsBytes, "tile LDS size");
+
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, int lane, uint32_t& total) {
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  total = readlane_u32(x, kWave - 1);
+  return x - v;
+}
+// the i with pre[i] <= f < pre[i+1], given pre[0] <= f < pre[kWave]
+__device__ __forceinline__ uint32_t tile_owner(const uint32_t* pre, uint32_t f) {
+  uint32_t lo = 0, hi = kWave;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pre[mid] <= f)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_coll_tiles(CollArgs a) {
+  extern __shared__ uint32_t lds_dyn[];
+  const int lane = lane_id();
+  CollTileLds& L = reinterpret_cast<CollTileLds*>(lds_dyn)[threadIdx.x >> 6];
+  uint8_t* pb = reinterpret_cast<uint8_t*>(L.perm32);
+  const uint64_t event = a.ctl->event;
+  uint32_t acc[3] = {0, 0, 0};
+  uint32_t pairs = 0, events = 0, nulls = 0;
+  const uint32_t ntiles = (a.ngbx + kWave - 1) / kWave;
+  for (uint32_t tile = wave_index(); tile < ntiles; tile += wave_count()) {
+    wave_sync();  // the previous tile is done with L
+    const uint32_t g = tile * kWave + static_cast<uint32_t>(lane);
+    const bool valid = g < a.ngbx;
+    uint32_t base = 0, nfull = 0, n = 0;
+    if (valid) {
+      base = a.off[g];
+      nfull = a.off[g + 1] - base;
+      n = a.nact[g];
+    }
+    const bool fits = valid && n <= kTileMaxN;
+    uint32_t ltot;
+    const uint32_t loff = wave_excl_scan(fits ? (n + 3u) & ~3u : 0u, lane, ltot);
+    const bool small = fits && loff + n <= kTilePermBytes;
+    // fisher_yates_shuffle (shuffle.hpp:80-87), one gridbox per lane
+    if (small) {
+      for (uint32_t k = 0; k < n; k += 4) L.perm32[(loff + k) >> 2] = k | ((k + 1) << 8) | ((k + 2) << 16) | ((k + 3) << 24);
+      uint8_t* p = pb + loff;
+      for (uint32_t t = 0; t + 1 < n; ++t) {
+        const uint32_t iter = n - 1 - t;
+        const uint32_t j = fy_index(a, g, base, n, t, event);
+        const uint8_t pi = p[iter];
+        p[iter] = p[j];
+        p[j] = pi;
+      }
+    }
+    const uint32_t np = small ? n / 2 : 0u;
+    const uint32_t nc = small ? nfull - 2u * np : 0u;
+    uint32_t tot_pairs, tot_carry;
+    const uint32_t pre = wave_excl_scan(np, lane, tot_pairs);
+    const uint32_t tpre = wave_excl_scan(nc, lane, tot_carry);
+    L.pre[lane] = pre;
+    L.tpre[lane] = tpre;
+    if (lane == kWave - 1) {
+      L.pre[kWave] = pre + np;
+      L.tpre[kWave] = tpre + nc;
+    }
+    L.base[lane] = base;
+    L.loff[lane] = loff;
+    L.n[lane] = n;
+    L.nulls[lane] = 0u;
+    wave_sync();
+    for (uint32_t f = lane; f < tot_pairs; f += kWave) {
+      const uint32_t i = tile_owner(L.pre, f);
+      const uint32_t jj = f - L.pre[i];
+      const uint32_t bi = L.base[i];
+      const uint8_t* pi = pb + L.loff[i];
+      if (coll_pair(a, tile * kWave + i, bi, L.n[i], jj, bi + pi[2 * jj], bi + pi[2 * jj + 1], event, events))
+        atomicAdd(&L.nulls[i], 1u);
+    }
+    for (uint32_t f = lane; f < tot_carry; f += kWave) {
+      const uint32_t i = tile_owner(L.tpre, f);
+      const uint32_t ni = L.n[i];
+      const uint32_t k = 2u * (ni / 2u) + (f - L.tpre[i]);
+      const uint32_t bi = L.base[i];
+      coll_carry(a, bi + (k < ni ? static_cast<uint32_t>(pb[L.loff[i] + k]) : k), bi + k);
+    }
+    wave_sync();
+    const uint32_t mynulls = L.nulls[lane];
+    pairs += np;
+    nulls += mynulls;
+    if (!a.timing_only) {
+      uint64_t m = __ballot(mynulls > 0u);
+      while (m) {
+        const int i = __ffsll(static_cast<unsigned long long>(m)) - 1;
+        m &= m - 1;
+        coll_remove_nulls(a, tile * kWave + static_cast<uint32_t>(i), readlane_u32(base, i), readlane_u32(n, i),
+                          readlane_u32(mynulls, i), lane);
+      }
+    }
+    uint64_t mb = __ballot(valid && !small);
+    while (mb) {
+      const int i = __ffsll(static_cast<unsigned long long>(mb)) - 1;
+      mb &= mb - 1;
+      const uint32_t bb = readlane_u32(base, i), nb = readlane_u32(n, i);
+      coll_gbx<false>(a, lane, tile * kWave + static_cast<uint32_t>(i), bb, readlane_u32(nfull, i), nb,
+                      a.big + 2ull * bb, a.big + 2ull * bb + nb, acc);
+    }
+  }
+  acc[0] += wave_sum_u32(pairs);
+  acc[1] += wave_sum_u32(events);
+  acc[2] += wave_sum_u32(nulls);
   if (lane == 0)
     for (int j = 0; j < 3; ++j) put_stat(a.stat, j, wave_index(), acc[j]);
 }
@@ -478,7 +654,7 @@ __device__ __forceinline__ uint32_t motion_gbx(const MotionArgs& a, uint32_t g, 
   if (lane < 28) a.cnt[28ull * g + lane] = mycount;
   return nfull;
 }
-__global__ void __launch_bounds__(256) k_motion(MotionArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(6) k_motion(MotionArgs a) {
   const int lane = lane_id();
   uint32_t moved = 0;
   for (uint32_t g = wave_index(); g < a.grid.ngbx; g += wave_count()) moved += motion_gbx(a, g, lane);
@@ -598,7 +774,7 @@ __device__ __forceinline__ void nsort_scatter_gbx(const NsortArgs& a, uint32_t g
     }
   }
 }
-__global__ void __launch_bounds__(256) k_nsort_scatter(NsortArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(8) k_nsort_scatter(NsortArgs a) {
   if (guard_off(a.flag)) return;
   const int lane = lane_id();
   for (uint32_t g = wave_index(); g < a.grid.ngbx; g += wave_count()) nsort_scatter_gbx(a, g, lane);
@@ -873,6 +1049,13 @@ void launch_qcond(cleo_ctx* c, const QcondArgs& a) {
 void launch_collisions(cleo_ctx* c, const CollArgs& args) {
   if (args.ngbx == 0) return;
   CollArgs a = args;
+  if (c->tiles.coll_flat) {
+    const unsigned nb = gbx_blocks((a.ngbx + kWave - 1) / kWave, a.wpb, c->tiles.coll_grid_waves);
+    a.stat = stat_slots(c, static_cast<uint64_t>(nb) * a.wpb);
+    hipLaunchKernelGGL(k_coll_tiles, dim3(nb), dim3(64 * a.wpb), a.wpb * kCollTileLdsBytes, c->stream, a);
+    launch_stats_reduce(c, nb * a.wpb, ST_COLL_PAIRS, 3);
+    return;
+  }
   const unsigned nb = gbx_blocks(a.ngbx, a.wpb, c->tiles.coll_grid_waves);
   a.stat = stat_slots(c, static_cast<uint64_t>(nb) * a.wpb);
   const size_t lds = static_cast<size_t>(a.wpb) * 2u * kCollLdsCap * sizeof(uint32_t);
@@ -1043,13 +1226,13 @@ __device__ __forceinline__ void cond_prep_gbx(const CondFlatArgs& a, uint32_t g,
     a.gs[2 * g + 1] = diffusion_factor(press, temp, psat);
   }
 }
-__global__ void __launch_bounds__(256) k_cond_prep(CondFlatArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(8) k_cond_prep(CondFlatArgs a) {
   const int lane = lane_id();
   for (uint32_t g = wave_index(); g < a.ngbx; g += wave_count()) cond_prep_gbx(a, g, lane);
 }
 
 // per active drop: Koehler factors, Newton solve, radius update, mass change * xi into dm[k]
-__global__ void __launch_bounds__(256) k_cond_drops(CondFlatArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_drops(CondFlatArgs a) {
   const uint32_t ndom = a.off[a.ngbx];
   uint32_t iters = 0, subs = 0, restarts = 0, solved = 0;
   for (uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < ndom;
@@ -1113,7 +1296,7 @@ __device__ __forceinline__ void cond_state_gbx(const CondFlatArgs& a, uint32_t g
     a.st.qc[g] = qc;
   }
 }
-__global__ void __launch_bounds__(256) k_cond_state(CondFlatArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(8) k_cond_state(CondFlatArgs a) {
   const int lane = lane_id();
   for (uint32_t g = wave_index(); g < a.ngbx; g += wave_count()) cond_state_gbx(a, g, lane);
 }
@@ -1184,7 +1367,7 @@ __device__ __forceinline__ uint32_t motion_drop(const MotionArgs& a, uint64_t k)
   }
   return key;
 }
-__global__ void __launch_bounds__(256) k_motion_flat(MotionArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(6) k_motion_flat(MotionArgs a) {
   const int lane = lane_id();
   const uint32_t ndom = a.off[a.grid.ngbx];
   uint32_t nmoved = 0;

@@ -7,6 +7,10 @@ namespace cleo {
 constexpr int kCollMaxWpb = 4;
 constexpr uint32_t kCollLdsCap = 512;  // drops per gridbox held in LDS; larger gridboxes use HBM scratch
 constexpr int kStatSlotsMax = 4;       // counters per wave of one kernel
+// tiled collisions: byte permutations of up to kTileMaxN drops, kTilePermBytes of them per wave
+constexpr uint32_t kTileMaxN = 255;
+constexpr uint32_t kTilePermBytes = 8192;
+constexpr uint32_t kCollTileLdsBytes = kTilePermBytes + 4u * (2u * 65u + 4u * 64u);
 
 struct CondArgs {
   SoA s;

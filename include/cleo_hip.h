@@ -266,6 +266,7 @@ typedef struct {
   uint32_t cond_grid_waves;        /* waves per launch; each loops over gridboxes (or drops) */
   uint32_t coll_grid_waves;        /*   0: one wave per gridbox (one thread per drop) */
   uint32_t motion_grid_waves;
+  uint32_t coll_flat;              /* 1: 64 gridboxes per wave, pairs flattened over the lanes */
 } cleo_tiles;
 int cleo_set_tiles(cleo_ctx* ctx, const cleo_tiles* tiles);
 int cleo_get_tiles(cleo_ctx* ctx, cleo_tiles* tiles);

@@ -23,6 +23,12 @@ def _both(gpu, oracle, pop, extra=0):
     return make_domain(gpu, pop, extra), make_domain(oracle, pop, extra)
 
 
+def _coll_variant(gpu, d, tiled):
+    """Collisions per gridbox (wave per gridbox) or tiled (64 gridboxes per wave)."""
+    t = abi.Tiles(1, 1, 1, 0, 0, 0, 0, 0, int(tiled))
+    assert gpu.lib.cleo_set_tiles(d.h, abi.ctypes_byref(t)) == 0
+
+
 def test_upload_sort_random_order(gpu, oracle):
     _say("case: generic sort of shuffled drops with out-of-domain keys")
     pop = P.coal_breakup(SEED, ndims=(4, 5, 3), mean=20)
@@ -58,11 +64,13 @@ def test_condensation(gpu, oracle, ndims, mean):
     assert sg["cond_drops"] == so["cond_drops"] == pop.n
 
 
+@pytest.mark.parametrize("tiled", [0, 1])
 @pytest.mark.parametrize("enact", [abi.ENACT_COAL, abi.ENACT_BREAKUP, abi.ENACT_COALBURE])
-def test_collisions_injected(gpu, oracle, enact):
-    _say(f"case: collisions injected enact={enact}")
+def test_collisions_injected(gpu, oracle, enact, tiled):
+    _say(f"case: collisions injected enact={enact} tiled={tiled}")
     pop = P.coal_breakup(SEED, ndims=(4, 4, 2), mean=128)
     g, o = _both(gpu, oracle, pop)
+    _coll_variant(gpu, g, tiled)
     rng = np.random.default_rng(7)
     for step in range(3):
         js, ph = injected_stream(o.refs(), rng)
@@ -76,10 +84,12 @@ def test_collisions_injected(gpu, oracle, enact):
     assert g.stats()["coll_events"] == o.stats()["coll_events"]
 
 
-def test_collisions_counter_rng_golovin_box(gpu, oracle):
-    _say("case: Golovin box 8192 drops, counter RNG, 5 steps (HBM scratch path)")
+@pytest.mark.parametrize("tiled", [0, 1])
+def test_collisions_counter_rng_golovin_box(gpu, oracle, tiled):
+    _say(f"case: Golovin box 8192 drops, counter RNG, 5 steps (HBM scratch path) tiled={tiled}")
     pop = P.golovin_box(SEED)
     g, o = _both(gpu, oracle, pop)
+    _coll_variant(gpu, g, tiled)
     g.rng_counter(99)
     o.rng_counter(99)
     p = coll_params(prob=abi.PROB_GOLOVIN, enact=abi.ENACT_COAL)
@@ -92,12 +102,14 @@ def test_collisions_counter_rng_golovin_box(gpu, oracle):
     assert sg["coll_events"] == so["coll_events"] and sg["coll_events"] > 0
 
 
-def test_nulls_and_resort(gpu, oracle):
-    _say("case: xi=1 twins produce nulls and a re-sort")
+@pytest.mark.parametrize("tiled", [0, 1])
+def test_nulls_and_resort(gpu, oracle, tiled):
+    _say(f"case: xi=1 twins produce nulls and a re-sort, tiled={tiled}")
     pop = P.coal_breakup(SEED, ndims=(3, 3, 2), mean=40)
     pop.supers["xi"][:] = 1
     pop.supers["radius"][:] = 30.0
     g, o = _both(gpu, oracle, pop)
+    _coll_variant(gpu, g, tiled)
     g.rng_counter(5)
     o.rng_counter(5)
     p = coll_params(prob=abi.PROB_CONST, param=1e30, enact=abi.ENACT_COAL)
@@ -109,6 +121,27 @@ def test_nulls_and_resort(gpu, oracle):
         assert np.array_equal(g.refs(), o.refs())
     assert o.stats()["nulls"] > 0
     assert g.stats()["nulls"] == o.stats()["nulls"]
+
+
+@pytest.mark.parametrize("ndims,mean", [((4, 4, 4), 200), ((2, 2, 2), 300), ((9, 5, 3), 3)])
+def test_collisions_tiled_mixed_sizes(gpu, oracle, ndims, mean):
+    _say(f"case: tiled collisions, ndims={ndims} mean={mean} (tile LDS overflow, >255-drop gridboxes)")
+    pop = P.coal_breakup(SEED, ndims=ndims, mean=mean)
+    g, o = _both(gpu, oracle, pop)
+    _coll_variant(gpu, g, 1)
+    rng = np.random.default_rng(11)
+    p = coll_params(enact=abi.ENACT_COALBURE, prob=abi.PROB_LONG)
+    for step in range(2):
+        js, ph = injected_stream(o.refs(), rng)
+        g.rng_injected(js, ph)
+        o.rng_injected(js, ph)
+        assert g.collisions(p) == 0
+        assert o.collisions(p) == 0
+        assert g.error()[0] == 0 and o.error()[0] == 0
+        assert_supers_match(g.download(), o.download(), what=f"tiled step {step}")
+    sg, so = g.stats(), o.stats()
+    for k in ("coll_pairs", "coll_events", "nulls"):
+        assert sg[k] == so[k], k
 
 
 def test_motion(gpu, oracle):
@@ -184,9 +217,10 @@ def test_tiles_do_not_change_results(gpu):
     s = sdm_params(cond=cond_params(), coll=coll_params(), motion=motion_params())
     outs, stats = [], []
     # the small grids make every wave loop over several gridboxes (or drop chunks)
-    for wpb, flat, gw in ((1, 0, 0), (2, 0, 0), (4, 0, 0), (1, 1, 0), (1, 0, 64), (2, 1, 64), (4, 0, 8), (1, 1, 4)):
+    for wpb, flat, gw in ((1, 0, 0), (2, 0, 0), (4, 0, 0), (1, 1, 0), (1, 0, 64), (2, 1, 64), (4, 0, 8), (1, 1, 4),
+                          (4, 1, 0), (2, 1, 1)):
         d = make_domain(gpu, pop)
-        t = abi.Tiles(wpb, wpb, wpb, flat, flat, gw, gw, gw)
+        t = abi.Tiles(wpb, wpb, wpb, flat, flat, gw, gw, gw, flat)
         assert gpu.lib.cleo_set_tiles(d.h, abi.ctypes_byref(t)) == 0
         d.set_sdm(s)
         d.rng_counter(11)
