@@ -27,17 +27,44 @@ METRIC = "superdroplet-steps/sec on 1 GPU (plain vs MLP warm-start), Newton iter
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
-# algorithmic HBM bytes (per drop, per gridbox) of each kernel's launch
+# algorithmic HBM bytes (per drop, per gridbox) of each stage's main kernel, by variant
 BYTES = {
     # read r, msol, xi; write r | read State (press, temp, qv, qc, vol), refs; write temp, qv, qc
     "k_condensation": (32, 5 * 8 + 8 + 3 * 8),
+    # read sdgbxindex, r, msol, xi; write r, dm | saturation ratio and diffusion factor, temp, refs
+    "k_cond_drops": (44, 2 * 8 + 8 + 8),
+    "k_cond_queue": (44, 2 * 8 + 8 + 8),
     # read 60-byte record; write 60-byte record in shuffled order | refs, nact, volume
     "k_collisions": (120, 8 + 4 + 8),
+    "k_coll_tiles": (120, 8 + 4 + 8),
     # read coords, radius, sdgbxindex; write coords, sdgbxindex | bounds, winds, counts
     "k_motion": (64, 6 * 8 + 6 * 8 + 28 * 4),
-    # read key + 60-byte record, write 60-byte record | counts, refs
-    "k_nsort_scatter": (124, 28 * 4 + 8 + 8),
+    "k_motion_flat": (64, 6 * 8 + 6 * 8 + 28 * 4),
 }
+
+
+def measured_traffic(kernel, ndims):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (profiles/traffic.json: KiB per dispatch, each scaled by what the counter reads for a copy of
+    known size), when it was measured on this workload; else None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if list(t.get("ndims", [])) != list(ndims) or kernel not in t.get("kernels", {}):
+        return None
+    k = t["kernels"][kernel]
+    return 1024.0 * (k["fetch_kib"] * t["fetch_scale"] + k["write_kib"] * t["write_scale"])
+
+
+def stage_kernel(stage, tiles):
+    """Name of the main kernel of a stage (0 condensation, 1 collisions, 2 motion) under a tiling."""
+    if stage == 0:
+        return {0: "k_condensation", 1: "k_cond_drops", 2: "k_cond_queue"}[tiles.cond_flat]
+    if stage == 1:
+        return "k_coll_tiles" if tiles.coll_flat else "k_collisions"
+    return "k_motion_flat" if tiles.motion_flat else "k_motion"
 
 
 def parse():
@@ -181,12 +208,18 @@ def run_gpu(args):
     rec(d.h, 5)
     stage_ms = dict(motion_sort_qcond=elapsed(d, 2, 3), condensation=elapsed(d, 3, 4) / reps,
                     collisions=elapsed(d, 4, 5))
+    # main kernel of each stage alone: HIP events around its launches on the library's stream
+    kernel_ms = {}
+    for stage in range(3):
+        ms = C.c_float()
+        d.check(L.cleo_time_kernel(d.h, stage, 3, C.byref(ms)), "time_kernel")
+        kernel_ms[stage_kernel(stage, tiles)] = float(ms.value)
     ngbx = int(np.prod(ndims))
     nd = ndomain(d)
     d.check(L.cleo_synchronize(d.h), "final sync")
     err = d.error()
     d.close()
-    return dict(n=n, ngbx=ngbx, ndims=ndims, plain=plain, stage_ms=stage_ms, nd=nd, tiles=tiles,
+    return dict(n=n, ngbx=ngbx, ndims=ndims, plain=plain, stage_ms=stage_ms, kernel_ms=kernel_ms, nd=nd, tiles=tiles,
                 gen_s=t_gen, learned=learned, err=err)
 
 
@@ -265,12 +298,15 @@ def main():
     ms_step = p["ms_total"] / args.steps
     value = p["drop_steps"] / (p["ms_total"] / 1e3)
     iters_plain = p["iters_per_drop"]
-    # dominant kernel: condensation (FP64 Newton); algorithmic bytes per launch
+    # dominant kernel: the stage kernel with the longest live HIP-event time; algorithmic bytes per
+    # launch over its mean launch duration
     nd = g["nd"]
-    bpd, bpg = BYTES["k_condensation"]
+    km = g["kernel_ms"]
+    dom = max(km, key=km.get)
+    bpd, bpg = BYTES[dom]
     alg_bytes = bpd * nd + bpg * g["ngbx"]
-    t_cond = g["stage_ms"]["condensation"] / 1e3
-    achieved = alg_bytes / t_cond / 1e9
+    achieved = alg_bytes / (km[dom] / 1e3) / 1e9
+    traffic = measured_traffic(dom, g["ndims"])
     cpu = None if args.no_cpu else run_cpu(args)
     out = {
         "metric": METRIC,
@@ -296,8 +332,10 @@ def main():
         "wall_s": p["wall_s"],
         "gen_s": g["gen_s"],
         "stage_ms": g["stage_ms"],
-        "roofline": {"bound": "hbm", "kernel": "k_condensation", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": alg_bytes, "kernel_ms": km[dom]},
+        "kernel_ms": km,
         "cpu_baseline": cpu,
     }
     if g["learned"]:

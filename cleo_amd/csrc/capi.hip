@@ -288,6 +288,7 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   if (!increasing(dom->zhalf, nz) || !increasing(dom->xhalf, nx) || !increasing(dom->yhalf, ny)) return CLEO_E_ARG;
   cleo_ctx* c = new cleo_ctx();
   HIP_TRY(c, hipGetDevice(&c->device));
+  HIP_TRY(c, hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, c->device));
   HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   c->grid.nz = static_cast<uint32_t>(nz);
   c->grid.nx = static_cast<uint32_t>(nx);
@@ -357,6 +358,8 @@ int cleo_destroy(cleo_ctx* c) {
     if (p) hipFree(p);
   if (c->h_ctl) hipHostFree(c->h_ctl);
   for (hipEvent_t& e : c->ev)
+    if (e) hipEventDestroy(e);
+  for (hipEvent_t& e : c->kev)
     if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -781,137 +784,115 @@ int cleo_set_tiles(cleo_ctx* c, const cleo_tiles* t) {
   if (!c || !t) return CLEO_E_ARG;
   for (uint32_t v : {t->cond_waves_per_block, t->coll_waves_per_block, t->motion_waves_per_block})
     if (v > 4) return fail(c, CLEO_E_ARG, "waves per block must be 0..4");
-  if (t->cond_flat > 1 || t->motion_flat > 1 || t->coll_flat > 1)
-    return fail(c, CLEO_E_ARG, "variant flags must be 0 or 1");
+  if (t->cond_flat > 2 || t->motion_flat > 1 || t->coll_flat > 1)
+    return fail(c, CLEO_E_ARG, "variant flags: cond_flat 0..2, motion_flat and coll_flat 0 or 1");
   c->tiles = *t;
   return CLEO_OK;
 }
 
-// Times each kernel at 1, 2 and 4 waves (gridboxes) per workgroup, the thread-per-drop variant,
-// and grids of one wave per gridbox or of 8192 / 32768 looping waves, on the resident population,
-// and keeps the fastest.  Drops and State are left unchanged: the kernels run on a copy in the spare
-// buffer, State fields that condensation writes are saved and restored.
-int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
-  if (!c) return CLEO_E_ARG;
-  if (c->ntot == 0) return fail(c, CLEO_E_STATE, "no superdroplets resident");
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
+// Runs one stage `reps` times with tiling `t` on a copy of the resident drops in the spare buffer
+// (collisions read the current buffer with timing_only set); State fields that condensation
+// writes, the active counts and the control block are restored afterwards.  Reports the fastest
+// and the mean duration, of the whole stage or (kernel_only) of its main kernel alone.
+static int time_stage(cleo_ctx* c, int kern, const cleo_tiles& t, int reps, bool kernel_only, float* tmin,
+                      float* tmean) {
   const uint32_t ngbx = c->grid.ngbx;
   const int cur = c->cur, spare = 1 - c->cur;
-  for (int s = 0; s < 2; ++s)
+  for (int s = 0; s < 2; ++s) {
     if (!c->ev[14 + s]) HIP_TRY(c, hipEventCreate(&c->ev[14 + s]));
-  auto copy_store = [&]() -> hipError_t {
-    const SoA& a = c->buf[cur];
-    const SoA& b = c->buf[spare];
-    const uint64_t n = c->ntot;
-    hipError_t e;
-    if ((e = hipMemcpyAsync(b.gbx, a.gbx, n * 4, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(b.c3, a.c3, n * 8, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(b.c1, a.c1, n * 8, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(b.c2, a.c2, n * 8, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(b.xi, a.xi, n * 8, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(b.r, a.r, n * 8, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(b.msol, a.msol, n * 8, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess) return e;
-    return hipMemcpyAsync(b.id, a.id, n * 8, hipMemcpyDeviceToDevice, c->stream);
-  };
-  // saved State fields (temp, qvap, qcond) and active counts
+    if (!c->kev[s]) HIP_TRY(c, hipEventCreate(&c->kev[s]));
+  }
   double* saved = nullptr;
   uint32_t* saved_nact = nullptr;
   HIP_TRY(c, dalloc(&saved, 3ull * ngbx));
   HIP_TRY(c, dalloc(&saved_nact, ngbx));
   HIP_TRY(c, hipMemcpyAsync(saved, c->st.temp, 3ull * ngbx * 8, hipMemcpyDeviceToDevice, c->stream));
   HIP_TRY(c, hipMemcpyAsync(saved_nact, c->nact, ngbx * 4ull, hipMemcpyDeviceToDevice, c->stream));
-  const cleo_cond_params cp = c->sdm_set ? c->sdm.cond
-                                         : cleo_cond_params{100, 1, 50, 1e-6, 1e-8, 0.001, 0, 10};
-  cleo_coll_params kp = c->sdm_set ? c->sdm.coll : cleo_coll_params{100, CLEO_PROB_LONG, 1.0, CLEO_ENACT_COALBURE, 5.0, 0.85};
-  const cleo_motion_params mp = c->sdm_set ? c->sdm.motion : cleo_motion_params{100, CLEO_TV_SIMMEL};
-  const uint64_t seed = c->seed;
   DevCtl saved_ctl;
   HIP_TRY(c, ctx_copy(c, &saved_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
-  float best[3] = {1e30f, 1e30f, 1e30f};
-  uint32_t pick[3] = {1, 1, 1};
-  uint32_t pick_flat[3] = {0, 0, 0};
-  uint32_t pick_gw[3] = {0, 0, 0};
+  const cleo_cond_params cp = c->sdm_set ? c->sdm.cond
+                                         : cleo_cond_params{100, 1, 50, 1e-6, 1e-8, 0.001, 0, 10};
+  const cleo_coll_params kp =
+      c->sdm_set ? c->sdm.coll : cleo_coll_params{100, CLEO_PROB_LONG, 1.0, CLEO_ENACT_COALBURE, 5.0, 0.85};
+  const cleo_motion_params mp = c->sdm_set ? c->sdm.motion : cleo_motion_params{100, CLEO_TV_SIMMEL};
   const cleo_tiles tiles0 = c->tiles;
+  c->tiles = t;
   int rc = CLEO_OK;
-  struct Cand {
-    uint32_t wpb, flat;
-  };
-  constexpr int kNC = 5;
-  const Cand cands[kNC] = {{1, 0}, {2, 0}, {4, 0}, {1, 1}, {4, 1}};
-  const uint32_t grid_waves[3] = {0, 8192, 32768};
-  for (int kern = 0; kern < 3 && rc == CLEO_OK; ++kern) {
-    for (int gi = 0; gi < 3 * kNC; ++gi) {
-      const Cand& cd = cands[gi % kNC];
-      const uint32_t gw = grid_waves[gi / kNC];
-      if (kern != 1 && cd.flat && cd.wpb != 1) continue;  // thread-per-drop kernels ignore wpb
-      c->tiles.cond_grid_waves = c->tiles.coll_grid_waves = c->tiles.motion_grid_waves = gw;
-      c->tiles.coll_flat = cd.flat;
-      float tmin = 1e30f;
-      for (int rep = 0; rep < 2; ++rep) {
-        if (copy_store() != hipSuccess) { rc = fail(c, CLEO_E_HIP, "autotune copy"); break; }
-        hipEventRecord(c->ev[14], c->stream);
-        if (kern == 0) {
-          CondArgs a = cond_args(c, cp, false);
-          a.s = c->buf[spare];
-          a.wpb = cd.wpb;
-          a.niters = nullptr;
-          if (cd.flat) {
-            CondFlatArgs f{a.s, a.off, a.nact, a.st, a.sol, a.cfg, a.ngbx, a.alter, a.fuse_qcond, a.ctl, nullptr,
-                           nullptr, a.warm_limit, c->gs, c->dm};
-            launch_condensation_flat(c, f);
-          } else {
-            launch_condensation(c, a);
-          }
-        } else if (kern == 1) {
-          CollArgs a{};
-          a.src = c->buf[cur];
-          a.dst = c->buf[spare];
-          a.off = c->off;
-          a.nact = c->nact;
-          a.vol = c->st.vol;
-          a.sol = c->solute;
-          a.k = CollCfg{kp.prob_kind, kp.prob_param, kp.enact_kind, kp.nfrags, kp.coalbure_coaleff,
-                        step2realtime(kp.interval)};
-          a.ngbx = ngbx;
-          a.wpb = cd.wpb;
-          a.ctl = c->ctl;
-          a.seed = seed;
-          a.inj_oj = c->inj_oj;
-          a.inj_op = c->inj_op;
-          a.big = c->big_scratch;
-          a.timing_only = 1;
-          launch_collisions(c, a);
-        } else {
-          MotionArgs a{};
-          a.s = c->buf[spare];
-          a.off = c->off;
-          a.grid = c->grid;
-          a.st = c->st;
-          a.delt = step2dimlesstime(mp.interval);
-          a.tv = mp.terminalv;
-          a.wpb = cd.wpb;
-          a.cnt = c->cnt;
-          a.ctl = c->ctl;
-          if (cd.flat)
-            launch_motion_flat(c, a);
-          else
-            launch_motion(c, a);
-        }
-        hipEventRecord(c->ev[15], c->stream);
-        hipEventSynchronize(c->ev[15]);
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, c->ev[14], c->ev[15]);
-        tmin = std::min(tmin, ms);
-        HIP_TRY(c, hipMemcpyAsync(c->st.temp, saved, 3ull * ngbx * 8, hipMemcpyDeviceToDevice, c->stream));
-        HIP_TRY(c, hipMemcpyAsync(c->nact, saved_nact, ngbx * 4ull, hipMemcpyDeviceToDevice, c->stream));
-      }
-      if (tmin < best[kern]) {
-        best[kern] = tmin;
-        pick[kern] = cd.wpb;
-        pick_flat[kern] = cd.flat;
-        pick_gw[kern] = gw;
-      }
+  float best = 1e30f, total = 0.f;
+  for (int rep = 0; rep < reps && rc == CLEO_OK; ++rep) {
+    const SoA& a0 = c->buf[cur];
+    const SoA& b0 = c->buf[spare];
+    const uint64_t n = c->ntot;
+    hipError_t e = hipMemcpyAsync(b0.gbx, a0.gbx, n * 4, hipMemcpyDeviceToDevice, c->stream);
+    void* dst8[7] = {b0.c3, b0.c1, b0.c2, b0.xi, b0.r, b0.msol, b0.id};
+    const void* src8[7] = {a0.c3, a0.c1, a0.c2, a0.xi, a0.r, a0.msol, a0.id};
+    for (int i = 0; i < 7 && e == hipSuccess; ++i)
+      e = hipMemcpyAsync(dst8[i], src8[i], n * 8, hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) {
+      rc = fail(c, CLEO_E_HIP, "stage timing copy");
+      break;
     }
+    c->kev_on = kernel_only;
+    hipEventRecord(c->ev[14], c->stream);
+    if (kern == 0) {
+      CondArgs a = cond_args(c, cp, false);
+      a.s = c->buf[spare];
+      a.niters = nullptr;
+      if (t.cond_flat) {
+        CondFlatArgs f{a.s, a.off, a.nact, a.st, a.sol, a.cfg, a.ngbx, a.alter, a.fuse_qcond, a.ctl, nullptr,
+                       nullptr, a.warm_limit, c->gs, c->dm};
+        launch_condensation_flat(c, f);
+      } else {
+        launch_condensation(c, a);
+      }
+    } else if (kern == 1) {
+      CollArgs a{};
+      a.src = c->buf[cur];
+      a.dst = c->buf[spare];
+      a.off = c->off;
+      a.nact = c->nact;
+      a.vol = c->st.vol;
+      a.sol = c->solute;
+      a.k = CollCfg{kp.prob_kind, kp.prob_param, kp.enact_kind, kp.nfrags, kp.coalbure_coaleff,
+                    step2realtime(kp.interval)};
+      a.ngbx = ngbx;
+      a.wpb = t.coll_waves_per_block ? t.coll_waves_per_block : 1;
+      a.ctl = c->ctl;
+      a.seed = c->seed;
+      a.inj_oj = c->inj_oj;
+      a.inj_op = c->inj_op;
+      a.big = c->big_scratch;
+      a.timing_only = 1;
+      launch_collisions(c, a);
+    } else {
+      MotionArgs a{};
+      a.s = c->buf[spare];
+      a.off = c->off;
+      a.grid = c->grid;
+      a.st = c->st;
+      a.delt = step2dimlesstime(mp.interval);
+      a.tv = mp.terminalv;
+      a.wpb = t.motion_waves_per_block ? t.motion_waves_per_block : 1;
+      a.cnt = c->cnt;
+      a.ctl = c->ctl;
+      if (t.motion_flat)
+        launch_motion_flat(c, a);
+      else
+        launch_motion(c, a);
+    }
+    hipEventRecord(c->ev[15], c->stream);
+    c->kev_on = false;
+    hipEventSynchronize(c->ev[15]);
+    float ms = 0.f;
+    if (kernel_only)
+      hipEventElapsedTime(&ms, c->kev[0], c->kev[1]);
+    else
+      hipEventElapsedTime(&ms, c->ev[14], c->ev[15]);
+    best = std::min(best, ms);
+    total += ms;
+    if (hipMemcpyAsync(c->st.temp, saved, 3ull * ngbx * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(c->nact, saved_nact, ngbx * 4ull, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+      rc = fail(c, CLEO_E_HIP, "stage timing restore");
   }
   c->tiles = tiles0;
   hipStreamSynchronize(c->stream);
@@ -919,10 +900,65 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
   hipFree(saved_nact);
   HIP_TRY(c, ctx_copy(c, c->ctl, &saved_ctl, sizeof(DevCtl), hipMemcpyHostToDevice));
   if (rc != CLEO_OK) return rc;
-  c->tiles = cleo_tiles{pick[0], pick[1], pick[2], pick_flat[0], pick_flat[2], pick_gw[0], pick_gw[1], pick_gw[2],
-                        pick_flat[1]};
+  if (tmin) *tmin = best;
+  if (tmean) *tmean = total / static_cast<float>(reps);
+  return CLEO_OK;
+}
+
+// Times each stage at 1, 2 and 4 waves (gridboxes) per workgroup, the thread-per-drop, tiled and
+// task-ring variants, and grids of one wave per gridbox or of 8192 / 32768 looping waves, on the
+// resident population, and keeps the fastest.  Drops and State are left unchanged.
+int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
+  if (!c) return CLEO_E_ARG;
+  if (c->ntot == 0) return fail(c, CLEO_E_STATE, "no superdroplets resident");
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  struct Cand {
+    uint32_t wpb, flat;
+  };
+  constexpr int kNC = 6;
+  const Cand cands[kNC] = {{1, 0}, {2, 0}, {4, 0}, {1, 1}, {4, 1}, {1, 2}};
+  const uint32_t grid_waves[3] = {0, 8192, 32768};
+  cleo_tiles best_t = c->tiles;
+  for (int kern = 0; kern < 3; ++kern) {
+    float best = 1e30f;
+    for (int gi = 0; gi < 3 * kNC; ++gi) {
+      const Cand& cd = cands[gi % kNC];
+      const uint32_t gw = grid_waves[gi / kNC];
+      if (kern != 1 && cd.flat && cd.wpb != 1) continue;  // thread-per-drop kernels ignore wpb
+      if (kern != 0 && cd.flat == 2) continue;              // the task ring is condensation's
+      cleo_tiles t = best_t;
+      if (kern == 0) {
+        t.cond_waves_per_block = cd.wpb;
+        t.cond_flat = cd.flat;
+        t.cond_grid_waves = gw;
+      } else if (kern == 1) {
+        t.coll_waves_per_block = cd.wpb;
+        t.coll_flat = cd.flat;
+        t.coll_grid_waves = gw;
+      } else {
+        t.motion_waves_per_block = cd.wpb;
+        t.motion_flat = cd.flat;
+        t.motion_grid_waves = gw;
+      }
+      float tmin = 0.f;
+      const int rc = time_stage(c, kern, t, 2, false, &tmin, nullptr);
+      if (rc != CLEO_OK) return rc;
+      if (tmin < best) {
+        best = tmin;
+        best_t = t;
+      }
+    }
+  }
+  c->tiles = best_t;
   if (chosen) *chosen = c->tiles;
   return CLEO_OK;
+}
+
+int cleo_time_kernel(cleo_ctx* c, int32_t stage, uint32_t reps, float* mean_ms) {
+  if (!c || !mean_ms || stage < 0 || stage > 2 || reps == 0) return CLEO_E_ARG;
+  if (c->ntot == 0) return fail(c, CLEO_E_STATE, "no superdroplets resident");
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return time_stage(c, stage, c->tiles, static_cast<int>(reps), true, nullptr, mean_ms);
 }
 
 int cleo_get_tiles(cleo_ctx* c, cleo_tiles* t) {

@@ -56,6 +56,7 @@ struct MLPDev;  // mlp.hip
 
 struct cleo_ctx {
   int device = 0;
+  int num_cu = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   cleo::Grid grid{};
   double* d_halves = nullptr;
@@ -113,6 +114,9 @@ struct cleo_ctx {
   cleo::MLPDev* mlp = nullptr;
   // timing events on `stream`
   hipEvent_t ev[16] = {};
+  // events around the main kernel of a stage, recorded by the launchers while kev_on is set
+  hipEvent_t kev[2] = {};
+  bool kev_on = false;
   std::string msg;
 };
 // Copies and fills of a context run in order on its stream and have completed when they return.

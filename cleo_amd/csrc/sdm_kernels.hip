@@ -104,7 +104,7 @@ __device__ __forceinline__ void cond_gbx(const CondArgs& a, uint32_t g, int lane
   acc[3] += restarts;
 }
 template <bool kFuseQcond>
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_condensation(CondArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_condensation(CondArgs a) {
   const int lane = lane_id();
   uint32_t acc[4] = {0, 0, 0, 0};
   for (uint32_t g = wave_index(); g < a.ngbx; g += wave_count()) cond_gbx<kFuseQcond>(a, g, lane, acc);
@@ -530,6 +530,33 @@ __global__ void __launch_bounds__(256) k_stats_reduce(const uint32_t* sp, uint32
 }
 
 // ============================ motion ==========================================================
+// Source gridbox that reaches `d` by the neighbour step `s` (s = (dz+1) + 3(dx+1) + 9(dy+1)),
+// or kOOB when the step would come from outside a finite axis.
+__device__ __forceinline__ uint32_t source_of(const Grid& gr, uint32_t d, int s) {
+  const int dz = s % 3 - 1, dx = (s / 3) % 3 - 1, dy = s / 9 - 1;
+  int jz = static_cast<int>(d % gr.nz) - dz;
+  int jx = static_cast<int>((d / gr.nz) % gr.nx) - dx;
+  int jy = static_cast<int>(d / (gr.nz * gr.nx)) - dy;
+  const int nz = static_cast<int>(gr.nz), nx = static_cast<int>(gr.nx), ny = static_cast<int>(gr.ny);
+  if (jz < 0 || jz >= nz) {
+    if (!gr.per[0]) return kOOB;
+    jz = (jz + nz) % nz;
+  }
+  if (jx < 0 || jx >= nx) {
+    if (!gr.per[1]) return kOOB;
+    jx = (jx + nx) % nx;
+  }
+  if (jy < 0 || jy >= ny) {
+    if (!gr.per[2]) return kOOB;
+    jy = (jy + ny) % ny;
+  }
+  return static_cast<uint32_t>(jz + nz * (jx + nx * jy));
+}
+
+// gridbox reached from g by the step s (kOOB off a finite axis)
+__device__ __forceinline__ uint32_t neighbour_of(const Grid& gr, uint32_t g, int s) { return source_of(gr, g, 26 - s); }
+
+
 __device__ __forceinline__ double interpolation(double blo, double bhi, double vlo, double vhi, double sd) {
   const double coord = fmin(bhi, fmax(blo, sd));  // predcorr.cpp:31-42
   const double alpha = (coord - blo) / (bhi - blo);
@@ -651,10 +678,15 @@ __device__ __forceinline__ uint32_t motion_gbx(const MotionArgs& a, uint32_t g, 
       mask &= ~m;
     }
   }
-  if (lane < 28) a.cnt[28ull * g + lane] = mycount;
+  // destination-major counts: row d column s = drops arriving at d by the step s; column 27 of the
+  // source's own row = drops leaving the domain (the table is zero-filled before motion)
+  if (lane == 27)
+    a.cnt[28ull * g + 27] = mycount;
+  else if (lane < 27 && mycount > 0)
+    a.cnt[28ull * neighbour_of(a.grid, g, lane) + lane] = mycount;
   return nfull;
 }
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(6) k_motion(MotionArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_motion(MotionArgs a) {
   const int lane = lane_id();
   uint32_t moved = 0;
   for (uint32_t g = wave_index(); g < a.grid.ngbx; g += wave_count()) moved += motion_gbx(a, g, lane);
@@ -662,46 +694,36 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(6) k_motion(MotionArgs a) 
 }
 
 // ============================ neighbour sort ==================================================
-// Source gridbox that reaches `d` by the neighbour step `s` (s = (dz+1) + 3(dx+1) + 9(dy+1)),
-// or kOOB when the step would come from outside a finite axis.
-__device__ __forceinline__ uint32_t source_of(const Grid& gr, uint32_t d, int s) {
-  const int dz = s % 3 - 1, dx = (s / 3) % 3 - 1, dy = s / 9 - 1;
-  int jz = static_cast<int>(d % gr.nz) - dz;
-  int jx = static_cast<int>((d / gr.nz) % gr.nx) - dx;
-  int jy = static_cast<int>(d / (gr.nz * gr.nx)) - dy;
-  const int nz = static_cast<int>(gr.nz), nx = static_cast<int>(gr.nx), ny = static_cast<int>(gr.ny);
-  if (jz < 0 || jz >= nz) {
-    if (!gr.per[0]) return kOOB;
-    jz = (jz + nz) % nz;
-  }
-  if (jx < 0 || jx >= nx) {
-    if (!gr.per[1]) return kOOB;
-    jx = (jx + nx) % nx;
-  }
-  if (jy < 0 || jy >= ny) {
-    if (!gr.per[2]) return kOOB;
-    jy = (jy + ny) % ny;
-  }
-  return static_cast<uint32_t>(jz + nz * (jx + nx * jy));
-}
-
 __device__ __forceinline__ bool guard_off(const uint32_t* flag) { return flag && *flag == 0u; }
 
-// drops arriving in each destination gridbox (thread per destination)
+// Per destination d (thread per d): the drops arriving in d, and in place of each count of row d
+// the drops that reach d from source gridboxes of lower index (the stable order puts them first).
 __global__ void k_nsort_tot(NsortArgs a) {
   if (guard_off(a.flag)) return;
   const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= a.grid.ngbx) return;
+  uint32_t* row = a.cnt + 28ull * d;
+  uint32_t cnt[27], src[27];
   uint32_t t = 0;
+#pragma unroll
   for (int s = 0; s < 27; ++s) {
-    const uint32_t src = source_of(a.grid, d, s);
-    if (src != kOOB) t += a.cnt[28ull * src + s];
+    cnt[s] = row[s];
+    src[s] = source_of(a.grid, d, s);
+    t += cnt[s];
+  }
+#pragma unroll
+  for (int s = 0; s < 27; ++s) {
+    uint32_t before = 0;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) before += src[q] < src[s] ? cnt[q] : 0u;
+    row[s] = before;
   }
   a.tot[d] = t;
 }
 
 // Stable scatter (sortsupers.hpp:63-76,167-193 under Serial): drop p of source gridbox g with
-// key d lands at  base[d] + (drops reaching d from gridboxes < g) + (earlier drops of g with key d).
+// key d lands at  base[d] + (drops reaching d from gridboxes < g) + (earlier drops of g with key d);
+// the middle term is row d of the count table after k_nsort_tot, at the step g -> d.
 __device__ __forceinline__ void nsort_scatter_gbx(const NsortArgs& a, uint32_t g, int lane) {
   const uint32_t sbase = a.off[g];
   const uint32_t nfull = a.off[g + 1] - sbase;
@@ -747,12 +769,15 @@ __device__ __forceinline__ void nsort_scatter_gbx(const NsortArgs& a, uint32_t g
                 near(vy, static_cast<int>(gy), static_cast<int>(a.grid.ny), a.grid.per[2]))) {
             if (lane == 0) set_error(a.ctl, CLEO_ERR_SORT_NONLOCAL, g, sbase + c + leader);
           }
-          uint32_t part = 0;
-          if (lane < 27) {
-            const uint32_t src = source_of(a.grid, v, lane);
-            if (src != kOOB && src < g) part = a.cnt[28ull * src + lane];
-          }
-          start = a.base[v] + wave_sum_u32(part);
+          // the step g -> v, then the start k_nsort_tot left in row v
+          auto step = [](int q, int r, int nd) {
+            const int dd = q - r;
+            return dd > 1 ? -1 : (dd < -1 ? 1 : dd);
+          };
+          const int sv = (step(vz, static_cast<int>(gz), static_cast<int>(a.grid.nz)) + 1) +
+                         3 * (step(vx, static_cast<int>(gx), static_cast<int>(a.grid.nx)) + 1) +
+                         9 * (step(vy, static_cast<int>(gy), static_cast<int>(a.grid.ny)) + 1);
+          start = a.base[v] + a.cnt[28ull * v + sv];
         }
         if (lane == slot) tkey = v;
       }
@@ -1010,6 +1035,12 @@ static inline unsigned nblocks(uint64_t n, uint64_t per) {
   const uint64_t b = (n + per - 1) / per;
   return static_cast<unsigned>(b == 0 ? 1 : b);
 }
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v);
+
+// events around the main kernel of a stage (stage timing), recorded only while kev_on is set
+static inline void kev_mark(cleo_ctx* c, int i) {
+  if (c->kev_on) (void)hipEventRecord(c->kev[i], c->stream);
+}
 // blocks of a gridbox loop: one wave per gridbox, capped at `grid_waves` waves when non-zero
 static inline unsigned gbx_blocks(uint64_t ngbx, uint32_t wpb, uint32_t grid_waves) {
   const unsigned b = nblocks(ngbx, wpb);
@@ -1035,10 +1066,12 @@ void launch_condensation(cleo_ctx* c, const CondArgs& args) {
   CondArgs a = args;
   const unsigned nb = gbx_blocks(a.ngbx, a.wpb, c->tiles.cond_grid_waves);
   a.stat = stat_slots(c, static_cast<uint64_t>(nb) * a.wpb);
+  kev_mark(c, 0);
   if (a.fuse_qcond)
     hipLaunchKernelGGL(k_condensation<true>, dim3(nb), dim3(64 * a.wpb), 0, c->stream, a);
   else
     hipLaunchKernelGGL(k_condensation<false>, dim3(nb), dim3(64 * a.wpb), 0, c->stream, a);
+  kev_mark(c, 1);
   launch_stats_reduce(c, nb * a.wpb, ST_COND_DROPS, 4);
 }
 void launch_qcond(cleo_ctx* c, const QcondArgs& a) {
@@ -1052,14 +1085,18 @@ void launch_collisions(cleo_ctx* c, const CollArgs& args) {
   if (c->tiles.coll_flat) {
     const unsigned nb = gbx_blocks((a.ngbx + kWave - 1) / kWave, a.wpb, c->tiles.coll_grid_waves);
     a.stat = stat_slots(c, static_cast<uint64_t>(nb) * a.wpb);
+    kev_mark(c, 0);
     hipLaunchKernelGGL(k_coll_tiles, dim3(nb), dim3(64 * a.wpb), a.wpb * kCollTileLdsBytes, c->stream, a);
+    kev_mark(c, 1);
     launch_stats_reduce(c, nb * a.wpb, ST_COLL_PAIRS, 3);
     return;
   }
   const unsigned nb = gbx_blocks(a.ngbx, a.wpb, c->tiles.coll_grid_waves);
   a.stat = stat_slots(c, static_cast<uint64_t>(nb) * a.wpb);
   const size_t lds = static_cast<size_t>(a.wpb) * 2u * kCollLdsCap * sizeof(uint32_t);
+  kev_mark(c, 0);
   hipLaunchKernelGGL(k_collisions, dim3(nb), dim3(64 * a.wpb), lds, c->stream, a);
+  kev_mark(c, 1);
   launch_stats_reduce(c, nb * a.wpb, ST_COLL_PAIRS, 3);
 }
 void launch_motion(cleo_ctx* c, const MotionArgs& args) {
@@ -1067,7 +1104,11 @@ void launch_motion(cleo_ctx* c, const MotionArgs& args) {
   MotionArgs a = args;
   const unsigned nb = gbx_blocks(a.grid.ngbx, a.wpb, c->tiles.motion_grid_waves);
   a.stat = stat_slots(c, static_cast<uint64_t>(nb) * a.wpb);
+  hipLaunchKernelGGL(k_fill_u32, dim3(stride_blocks(28ull * a.grid.ngbx)), dim3(256), 0, c->stream, a.cnt,
+                     28ull * a.grid.ngbx, 0u);
+  kev_mark(c, 0);
   hipLaunchKernelGGL(k_motion, dim3(nb), dim3(64 * a.wpb), 0, c->stream, a);
+  kev_mark(c, 1);
   launch_stats_reduce(c, nb * a.wpb, ST_MOTION_DROPS, 1);
 }
 void launch_stats_reduce(cleo_ctx* c, uint32_t nslot, int first, int ncount) {
@@ -1226,7 +1267,7 @@ __device__ __forceinline__ void cond_prep_gbx(const CondFlatArgs& a, uint32_t g,
     a.gs[2 * g + 1] = diffusion_factor(press, temp, psat);
   }
 }
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(8) k_cond_prep(CondFlatArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(6) k_cond_prep(CondFlatArgs a) {
   const int lane = lane_id();
   for (uint32_t g = wave_index(); g < a.ngbx; g += wave_count()) cond_prep_gbx(a, g, lane);
 }
@@ -1275,6 +1316,234 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_drops(CondFlatAr
   }
 }
 
+// ---- condensation with a per-wave task ring ------------------------------------------------------
+// Newton iteration counts differ from drop to drop (2 for about a quarter, 5-6 for most, tens for
+// a few), so 64 drops solved in lockstep leave most lanes idle.  Here each lane runs the solve of
+// its drop as a state machine, one g-evaluation per tick, and a lane whose drop is done takes the
+// next prepared task from a ring in LDS; a setup pass prepares 64 drops at a time whenever fewer
+// than 64 tasks wait.  Every drop goes through exactly the operations of solve_condensation
+// (initial evaluation, two iterations, restart until converged, sub-steps, learned guess), so the
+// result does not depend on the scheduling.
+struct CondRing {
+  double s_ratio[kRing], akoh[kRing], bkoh[kRing], inv_ffv[kRing], r0[kRing], zplain[kRing], zwarm[kRing],
+      mindelt[kRing], dryr[kRing], oldm[kRing], msol[kRing];
+  uint64_t xi[kRing];
+  uint32_t k[kRing], unique[kRing];
+};
+static_assert(sizeof(CondRing) == kCondRingBytes, "ring size");
+
+enum : uint32_t { PH_EVAL = 0, PH_FIRST = 1, PH_SECOND = 2, PH_UNTIL = 3 };
+enum : uint32_t { MODE_PLAIN = 0, MODE_WARM = 1, MODE_SUB = 2 };
+
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_cond_queue(CondFlatArgs a) {
+  extern __shared__ uint32_t lds_dyn[];
+  CondRing& R = reinterpret_cast<CondRing*>(lds_dyn)[threadIdx.x >> 6];
+  const int lane = lane_id();
+  const NewtonCfg cfg = a.cfg;
+  const uint64_t ndom = a.off[a.ngbx];
+  const uint64_t nw = wave_count(), w = wave_index();
+  const uint64_t k_hi = ndom * (w + 1) / nw;
+  uint64_t next = ndom * w / nw;  // first drop not yet prepared (wave-uniform)
+  uint32_t head = 0, tail = 0;    // ring positions handed out / prepared (wave-uniform)
+  // lane state
+  bool busy = false, restarted = false, maxed = false;
+  uint32_t k = 0, phase = PH_EVAL, mode = MODE_PLAIN, niter = 0, limit = 0, iters = 0;
+  ODEConsts c{0, 0, 0, 0, 0};
+  double rprev = 0, r0 = 0, subdelt = 0, remdelt = 0, mindelt = 0, zplain = 0;
+  double z = 0, gz = 0, dgz = 0, z0 = 0, g0 = 0, dg0 = 0, dryr = 0, oldm = 0, msol = 0;
+  uint64_t xi = 0;
+  uint32_t st_solved = 0, st_iters = 0, st_subs = 0, st_restarts = 0;
+  while (true) {
+    if (tail - head < static_cast<uint32_t>(kWave) && next < k_hi) {
+      // setup pass: Koehler factors, ventilation, initial guess and uniqueness criteria of the next
+      // 64 drops (condensation.cpp:44-56, impliciteuler.cpp:35-54)
+      const uint64_t kk = next + static_cast<uint64_t>(lane);
+      bool act = false;
+      uint32_t g = 0;
+      if (kk < k_hi) {
+        g = a.s.gbx[kk];
+        act = g < a.ngbx && kk < a.off[g] + a.nact[g];
+      }
+      const uint64_t m = __ballot(act);
+      if (act) {
+        const uint32_t slot = (tail + static_cast<uint32_t>(__popcll(m & lanemask_lt()))) % kRing;
+        const double s_ratio = a.gs[2 * g], ffactor = a.gs[2 * g + 1], temp = a.st.temp[g];
+        const double r = a.s.r[kk], ms = a.s.msol[kk];
+        bool bad = false;
+        const double om = condensate_mass(r, ms, a.sol, &bad);
+        if (bad) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, g, kk);
+        double akoh, bkoh;
+        kohler_factors(ms, temp, a.sol, akoh, bkoh);
+        const double ffv = ffactor / ventilation_factor(r);
+        const ODEConsts cc{s_ratio, akoh, bkoh, ffv, 1.0 / ffv};
+        const double zg = initialguess(cc, r);
+        const double critdelt = critical_timestep(cc);
+        const bool uniq = first_unique_criteria(cc, r, zg) || (cfg.delt <= critdelt);
+        R.s_ratio[slot] = s_ratio;
+        R.akoh[slot] = akoh;
+        R.bkoh[slot] = bkoh;
+        R.inv_ffv[slot] = cc.inv_ffv;
+        R.r0[slot] = r;
+        R.zplain[slot] = zg;
+        R.zwarm[slot] = a.warm ? a.warm[kk] : __longlong_as_double(0x7FF8000000000000ll);
+        R.mindelt[slot] = uniq ? 0.0 : fmax(critdelt, cfg.minsubdelt);
+        R.dryr[slot] = dryradius(ms, a.sol);
+        R.oldm[slot] = om;
+        R.msol[slot] = ms;
+        R.xi[slot] = a.s.xi[kk];
+        R.k[slot] = static_cast<uint32_t>(kk);
+        R.unique[slot] = uniq ? 1u : 0u;
+      }
+      tail += static_cast<uint32_t>(__popcll(m));
+      next += kWave;
+      wave_sync();
+    }
+    // hand the waiting tasks to idle lanes
+    const uint64_t midle = __ballot(!busy);
+    const uint32_t avail = tail - head;
+    if (!busy) {
+      const uint32_t rank = static_cast<uint32_t>(__popcll(midle & lanemask_lt()));
+      if (rank < avail) {
+        const uint32_t slot = (head + rank) % kRing;
+        c = ODEConsts{R.s_ratio[slot], R.akoh[slot], R.bkoh[slot], 0.0, R.inv_ffv[slot]};
+        r0 = R.r0[slot];
+        zplain = R.zplain[slot];
+        mindelt = R.mindelt[slot];
+        dryr = R.dryr[slot];
+        oldm = R.oldm[slot];
+        msol = R.msol[slot];
+        xi = R.xi[slot];
+        k = R.k[slot];
+        const double zw = R.zwarm[slot];
+        rprev = r0;
+        iters = 0;
+        restarted = maxed = false;
+        if (R.unique[slot]) {
+          subdelt = cfg.delt;
+          if (zw == zw) {
+            mode = MODE_WARM;
+            limit = a.warm_limit;
+            z = zw;
+          } else {
+            mode = MODE_PLAIN;
+            limit = cfg.maxniters;
+            z = zplain;
+          }
+        } else {
+          mode = MODE_SUB;
+          limit = cfg.maxniters;
+          remdelt = cfg.delt;
+          subdelt = fmin(mindelt, remdelt);
+          z = zplain;
+        }
+        phase = PH_EVAL;
+        busy = true;
+      }
+    }
+    head += min(static_cast<uint32_t>(__popcll(midle)), avail);
+    wave_sync();
+    if (!__any(busy)) {
+      if (next >= k_hi && head == tail) break;
+      continue;
+    }
+    if (busy) {
+      // one evaluation of g: the initial state of a segment, or a Newton iteration
+      // (newton_iterate, impliciteuler.cpp:240-255)
+      double zn = z;
+      if (phase != PH_EVAL) zn = fmax(z * (1 - gz / dgz), 1e-8);
+      double g1, dg1;
+      ode_g_dg(c, subdelt, rprev, zn, g1, dg1);
+      bool segok = false, segfail = false;
+      if (phase == PH_EVAL) {
+        z0 = z = zn;
+        g0 = gz = g1;
+        dg0 = dgz = dg1;
+        phase = PH_FIRST;
+      } else {
+        ++iters;
+        const bool conv = fabs(g1 - gz) < cfg.rtol * fabs(g1) + cfg.atol;
+        z = zn;
+        gz = g1;
+        dgz = dg1;
+        if (phase == PH_FIRST) {
+          phase = PH_SECOND;
+        } else if (phase == PH_SECOND) {  // integrate_condensation_ode: restart unless converged
+          if (conv) {
+            segok = true;
+          } else {
+            z = z0;
+            gz = g0;
+            dgz = dg0;
+            phase = PH_UNTIL;
+            niter = 1;
+            segfail = niter > limit;
+          }
+        } else {  // newtonraphson_untilconverged (impliciteuler.cpp:207-227)
+          ++niter;
+          if (conv)
+            segok = true;
+          else
+            segfail = niter > limit;
+        }
+      }
+      if (segfail) {
+        if (mode == MODE_WARM) {  // the learned guess failed: the reference path from its guess
+          restarted = true;
+          mode = MODE_PLAIN;
+          limit = cfg.maxniters;
+          subdelt = cfg.delt;
+          rprev = r0;
+          z = zplain;
+          phase = PH_EVAL;
+        } else {
+          maxed = true;
+          segok = true;
+        }
+      }
+      if (segok) {
+        bool done = true;
+        if (mode == MODE_SUB) {  // substepping loop of solve_condensation
+          rprev = sqrt(z);
+          remdelt -= subdelt;
+          if (remdelt > 0.0) {
+            subdelt = fmin(mindelt, remdelt);
+            phase = PH_EVAL;
+            done = false;
+            if (!(subdelt > 0.0)) {  // no progress possible: stop with the reference's substep error
+              set_error(a.ctl, CLEO_ERR_SUBSTEP, a.s.gbx[k], k);
+              done = true;
+            }
+          }
+        }
+        if (done) {
+          const double rnew = fmax(sqrt(z), dryr);  // change_radius (superdrop_attrs.cpp:35-45)
+          a.s.r[k] = rnew;
+          bool bad = false;
+          a.dm[k] = (condensate_mass(rnew, msol, a.sol, &bad) - oldm) * static_cast<double>(xi);
+          if (bad) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, a.s.gbx[k], k);
+          if (maxed) set_error(a.ctl, CLEO_ERR_NEWTON_MAXITERS, a.s.gbx[k], k);
+          if (a.niters) a.niters[k] = iters;
+          st_solved += 1;
+          st_iters += iters;
+          st_subs += mode == MODE_SUB ? 1u : 0u;
+          st_restarts += restarted ? 1u : 0u;
+          busy = false;
+        }
+      }
+    }
+  }
+  st_solved = wave_sum_u32(st_solved);
+  st_iters = wave_sum_u32(st_iters);
+  st_subs = wave_sum_u32(st_subs);
+  st_restarts = wave_sum_u32(st_restarts);
+  if (lane == 0) {
+    put_stat(a.stat, 0, wave_index(), st_solved);
+    put_stat(a.stat, 1, wave_index(), st_iters);
+    put_stat(a.stat, 2, wave_index(), st_subs);
+    put_stat(a.stat, 3, wave_index(), st_restarts);
+  }
+}
+
 // per gridbox: Σ dm in drop order and the thermodynamic feedback (condensation.cpp:126-137)
 __device__ __forceinline__ void cond_state_gbx(const CondFlatArgs& a, uint32_t g, int lane) {
   const uint32_t base = a.off[g];
@@ -1308,7 +1577,8 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
 }
 
 // MoveSupersInGridboxesFunctor per drop; direction counts aggregated per wave, then atomics
-// one drop: move it and return its direction key g*28 + direction
+// one drop: move it and return its count-table key: 28*d + step for destination d, 28*g + 27 when
+// it leaves the domain
 __device__ __forceinline__ uint32_t motion_drop(const MotionArgs& a, uint64_t k) {
   uint32_t key;
   {
@@ -1363,11 +1633,11 @@ __device__ __forceinline__ uint32_t motion_drop(const MotionArgs& a, uint64_t k)
       if (bad) set_error(a.ctl, CLEO_ERR_BOUNDS, g, k);
       dir = (dz + 1) + 3 * (dx + 1) + 9 * (dy + 1);
     }
-    key = 28u * g + static_cast<uint32_t>(dir);
+    key = idx != kOOB ? 28u * idx + static_cast<uint32_t>(dir) : 28u * g + 27u;
   }
   return key;
 }
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(6) k_motion_flat(MotionArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_motion_flat(MotionArgs a) {
   const int lane = lane_id();
   const uint32_t ndom = a.off[a.grid.ngbx];
   uint32_t nmoved = 0;
@@ -1394,11 +1664,24 @@ void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& args) {
   if (args.ngbx == 0) return;
   CondFlatArgs a = args;
   const uint32_t gw = c->tiles.cond_grid_waves;
-  const unsigned nb = drop_blocks(c->ntot, gw);
-  a.stat = stat_slots(c, 4ull * nb);  // every wave of k_cond_drops writes its slot
   hipLaunchKernelGGL(k_cond_prep, dim3(gbx_blocks(a.ngbx, 4, gw)), dim3(256), 0, c->stream, a);
-  hipLaunchKernelGGL(k_cond_drops, dim3(nb), dim3(256), 0, c->stream, a);
-  launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
+  if (c->tiles.cond_flat == 2) {
+    // task-ring kernel: waves of 4 per block; by default as many waves as fit on the device at once
+    const uint64_t waves = gw ? gw : static_cast<uint64_t>(c->num_cu) * kCondQueueWavesPerCu;
+    const unsigned nb = static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(waves / 4, (c->ntot + 255) / 256)));
+    a.stat = stat_slots(c, 4ull * nb);
+    kev_mark(c, 0);
+    hipLaunchKernelGGL(k_cond_queue, dim3(nb), dim3(256), 4 * kCondRingBytes, c->stream, a);
+    kev_mark(c, 1);
+    launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
+  } else {
+    const unsigned nb = drop_blocks(c->ntot, gw);
+    a.stat = stat_slots(c, 4ull * nb);  // every wave of k_cond_drops writes its slot
+    kev_mark(c, 0);
+    hipLaunchKernelGGL(k_cond_drops, dim3(nb), dim3(256), 0, c->stream, a);
+    kev_mark(c, 1);
+    launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
+  }
   hipLaunchKernelGGL(k_cond_state, dim3(gbx_blocks(a.ngbx, 4, gw)), dim3(256), 0, c->stream, a);
 }
 
@@ -1409,7 +1692,9 @@ void launch_motion_flat(cleo_ctx* c, const MotionArgs& args) {
   a.stat = stat_slots(c, 4ull * nb);  // every wave of k_motion_flat writes its slot
   hipLaunchKernelGGL(k_fill_u32, dim3(stride_blocks(28ull * a.grid.ngbx)), dim3(256), 0, c->stream, a.cnt,
                      28ull * a.grid.ngbx, 0u);
+  kev_mark(c, 0);
   hipLaunchKernelGGL(k_motion_flat, dim3(nb), dim3(256), 0, c->stream, a);
+  kev_mark(c, 1);
   launch_stats_reduce(c, 4u * nb, ST_MOTION_DROPS, 1);
 }
 

@@ -11,6 +11,10 @@ constexpr int kStatSlotsMax = 4;       // counters per wave of one kernel
 constexpr uint32_t kTileMaxN = 255;
 constexpr uint32_t kTilePermBytes = 8192;
 constexpr uint32_t kCollTileLdsBytes = kTilePermBytes + 4u * (2u * 65u + 4u * 64u);
+// condensation task ring: prepared drops per wave, and waves per CU of the default grid
+constexpr uint32_t kRing = 128;
+constexpr uint32_t kCondRingBytes = kRing * (11u * 8u + 8u + 2u * 4u);
+constexpr uint32_t kCondQueueWavesPerCu = 12;
 
 struct CondArgs {
   SoA s;
@@ -77,7 +81,7 @@ struct NsortArgs {
   SoA src, dst;
   const uint32_t* off;     // current refs (ngbx+1)
   const uint32_t* base;    // new refs (ngbx+1)
-  const uint32_t* cnt;     // ngbx*28
+  uint32_t* cnt;           // ngbx*28: arrivals per (destination, step), then start offsets
   const uint32_t* oobpre;  // ngbx+1
   uint32_t* tot;
   Grid grid;

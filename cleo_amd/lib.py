@@ -38,6 +38,7 @@ def load():
         lib.cleo_event_elapsed.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_float)]
         lib.cleo_ndomain.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         lib.cleo_autotune.argtypes = [C.c_void_p, C.c_void_p]
+        lib.cleo_time_kernel.argtypes = [C.c_void_p, C.c_int32, C.c_uint32, C.POINTER(C.c_float)]
         _backend = b
     return _backend
 

@@ -261,7 +261,9 @@ typedef struct {
   uint32_t cond_waves_per_block;   /* gridboxes (waves) per workgroup, 1..4 */
   uint32_t coll_waves_per_block;
   uint32_t motion_waves_per_block;
-  uint32_t cond_flat;              /* 1: one thread per drop + per-gridbox prep/sum kernels */
+  uint32_t cond_flat;              /* 1: one thread per drop + per-gridbox prep/sum kernels;
+                                      2: the same with each wave's drops fed to its lanes from a
+                                         task ring (lanes do not wait for slower drops) */
   uint32_t motion_flat;            /* 1: one thread per drop, direction counts by atomics */
   uint32_t cond_grid_waves;        /* waves per launch; each loops over gridboxes (or drops) */
   uint32_t coll_grid_waves;        /*   0: one wave per gridbox (one thread per drop) */
@@ -272,6 +274,10 @@ int cleo_set_tiles(cleo_ctx* ctx, const cleo_tiles* tiles);
 int cleo_get_tiles(cleo_ctx* ctx, cleo_tiles* tiles);
 /* Time each candidate tiling on the resident population and keep the fastest. */
 int cleo_autotune(cleo_ctx* ctx, cleo_tiles* chosen);
+/* Mean duration, by HIP events on the context's stream, of the main kernel of one stage over
+   `reps` launches with the current tiling, on a copy of the resident drops (drops and State stay
+   unchanged): stage 0 condensation (the per-drop kernel), 1 collisions, 2 motion. */
+int cleo_time_kernel(cleo_ctx* ctx, int32_t stage, uint32_t reps, float* mean_ms);
 
 /* ---- learned warm start ------------------------------------------------------------- */
 /* MLP 6 -> 64 -> 64 -> 1, SiLU, BF16 weights, FP32 accumulate. Inputs per drop:

@@ -36,7 +36,8 @@ def test_struct_layouts_match_header():
 
 def test_oracle_exports_mirror(oracle):
     names = [n.replace("cleo_", "orc_") for n in declared()
-             if not n.startswith(("cleo_mlp", "cleo_event", "cleo_autotune", "cleo_set_tiles", "cleo_get_tiles",
+             if not n.startswith(("cleo_mlp", "cleo_event", "cleo_autotune", "cleo_time_kernel", "cleo_set_tiles",
+                                  "cleo_get_tiles",
                                   "cleo_device_info", "cleo_upload_supers_aos", "cleo_download_supers_aos",
                                   "cleo_upload_supers_device", "cleo_run_steps_graph", "cleo_synchronize",
                                   "cleo_record_guess", "cleo_download_guess", "cleo_ndomain"))]

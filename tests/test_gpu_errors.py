@@ -28,12 +28,16 @@ def test_cfl_violation(gpu, oracle):
         assert _code(d, rc) == 8
 
 
-def test_newton_maxiters(gpu, oracle):
-    print("case: Newton maxniters abort", flush=True)
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_newton_maxiters(gpu, oracle, variant):
+    print(f"case: Newton maxniters abort, condensation variant {variant}", flush=True)
     pop = P.condensation(SEED, ndims=(2, 2, 2), mean=20)
     pop.state["qvap"] *= 1.01
     p = cond_params(maxniters=1, rtol=0.0, atol=0.0)
-    for d in _both(gpu, oracle, pop):
+    g, o = _both(gpu, oracle, pop)
+    t = abi.Tiles(1, 1, 1, variant, 0, 0, 0, 0, 0)
+    assert gpu.lib.cleo_set_tiles(g.h, abi.ctypes_byref(t)) == 0
+    for d in (g, o):
         rc = d.condensation(p)
         assert _code(d, rc) == 4
 

@@ -83,11 +83,15 @@ def test_mlp_gradient_vs_oracle(gpu):
     assert rel < 5e-2 and cos > 0.998
 
 
-def test_training_and_warm_start(gpu, oracle):
-    _say("case: train on solver samples, warm-started condensation vs oracle with the same guesses")
+@pytest.mark.parametrize("variant", [0, 2])
+def test_training_and_warm_start(gpu, oracle, variant):
+    _say(f"case: train on solver samples, warm-started condensation vs oracle with the same guesses, "
+         f"condensation variant {variant}")
     pop = P.condensation(SEED, ndims=(8, 8, 8), mean=40)
     pop.state["qvap"] *= 1.005
     d = make_domain(gpu, pop)
+    t = abi.Tiles(1, 1, 1, variant, 0, 0, 0, 0, 0)
+    assert gpu.lib.cleo_set_tiles(d.h, abi.ctypes_byref(t)) == 0
     cp = cond_params()
     n = pop.n
     xd = torch.zeros((n, 8), dtype=torch.float32, device="cuda")

@@ -43,11 +43,14 @@ def test_upload_sort_random_order(gpu, oracle):
     assert np.array_equal(g.refs(), o.refs())
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("ndims,mean", [((4, 4, 4), 122), ((2, 3, 2), 7)])
-def test_condensation(gpu, oracle, ndims, mean):
-    _say(f"case: condensation ndims={ndims} mean={mean}")
+def test_condensation(gpu, oracle, ndims, mean, variant):
+    _say(f"case: condensation ndims={ndims} mean={mean} variant={variant}")
     pop = P.condensation(SEED, ndims=ndims, mean=mean)
     g, o = _both(gpu, oracle, pop)
+    t = abi.Tiles(1, 1, 1, variant, 0, 4 if variant == 2 else 0, 0, 0, 0)
+    assert gpu.lib.cleo_set_tiles(g.h, abi.ctypes_byref(t)) == 0
     g.record_niters(True)
     o.record_niters(True)
     p = cond_params()
@@ -216,11 +219,14 @@ def test_tiles_do_not_change_results(gpu):
     pop = P.coupled(SEED, ndims=(8, 6, 4), mean=64)
     s = sdm_params(cond=cond_params(), coll=coll_params(), motion=motion_params())
     outs, stats = [], []
-    # the small grids make every wave loop over several gridboxes (or drop chunks)
-    for wpb, flat, gw in ((1, 0, 0), (2, 0, 0), (4, 0, 0), (1, 1, 0), (1, 0, 64), (2, 1, 64), (4, 0, 8), (1, 1, 4),
-                          (4, 1, 0), (2, 1, 1)):
+    # the small grids make every wave loop over several gridboxes (or drop chunks); cond_flat 2 is
+    # the task-ring condensation (with 8 or 1 waves its rings drain many drops each)
+    rows = [(wpb, flat, gw, flat) for wpb, flat, gw in ((1, 0, 0), (2, 0, 0), (4, 0, 0), (1, 1, 0), (1, 0, 64),
+                                                           (2, 1, 64), (4, 0, 8), (1, 1, 4), (4, 1, 0), (2, 1, 1))]
+    rows += [(1, 2, 0, 1), (1, 2, 8, 0), (2, 2, 4, 1)]
+    for wpb, cflat, gw, flat in rows:
         d = make_domain(gpu, pop)
-        t = abi.Tiles(wpb, wpb, wpb, flat, flat, gw, gw, gw, flat)
+        t = abi.Tiles(wpb, wpb, wpb, cflat, flat, gw, gw, gw, flat)
         assert gpu.lib.cleo_set_tiles(d.h, abi.ctypes_byref(t)) == 0
         d.set_sdm(s)
         d.rng_counter(11)
