@@ -11,6 +11,33 @@
 
 namespace cleo {
 
+// ============================ per-gridbox sums ================================================
+// Drop-order sums of v(k) over [lo, lo + len) for the gridboxes of a tile of 64 (lane i passes its
+// own gridbox's range and gets its sum).  The tile's drops [t_lo, t_hi) are staged through an LDS
+// window of kSumWin values with coalesced loads; each lane then adds its gridbox's values left to
+// right, exactly as a serial loop does (the reference's sums run in drop order under Serial).
+template <class V>
+__device__ __forceinline__ double tile_sum(double* win, uint32_t t_lo, uint32_t t_hi, uint32_t lo, uint32_t len, V v,
+                                           int lane) {
+  double acc = 0.0;
+  for (uint32_t w = t_lo; w < t_hi; w += kSumWin) {
+    const uint32_t we = min(w + kSumWin, t_hi);
+    for (uint32_t j = lane; j < we - w; j += kWave) win[j] = v(w + j);
+    wave_sync();
+    const uint32_t a0 = max(lo, w), a1 = min(lo + len, we);
+    for (uint32_t k = a0; k < a1; ++k) acc += win[k - w];
+    wave_sync();
+  }
+  return acc;
+}
+// condensed water of drop k (condensate_mass * xi, superdrop_attrs.hpp:207-216)
+__device__ __forceinline__ double qcond_term(const SoA& s, const Solute& sol, DevCtl* ctl, uint32_t k) {
+  bool bad = false;
+  const double m = condensate_mass(s.r[k], s.msol[k], sol, &bad) * static_cast<double>(s.xi[k]);
+  if (bad) set_error(ctl, CLEO_ERR_NEG_CONDENSATE, s.gbx[k], k);
+  return m;
+}
+
 // ============================ condensation ====================================================
 // DoCondensation::do_condensation for one gridbox per wave (condensation.hpp:141-150,
 // condensation.cpp:44-137).  With `fuse_qcond`, the gridbox qcond of
@@ -112,28 +139,26 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_condensation(CondArgs
     for (int j = 0; j < 4; ++j) put_stat(a.stat, j, wave_index(), acc[j]);
 }
 
-// EffectOnHydrometeorStatesFunctor (movesupersindomain.hpp:95-136), standalone
-__device__ __forceinline__ void qcond_gbx(const QcondArgs& a, uint32_t g, int lane) {
-  const uint32_t base = a.off[g];
-  const uint32_t nfull = a.off[g + 1] - base;
-  bool bad = false;
-  double tot = 0.0;
-  for (uint32_t c = 0; c < nfull; c += kWave) {
-    const uint32_t k = base + c + lane;
-    double v = 0.0;
-    if (c + lane < nfull) v = condensate_mass(a.s.r[k], a.s.msol[k], a.sol, &bad) * static_cast<double>(a.s.xi[k]);
-    tot = ordered_add(tot, v, static_cast<int>(min(static_cast<uint32_t>(kWave), nfull - c)));
-  }
-  if (bad) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, g, base);
-  if (lane == 0) {
-    const double rho_cond = tot / a.st.vol[g] * dimless::R0cubed_VOL0;
-    const double rho_dry = dry_air_density(a.st.press[g], a.st.temp[g], a.st.qv[g]);
-    a.st.qc[g] = rho_cond / rho_dry;
-  }
-}
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(8) k_qcond(QcondArgs a) {
+// EffectOnHydrometeorStatesFunctor (movesupersindomain.hpp:95-136), standalone: 64 gridboxes per
+// wave, drop-order sums of condensed mass, then one lane per gridbox for its qcond
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_qcond(QcondArgs a) {
+  __shared__ double win_all[4][kSumWin];
+  double* win = win_all[threadIdx.x >> 6];
   const int lane = lane_id();
-  for (uint32_t g = wave_index(); g < a.ngbx; g += wave_count()) qcond_gbx(a, g, lane);
+  const uint32_t ntiles = (a.ngbx + kWave - 1) / kWave;
+  for (uint32_t tile = wave_index(); tile < ntiles; tile += wave_count()) {
+    const uint32_t g0 = tile * kWave, g = g0 + static_cast<uint32_t>(lane);
+    const bool valid = g < a.ngbx;
+    const uint32_t t_lo = a.off[g0], t_hi = a.off[min(g0 + kWave, a.ngbx)];
+    const uint32_t lo = valid ? a.off[g] : t_hi, hi = valid ? a.off[g + 1] : t_hi;
+    const double tot = tile_sum(win, t_lo, t_hi, lo, hi - lo, [&](uint32_t k) { return qcond_term(a.s, a.sol, a.ctl, k); },
+                                lane);
+    if (valid) {
+      const double rho_cond = tot / a.st.vol[g] * dimless::R0cubed_VOL0;
+      const double rho_dry = dry_air_density(a.st.press[g], a.st.temp[g], a.st.qv[g]);
+      a.st.qc[g] = rho_cond / rho_dry;
+    }
+  }
 }
 
 // ============================ collisions ======================================================
@@ -1076,7 +1101,7 @@ void launch_condensation(cleo_ctx* c, const CondArgs& args) {
 }
 void launch_qcond(cleo_ctx* c, const QcondArgs& a) {
   if (a.ngbx == 0) return;
-  hipLaunchKernelGGL(k_qcond, dim3(gbx_blocks(a.ngbx, a.wpb, c->tiles.cond_grid_waves)), dim3(64 * a.wpb), 0,
+  hipLaunchKernelGGL(k_qcond, dim3(gbx_blocks((a.ngbx + kWave - 1) / kWave, 4, c->tiles.cond_grid_waves)), dim3(256), 0,
                      c->stream, a);
 }
 void launch_collisions(cleo_ctx* c, const CollArgs& args) {
@@ -1241,35 +1266,32 @@ namespace cleo {
 
 // per gridbox: optional qcond recompute (ordered, before any radius changes), then the
 // saturation ratio and diffusion factor condensation uses (condensation.cpp:44-56)
-__device__ __forceinline__ void cond_prep_gbx(const CondFlatArgs& a, uint32_t g, int lane) {
-  const double temp = a.st.temp[g], press = a.st.press[g], qv = a.st.qv[g];
-  if (a.fuse_qcond) {
-    const uint32_t base = a.off[g];
-    const uint32_t nfull = a.off[g + 1] - base;
-    bool bad = false;
-    double tot = 0.0;
-    for (uint32_t c = 0; c < nfull; c += kWave) {
-      const uint32_t k = base + c + lane;
-      double v = 0.0;
-      if (c + lane < nfull) v = condensate_mass(a.s.r[k], a.s.msol[k], a.sol, &bad) * static_cast<double>(a.s.xi[k]);
-      tot = ordered_add(tot, v, static_cast<int>(min(static_cast<uint32_t>(kWave), nfull - c)));
-    }
-    if (bad) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, g, base);
-    if (lane == 0) {
-      const double rho_cond = tot / a.st.vol[g] * dimless::R0cubed_VOL0;
-      a.st.qc[g] = rho_cond / dry_air_density(press, temp, qv);
-    }
-  }
-  if (lane == 0) {
-    if (temp <= 0) set_error(a.ctl, CLEO_ERR_TEMP_NONPOSITIVE, g, a.off[g]);
-    const double psat = saturation_pressure(temp);
-    a.gs[2 * g] = supersaturation_ratio(press, qv, psat);
-    a.gs[2 * g + 1] = diffusion_factor(press, temp, psat);
-  }
-}
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(6) k_cond_prep(CondFlatArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_prep(CondFlatArgs a) {
+  __shared__ double win_all[4][kSumWin];
+  double* win = win_all[threadIdx.x >> 6];
   const int lane = lane_id();
-  for (uint32_t g = wave_index(); g < a.ngbx; g += wave_count()) cond_prep_gbx(a, g, lane);
+  const uint32_t ntiles = (a.ngbx + kWave - 1) / kWave;
+  for (uint32_t tile = wave_index(); tile < ntiles; tile += wave_count()) {
+    const uint32_t g0 = tile * kWave, g = g0 + static_cast<uint32_t>(lane);
+    const bool valid = g < a.ngbx;
+    double tot = 0.0;
+    if (a.fuse_qcond) {
+      const uint32_t t_lo = a.off[g0], t_hi = a.off[min(g0 + kWave, a.ngbx)];
+      const uint32_t lo = valid ? a.off[g] : t_hi, hi = valid ? a.off[g + 1] : t_hi;
+      tot = tile_sum(win, t_lo, t_hi, lo, hi - lo, [&](uint32_t k) { return qcond_term(a.s, a.sol, a.ctl, k); }, lane);
+    }
+    if (valid) {
+      const double temp = a.st.temp[g], press = a.st.press[g], qv = a.st.qv[g];
+      if (a.fuse_qcond) {
+        const double rho_cond = tot / a.st.vol[g] * dimless::R0cubed_VOL0;
+        a.st.qc[g] = rho_cond / dry_air_density(press, temp, qv);
+      }
+      if (temp <= 0) set_error(a.ctl, CLEO_ERR_TEMP_NONPOSITIVE, g, a.off[g]);
+      const double psat = saturation_pressure(temp);
+      a.gs[2 * g] = supersaturation_ratio(press, qv, psat);
+      a.gs[2 * g + 1] = diffusion_factor(press, temp, psat);
+    }
+  }
 }
 
 // per active drop: Koehler factors, Newton solve, radius update, mass change * xi into dm[k]
@@ -1545,29 +1567,29 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_cond_queue(CondFlatAr
 }
 
 // per gridbox: Σ dm in drop order and the thermodynamic feedback (condensation.cpp:126-137)
-__device__ __forceinline__ void cond_state_gbx(const CondFlatArgs& a, uint32_t g, int lane) {
-  const uint32_t base = a.off[g];
-  const uint32_t n = a.nact[g];
-  double tot = 0.0;
-  for (uint32_t c = 0; c < n; c += kWave) {
-    const double v = (c + lane < n) ? a.dm[base + c + lane] : 0.0;
-    tot = ordered_add(tot, v, static_cast<int>(min(static_cast<uint32_t>(kWave), n - c)));
-  }
-  if (lane == 0 && a.alter) {
-    const double temp = a.st.temp[g], press = a.st.press[g];
-    double qv = a.st.qv[g], qc = a.st.qc[g];
-    const double totrho = tot / a.st.vol[g] * dimless::R0cubed_VOL0;
-    const double dq = totrho / dry_air_density(press, temp, qv);
-    qv -= dq;
-    qc += dq;
-    a.st.temp[g] = temp + (dimless::Latent_v / moist_specific_heat(qv, qc)) * dq;
-    a.st.qv[g] = qv;
-    a.st.qc[g] = qc;
-  }
-}
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(8) k_cond_state(CondFlatArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_state(CondFlatArgs a) {
+  __shared__ double win_all[4][kSumWin];
+  double* win = win_all[threadIdx.x >> 6];
   const int lane = lane_id();
-  for (uint32_t g = wave_index(); g < a.ngbx; g += wave_count()) cond_state_gbx(a, g, lane);
+  const uint32_t ntiles = (a.ngbx + kWave - 1) / kWave;
+  for (uint32_t tile = wave_index(); tile < ntiles; tile += wave_count()) {
+    const uint32_t g0 = tile * kWave, g = g0 + static_cast<uint32_t>(lane);
+    const bool valid = g < a.ngbx;
+    const uint32_t t_lo = a.off[g0], t_hi = a.off[min(g0 + kWave, a.ngbx)];
+    const uint32_t lo = valid ? a.off[g] : t_hi, n = valid ? a.nact[g] : 0u;
+    const double tot = tile_sum(win, t_lo, t_hi, lo, n, [&](uint32_t k) { return a.dm[k]; }, lane);
+    if (valid && a.alter) {
+      const double temp = a.st.temp[g], press = a.st.press[g];
+      double qv = a.st.qv[g], qc = a.st.qc[g];
+      const double totrho = tot / a.st.vol[g] * dimless::R0cubed_VOL0;
+      const double dq = totrho / dry_air_density(press, temp, qv);
+      qv -= dq;
+      qc += dq;
+      a.st.temp[g] = temp + (dimless::Latent_v / moist_specific_heat(qv, qc)) * dq;
+      a.st.qv[g] = qv;
+      a.st.qc[g] = qc;
+    }
+  }
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
@@ -1664,7 +1686,8 @@ void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& args) {
   if (args.ngbx == 0) return;
   CondFlatArgs a = args;
   const uint32_t gw = c->tiles.cond_grid_waves;
-  hipLaunchKernelGGL(k_cond_prep, dim3(gbx_blocks(a.ngbx, 4, gw)), dim3(256), 0, c->stream, a);
+  const uint64_t ntiles = (a.ngbx + kWave - 1) / kWave;
+  hipLaunchKernelGGL(k_cond_prep, dim3(gbx_blocks(ntiles, 4, gw)), dim3(256), 0, c->stream, a);
   if (c->tiles.cond_flat == 2) {
     // task-ring kernel: waves of 4 per block; by default as many waves as fit on the device at once
     const uint64_t waves = gw ? gw : static_cast<uint64_t>(c->num_cu) * kCondQueueWavesPerCu;
@@ -1682,7 +1705,7 @@ void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& args) {
     kev_mark(c, 1);
     launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
   }
-  hipLaunchKernelGGL(k_cond_state, dim3(gbx_blocks(a.ngbx, 4, gw)), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(k_cond_state, dim3(gbx_blocks(ntiles, 4, gw)), dim3(256), 0, c->stream, a);
 }
 
 void launch_motion_flat(cleo_ctx* c, const MotionArgs& args) {

@@ -15,6 +15,8 @@ constexpr uint32_t kCollTileLdsBytes = kTilePermBytes + 4u * (2u * 65u + 4u * 64
 constexpr uint32_t kRing = 128;
 constexpr uint32_t kCondRingBytes = kRing * (11u * 8u + 8u + 2u * 4u);
 constexpr uint32_t kCondQueueWavesPerCu = 12;
+// LDS window (values) of the per-gridbox drop-order sums
+constexpr uint32_t kSumWin = 1024;
 
 struct CondArgs {
   SoA s;
