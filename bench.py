@@ -40,6 +40,9 @@ BYTES = {
     # read coords, radius, sdgbxindex; write coords, sdgbxindex | bounds, winds, counts
     "k_motion": (64, 6 * 8 + 6 * 8 + 28 * 4),
     "k_motion_flat": (64, 6 * 8 + 6 * 8 + 28 * 4),
+    # fused motion + sort, moving pass: read coords, radius, xi, msol, id; write the 60-byte record
+    # at its sorted place | bounds, winds, refs, run starts
+    "k_motion_scatter": (112, 6 * 8 + 6 * 8 + 8 + 4 * 8),
 }
 
 
@@ -64,7 +67,7 @@ def stage_kernel(stage, tiles):
         return {0: "k_condensation", 1: "k_cond_drops", 2: "k_cond_queue"}[tiles.cond_flat]
     if stage == 1:
         return "k_coll_tiles" if tiles.coll_flat else "k_collisions"
-    return "k_motion_flat" if tiles.motion_flat else "k_motion"
+    return {0: "k_motion", 1: "k_motion_flat", 2: "k_motion_scatter"}[tiles.motion_flat]
 
 
 def parse():
@@ -81,7 +84,7 @@ def parse():
     ap.add_argument("--tiles", type=str, default=None,
                     help="launch tiling as 8 comma-separated cleo_tiles fields (skips the autotuner)")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="copy one 1 GiB device buffer first (reference point for byte counters)")
+                    help="negate one 1 GiB device buffer first (reference point for byte counters)")
     return ap.parse_args()
 
 
@@ -109,8 +112,10 @@ def run_gpu(args):
     L = b.lib
     dev = torch.device("cuda:0")
     if args.copy_probe:
+        # reference point for byte counters: one elementwise kernel that reads 1 GiB and writes 1 GiB
         src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
-        dst = src.clone()
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)
         torch.cuda.synchronize()
         del src, dst
     ndims = (args.nz, 128, 128)
@@ -168,15 +173,16 @@ def run_gpu(args):
 
     d = new_domain()
     tiles = abi.Tiles()
+    t = 0
+    if args.warmup:
+        d.check(L.cleo_run_steps_graph(d.h, t, span, args.warmup), "warmup")
+        t += args.warmup * span
+    # tilings do not change results, so the choice can be made on the warmed-up population
     if args.tiles:
         tiles = abi.Tiles(*[int(v) for v in args.tiles.split(",")])
         d.check(L.cleo_set_tiles(d.h, C.byref(tiles)), "set_tiles")
     elif not args.no_autotune:
         d.check(L.cleo_autotune(d.h, C.byref(tiles)), "autotune")
-    t = 0
-    if args.warmup:
-        d.check(L.cleo_run_steps_graph(d.h, t, span, args.warmup), "warmup")
-        t += args.warmup * span
     dl = None
     if not args.no_mlp:
         # identical twin for the learned leg: same upload, same warm-up, same random streams

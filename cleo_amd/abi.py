@@ -89,7 +89,7 @@ class Tiles(C.Structure):
     _fields_ = [("cond_waves_per_block", C.c_uint32), ("coll_waves_per_block", C.c_uint32),
                 ("motion_waves_per_block", C.c_uint32), ("cond_flat", C.c_uint32), ("motion_flat", C.c_uint32),
                 ("cond_grid_waves", C.c_uint32), ("coll_grid_waves", C.c_uint32), ("motion_grid_waves", C.c_uint32),
-                ("coll_flat", C.c_uint32)]
+                ("coll_flat", C.c_uint32), ("cond_defer", C.c_uint32)]
 
 
 class MLPTrainParams(C.Structure):

@@ -109,7 +109,7 @@ CondArgs cond_args(cleo_ctx* c, const cleo_cond_params& p, bool fuse_qcond) {
   a.nact = c->nact;
   a.st = c->st;
   a.sol = c->solute;
-  a.cfg = NewtonCfg{step2dimlesstime(p.interval), p.minsubdelt_s / dimless::TIME0, p.rtol, p.atol, p.maxniters};
+  a.cfg = NewtonCfg{step2dimlesstime(p.interval), p.minsubdelt_s / dimless::TIME0, p.rtol, p.atol, p.maxniters, 0u};
   a.ngbx = c->grid.ngbx;
   a.wpb = wpb_or1(c->tiles.cond_waves_per_block);
   a.alter = p.do_alter_thermo;
@@ -205,11 +205,15 @@ int enqueue_motion(cleo_ctx* c, const cleo_motion_params& p, bool qcond) {
   a.wpb = wpb_or1(c->tiles.motion_waves_per_block);
   a.cnt = c->cnt;
   a.ctl = c->ctl;
-  if (c->tiles.motion_flat)
-    launch_motion_flat(c, a);
-  else
-    launch_motion(c, a);
-  neighbour_sort(c, c->cur, 1 - c->cur, nullptr, false);
+  if (c->tiles.motion_flat == 2) {
+    motion_sort_fused(c, a, c->buf[1 - c->cur], 1 - c->cur);
+  } else {
+    if (c->tiles.motion_flat)
+      launch_motion_flat(c, a);
+    else
+      launch_motion(c, a);
+    neighbour_sort(c, c->cur, 1 - c->cur, nullptr, false);
+  }
   if (qcond) {
     QcondArgs q{c->buf[c->cur], c->off, c->st, c->solute, c->grid.ngbx, wpb_or1(c->tiles.cond_waves_per_block), c->ctl};
     launch_qcond(c, q);
@@ -326,6 +330,9 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   // a gridbox launch has at most ngbx + 3 waves, a drop launch 4 per 256 drops
   c->sp_slots = std::max<uint64_t>(ngbx + 4, 4 * ((maxn + 255) / 256) + 4);
   HIP_TRY(c, dalloc(&c->sp, kStatSlotsMax * c->sp_slots));
+  // deferred list: waves of a drop launch times (a quarter of their drops + 16)
+  HIP_TRY(c, dalloc(&c->dcount, c->sp_slots));
+  HIP_TRY(c, dalloc(&c->dlist, maxn / 4 + 20 * c->sp_slots + 64));
   c->big_scratch_n = 2 * maxn;
   HIP_TRY(c, dalloc(&c->ctl, 1));
   HIP_TRY(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_ctl), sizeof(DevCtl), hipHostMallocDefault));
@@ -354,7 +361,7 @@ int cleo_destroy(cleo_ctx* c) {
                   static_cast<void*>(c->niters), static_cast<void*>(c->guess), static_cast<void*>(c->rs_key[0]),
                   static_cast<void*>(c->rs_key[1]), static_cast<void*>(c->rs_idx[0]), static_cast<void*>(c->rs_idx[1]),
                   static_cast<void*>(c->rs_cnt), static_cast<void*>(c->gs), static_cast<void*>(c->dm),
-                  static_cast<void*>(c->sp)})
+                  static_cast<void*>(c->sp), static_cast<void*>(c->dlist), static_cast<void*>(c->dcount)})
     if (p) hipFree(p);
   if (c->h_ctl) hipHostFree(c->h_ctl);
   for (hipEvent_t& e : c->ev)
@@ -784,8 +791,9 @@ int cleo_set_tiles(cleo_ctx* c, const cleo_tiles* t) {
   if (!c || !t) return CLEO_E_ARG;
   for (uint32_t v : {t->cond_waves_per_block, t->coll_waves_per_block, t->motion_waves_per_block})
     if (v > 4) return fail(c, CLEO_E_ARG, "waves per block must be 0..4");
-  if (t->cond_flat > 2 || t->motion_flat > 1 || t->coll_flat > 1)
-    return fail(c, CLEO_E_ARG, "variant flags: cond_flat 0..2, motion_flat and coll_flat 0 or 1");
+  if (t->cond_flat > 2 || t->motion_flat > 2 || t->coll_flat > 1)
+    return fail(c, CLEO_E_ARG, "variant flags: cond_flat and motion_flat 0..2, coll_flat 0 or 1");
+  if (t->cond_defer == 1) return fail(c, CLEO_E_ARG, "cond_defer must be 0 or at least 2");
   c->tiles = *t;
   return CLEO_OK;
 }
@@ -819,6 +827,13 @@ static int time_stage(cleo_ctx* c, int kern, const cleo_tiles& t, int reps, bool
   c->tiles = t;
   int rc = CLEO_OK;
   float best = 1e30f, total = 0.f;
+  // motion is timed with its sort, into a scratch store (the current one stays untouched)
+  SoA sorted{};
+  if (kern == 2 && alloc_soa(sorted, c->ntot) != hipSuccess) {
+    free_soa(sorted);
+    c->tiles = tiles0;
+    return fail(c, CLEO_E_HIP, "stage timing buffer");
+  }
   for (int rep = 0; rep < reps && rc == CLEO_OK; ++rep) {
     const SoA& a0 = c->buf[cur];
     const SoA& b0 = c->buf[spare];
@@ -875,10 +890,18 @@ static int time_stage(cleo_ctx* c, int kern, const cleo_tiles& t, int reps, bool
       a.wpb = t.motion_waves_per_block ? t.motion_waves_per_block : 1;
       a.cnt = c->cnt;
       a.ctl = c->ctl;
-      if (t.motion_flat)
-        launch_motion_flat(c, a);
-      else
-        launch_motion(c, a);
+      if (t.motion_flat == 2) {
+        motion_sort_fused(c, a, sorted, -1);
+      } else {
+        if (t.motion_flat)
+          launch_motion_flat(c, a);
+        else
+          launch_motion(c, a);
+        const bool on = c->kev_on;
+        c->kev_on = false;  // the main kernel of the unfused stage is the motion kernel
+        neighbour_sort_to(c, c->buf[spare], sorted, -1);
+        c->kev_on = on;
+      }
     }
     hipEventRecord(c->ev[15], c->stream);
     c->kev_on = false;
@@ -896,6 +919,7 @@ static int time_stage(cleo_ctx* c, int kern, const cleo_tiles& t, int reps, bool
   }
   c->tiles = tiles0;
   hipStreamSynchronize(c->stream);
+  free_soa(sorted);
   hipFree(saved);
   hipFree(saved_nact);
   HIP_TRY(c, ctx_copy(c, c->ctl, &saved_ctl, sizeof(DevCtl), hipMemcpyHostToDevice));
@@ -916,7 +940,7 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
     uint32_t wpb, flat;
   };
   constexpr int kNC = 6;
-  const Cand cands[kNC] = {{1, 0}, {2, 0}, {4, 0}, {1, 1}, {4, 1}, {1, 2}};
+  const Cand cands[kNC] = {{1, 0}, {2, 0}, {4, 0}, {1, 1}, {4, 1}, {1, 2}};  // {waves per block, variant}
   const uint32_t grid_waves[3] = {0, 8192, 32768};
   cleo_tiles best_t = c->tiles;
   for (int kern = 0; kern < 3; ++kern) {
@@ -925,7 +949,7 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
       const Cand& cd = cands[gi % kNC];
       const uint32_t gw = grid_waves[gi / kNC];
       if (kern != 1 && cd.flat && cd.wpb != 1) continue;  // thread-per-drop kernels ignore wpb
-      if (kern != 0 && cd.flat == 2) continue;              // the task ring is condensation's
+      if (kern == 1 && cd.flat == 2) continue;              // collisions have no third variant
       cleo_tiles t = best_t;
       if (kern == 0) {
         t.cond_waves_per_block = cd.wpb;
@@ -942,6 +966,22 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
       }
       float tmin = 0.f;
       const int rc = time_stage(c, kern, t, 2, false, &tmin, nullptr);
+      if (rc != CLEO_OK) return rc;
+      if (tmin < best) {
+        best = tmin;
+        best_t = t;
+      }
+    }
+  }
+  // thread-per-drop condensation: the iteration budget after which slow drops are set aside
+  if (best_t.cond_flat == 1) {
+    float best = 1e30f;
+    const cleo_tiles base_t = best_t;
+    for (uint32_t budget : {0u, 6u, 8u, 12u}) {
+      cleo_tiles t = base_t;
+      t.cond_defer = budget;
+      float tmin = 0.f;
+      const int rc = time_stage(c, 0, t, 2, false, &tmin, nullptr);
       if (rc != CLEO_OK) return rc;
       if (tmin < best) {
         best = tmin;

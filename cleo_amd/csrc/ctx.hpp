@@ -98,11 +98,14 @@ struct cleo_ctx {
   // configuration
   cleo_sdm_params sdm{};
   bool sdm_set = false;
-  cleo_tiles tiles{1, 1, 1, 0, 0, 0, 0, 0, 0};
+  cleo_tiles tiles{1, 1, 1, 0, 0, 0, 0, 0, 0, 0};
   double* gs = nullptr;  // 2*ngbx: per-gridbox saturation ratio and diffusion factor
   double* dm = nullptr;  // maxn: per-drop condensed mass (flat condensation)
   // per-wave statistics: up to kStatSlotsMax counters per wave, reduced into DevCtl::stats
   uint32_t* sp = nullptr;
+  // deferred-drop list of the thread-per-drop condensation (segments per wave) and their counts
+  uint32_t* dlist = nullptr;
+  uint32_t* dcount = nullptr;
   uint64_t sp_slots = 0;  // slots per counter: max(ngbx, waves of a thread-per-drop launch)
   // per-drop Newton iteration record
   bool rec_niters = false;

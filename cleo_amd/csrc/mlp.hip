@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(256) k_mlp_samples(SampleArgs a) {
     const double ffactor = diffusion_factor(press, temp, psat);
     double akoh, bkoh;
     kohler_factors(msol, temp, a.sol, akoh, bkoh);
-    SolveOut o{0, false, false, false};
+    SolveOut o{0, false, false, false, false};
     const double newr = solve_condensation(a.cfg, s_ratio, akoh, bkoh, ffactor, r,
                                            __longlong_as_double(0x7FF8000000000000ll), 0u, o);
     if (!o.substep && !o.maxed && newr > 0.0) {
@@ -614,7 +614,7 @@ int cleo_mlp_make_samples(cleo_ctx* c, const cleo_cond_params* p, float* x_out, 
   a.st = c->st;
   a.sol = c->solute;
   a.cfg = NewtonCfg{static_cast<double>(p->interval) / 100000.0, p->minsubdelt_s / dimless::TIME0, p->rtol,
-                    p->atol, p->maxniters};
+                    p->atol, p->maxniters, 0u};
   a.ngbx = c->grid.ngbx;
   a.x = x;
   a.y = y;

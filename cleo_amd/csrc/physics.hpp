@@ -181,6 +181,7 @@ struct ODEConsts {
 struct NewtonCfg {
   double delt, minsubdelt, rtol, atol;
   uint32_t maxniters;
+  uint32_t budget;  // iterations after which a solve gives up as deferred (0: never)
 };
 // g(z) (impliciteuler.cpp:270-281) and dg/dz (impliciteuler.cpp:291-301) at the same z.  The
 // reference's quotients akoh/r, bkoh/r^3, subdelt/(z ffactor) and rprev/r are products with one
@@ -222,12 +223,19 @@ __device__ __forceinline__ bool newton_iterate(const NewtonCfg& cfg, const ODECo
   return current < converged;
 }
 // newtonraphson_untilconverged (impliciteuler.cpp:207-227): false when `limit` is exceeded
+// With a budget, the loop also stops once `iters` reaches it and flags the solve as deferred (the
+// caller abandons it; the drop is solved again without a budget).
 __device__ __forceinline__ bool newton_until(const NewtonCfg& cfg, const ODEConsts& c, uint32_t limit,
-                                             double subdelt, double rprev, NewtonState& s, uint32_t& iters) {
+                                             double subdelt, double rprev, NewtonState& s, uint32_t& iters,
+                                             bool& deferred) {
   uint32_t niter = 1;
   bool conv = false;
   while (!conv) {
     if (niter > limit) return false;
+    if (cfg.budget && iters >= cfg.budget) {
+      deferred = true;
+      return false;
+    }
     conv = newton_iterate(cfg, c, subdelt, rprev, s, iters);
     niter += 1;
   }
@@ -236,7 +244,7 @@ __device__ __forceinline__ bool newton_until(const NewtonCfg& cfg, const ODECons
 // integrate_condensation_ode (impliciteuler.cpp:128-141): 2 iterations, then restart from the guess
 __device__ __forceinline__ double newton_integrate(const NewtonCfg& cfg, const ODEConsts& c, double subdelt,
                                                    double rprev, double zguess, uint32_t& iters,
-                                                   bool& maxed) {
+                                                   bool& maxed, bool& deferred) {
   NewtonState s0;
   s0.z = zguess;
   ode_g_dg(c, subdelt, rprev, zguess, s0.g, s0.dg);
@@ -245,7 +253,7 @@ __device__ __forceinline__ double newton_integrate(const NewtonCfg& cfg, const O
   conv = newton_iterate(cfg, c, subdelt, rprev, s, iters);
   if (conv) return s.z;
   s = s0;
-  if (!newton_until(cfg, c, cfg.maxniters, subdelt, rprev, s, iters)) maxed = true;
+  if (!newton_until(cfg, c, cfg.maxniters, subdelt, rprev, s, iters, deferred) && !deferred) maxed = true;
   return s.z;
 }
 __device__ __forceinline__ double initialguess(const ODEConsts& c, double rprev) {
@@ -274,6 +282,7 @@ __device__ __forceinline__ double critical_timestep(const ODEConsts& c) {
 struct SolveOut {
   uint32_t iters;
   bool substep, restarted, maxed;
+  bool deferred;  // the budget ran out: nothing of this solve is kept
 };
 // solve_condensation (impliciteuler.cpp:35-54). `warm` is a learned guess for z = r^2 or NaN.
 // Learned path: the unchanged two-phase Newton from `warm` with `warm_limit` as until-converged
@@ -299,15 +308,16 @@ __device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, doubl
         rsqrd = s.z;
       } else {
         s = s0;
-        if (newton_until(cfg, c, warm_limit, cfg.delt, rprev, s, o.iters)) {
+        if (newton_until(cfg, c, warm_limit, cfg.delt, rprev, s, o.iters, o.deferred)) {
           rsqrd = s.z;
         } else {
+          if (o.deferred) return 0.0;
           o.restarted = true;
-          rsqrd = newton_integrate(cfg, c, cfg.delt, rprev, ziter, o.iters, o.maxed);
+          rsqrd = newton_integrate(cfg, c, cfg.delt, rprev, ziter, o.iters, o.maxed, o.deferred);
         }
       }
     } else {
-      rsqrd = newton_integrate(cfg, c, cfg.delt, rprev, ziter, o.iters, o.maxed);
+      rsqrd = newton_integrate(cfg, c, cfg.delt, rprev, ziter, o.iters, o.maxed, o.deferred);
     }
   } else {
     o.substep = true;
@@ -317,7 +327,8 @@ __device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, doubl
     double rp = rprev;
     while (remdelt > 0.0) {
       const double subdelt = fmin(mindelt, remdelt);
-      ziter = newton_integrate(cfg, c, subdelt, rp, ziter, o.iters, o.maxed);
+      ziter = newton_integrate(cfg, c, subdelt, rp, ziter, o.iters, o.maxed, o.deferred);
+      if (o.deferred) return 0.0;
       rp = sqrt(ziter);  // pow(ziter, 0.5)
       remdelt -= subdelt;
     }

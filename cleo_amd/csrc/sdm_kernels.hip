@@ -88,7 +88,7 @@ __device__ __forceinline__ void cond_gbx(const CondArgs& a, uint32_t g, int lane
       double akoh, bkoh;
       kohler_factors(msol, temp, a.sol, akoh, bkoh);
       const double warm = a.warm ? a.warm[k] : __longlong_as_double(0x7FF8000000000000ll);
-      SolveOut o{0, false, false, false};
+      SolveOut o{0, false, false, false, false};
       const double newr = solve_condensation(a.cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
       const double rnew = fmax(newr, dryradius(msol, a.sol));  // change_radius (superdrop_attrs.cpp:35-45)
       a.s.r[k] = rnew;
@@ -187,56 +187,65 @@ __device__ __forceinline__ uint32_t fy_index(const CollArgs& a, uint32_t g, uint
 }
 
 // One collision pair (2jj, 2jj+1) of gridbox g (n active drops) from the drops at ka, kb of `src`
-// (collisions.hpp:180-230): probability, enactment, and both drops written to their shuffled
-// positions base+2jj, base+2jj+1 of `dst` (updated attributes, other fields carried).  Returns
-// true when one of the two became null; `events` counts enactments.
-__device__ __forceinline__ bool coll_pair(const CollArgs& a, uint32_t g, uint32_t base, uint32_t n, uint32_t jj,
-                                          uint32_t ka, uint32_t kb, uint64_t event, uint32_t& events) {
+// (collisions.hpp:180-230): probability and enactment on (xi, r, msol) of both drops.
+struct PairOut {
+  DropAttrs A, B;  // the drops at ka and kb after the enactment
+  bool nullA, nullB;
+  bool enacted;
+  int err;
+};
+__device__ __forceinline__ PairOut coll_pair_eval(const CollArgs& a, uint32_t g, uint32_t n, uint32_t jj, uint32_t ka,
+                                                  uint32_t kb, uint64_t event) {
   const uint32_t npairs = n / 2;
   const double scale_p = n * (n - 1.0) / (2.0 * npairs);
   const double VOLUME = a.vol[g] * dimless::VOL0;
-  DropAttrs A{a.src.xi[ka], a.src.r[ka], a.src.msol[ka]};
-  DropAttrs B{a.src.xi[kb], a.src.r[kb], a.src.msol[kb]};
-  const bool a_first = !(A.xi < B.xi);  // assign_drops (collisions.hpp:94-101)
-  DropAttrs& d1 = a_first ? A : B;
-  DropAttrs& d2 = a_first ? B : A;
+  PairOut o;
+  o.A = DropAttrs{a.src.xi[ka], a.src.r[ka], a.src.msol[ka]};
+  o.B = DropAttrs{a.src.xi[kb], a.src.r[kb], a.src.msol[kb]};
+  const bool a_first = !(o.A.xi < o.B.xi);  // assign_drops (collisions.hpp:94-101)
+  DropAttrs& d1 = a_first ? o.A : o.B;
+  DropAttrs& d2 = a_first ? o.B : o.A;
   const double prob_jk = pair_probability(a.k, d1.r, d2.r, VOLUME);
   const double prob = scale_p * static_cast<double>(d1.xi) * prob_jk;
   double phi_coll, phi_out;
   if (a.inj_phi) {
-    const uint64_t o = a.inj_op[g] + 2ull * jj;
-    phi_coll = a.inj_phi[o];
-    phi_out = a.inj_phi[o + 1];
+    const uint64_t q = a.inj_op[g] + 2ull * jj;
+    phi_coll = a.inj_phi[q];
+    phi_out = a.inj_phi[q + 1];
   } else {
     uint32_t r4[4];
     rng_block(a.seed, event, 1u, g, jj, r4);
     phi_coll = u53(r4[0], r4[1]);
     phi_out = u53(r4[2], r4[3]);
   }
-  int err = 0;
-  const uint32_t res = enact_pair(a.k, d1, d2, prob, phi_coll, phi_out, a.sol, err);
-  if (err) set_error(a.ctl, err, g, base + 2 * jj);
-  events += (res >> 1) & 1u;
-  const bool nullA = (res & 1u) && a_first;
-  const bool nullB = (res & 1u) && !a_first;
-  const uint32_t oa = base + 2 * jj, ob = oa + 1;
-  a.dst.gbx[oa] = nullA ? kOOB : a.src.gbx[ka];
-  a.dst.c3[oa] = a.src.c3[ka];
-  a.dst.c1[oa] = a.src.c1[ka];
-  a.dst.c2[oa] = a.src.c2[ka];
-  a.dst.xi[oa] = A.xi;
-  a.dst.r[oa] = A.r;
-  a.dst.msol[oa] = A.msol;
-  a.dst.id[oa] = a.src.id[ka];
-  a.dst.gbx[ob] = nullB ? kOOB : a.src.gbx[kb];
-  a.dst.c3[ob] = a.src.c3[kb];
-  a.dst.c1[ob] = a.src.c1[kb];
-  a.dst.c2[ob] = a.src.c2[kb];
-  a.dst.xi[ob] = B.xi;
-  a.dst.r[ob] = B.r;
-  a.dst.msol[ob] = B.msol;
-  a.dst.id[ob] = a.src.id[kb];
-  return res & 1u;
+  o.err = 0;
+  const uint32_t res = enact_pair(a.k, d1, d2, prob, phi_coll, phi_out, a.sol, o.err);
+  o.enacted = (res >> 1) & 1u;
+  o.nullA = (res & 1u) && a_first;
+  o.nullB = (res & 1u) && !a_first;
+  return o;
+}
+// drop of `src` at ks written to `dst` at kd with attributes D (other fields carried)
+__device__ __forceinline__ void coll_write(const CollArgs& a, uint32_t kd, uint32_t ks, const DropAttrs& D, bool isnull) {
+  a.dst.gbx[kd] = isnull ? kOOB : a.src.gbx[ks];
+  a.dst.c3[kd] = a.src.c3[ks];
+  a.dst.c1[kd] = a.src.c1[ks];
+  a.dst.c2[kd] = a.src.c2[ks];
+  a.dst.xi[kd] = D.xi;
+  a.dst.r[kd] = D.r;
+  a.dst.msol[kd] = D.msol;
+  a.dst.id[kd] = a.src.id[ks];
+}
+// the pair written to its shuffled positions base+2jj, base+2jj+1 of `dst`; true when one of the
+// two became null; `events` counts enactments
+__device__ __forceinline__ bool coll_pair(const CollArgs& a, uint32_t g, uint32_t base, uint32_t n, uint32_t jj,
+                                          uint32_t ka, uint32_t kb, uint64_t event, uint32_t& events) {
+  const PairOut o = coll_pair_eval(a, g, n, jj, ka, kb, event);
+  if (o.err) set_error(a.ctl, o.err, g, base + 2 * jj);
+  events += o.enacted ? 1u : 0u;
+  coll_write(a, base + 2 * jj, ka, o.A, o.nullA);
+  coll_write(a, base + 2 * jj + 1, kb, o.B, o.nullB);
+  return o.nullA || o.nullB;
 }
 // drop k of a gridbox that takes no part in a pair (the unpaired last drop and the inactive
 // tail) is carried unchanged from src[ks] to dst[kd]
@@ -400,10 +409,8 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_collisions(CollArgs a
 // per-gridbox path through HBM scratch.
 struct CollTileLds {
   uint32_t perm32[kTilePermBytes / 4];  // byte permutations, gridbox i from byte loff[i]
-  uint32_t pre[kWave + 1];              // pairs of the gridboxes before i
-  uint32_t tpre[kWave + 1];             // carried drops of the gridboxes before i
-  uint32_t base[kWave];
-  uint32_t loff[kWave];
+  uint32_t pre[kWave + 1];              // first drop of gridbox i relative to the tile's first
+  uint32_t loff[kWave];                 // kBigGbx: the gridbox takes the per-gridbox path
   uint32_t n[kWave];
   uint32_t nulls[kWave];
 };
@@ -468,35 +475,38 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_coll_tiles(CollArgs a
       }
     }
     const uint32_t np = small ? n / 2 : 0u;
-    const uint32_t nc = small ? nfull - 2u * np : 0u;
-    uint32_t tot_pairs, tot_carry;
-    const uint32_t pre = wave_excl_scan(np, lane, tot_pairs);
-    const uint32_t tpre = wave_excl_scan(nc, lane, tot_carry);
-    L.pre[lane] = pre;
-    L.tpre[lane] = tpre;
-    if (lane == kWave - 1) {
-      L.pre[kWave] = pre + np;
-      L.tpre[kWave] = tpre + nc;
-    }
-    L.base[lane] = base;
-    L.loff[lane] = loff;
+    // the tile's drops [t_lo, t_hi) in destination order: lane L writes position t_lo + f, f = L,
+    // L + 64, ..., so every store of the wave is one contiguous run; both lanes of a pair evaluate
+    // it and each writes its own drop
+    const uint32_t t_lo = readlane_u32(base, 0);
+    const uint32_t t_hi = a.off[min(tile * kWave + kWave, a.ngbx)];
+    L.pre[lane] = valid ? base - t_lo : t_hi - t_lo;
+    if (lane == kWave - 1) L.pre[kWave] = t_hi - t_lo;
+    L.loff[lane] = small ? loff : kBigGbx;
     L.n[lane] = n;
     L.nulls[lane] = 0u;
     wave_sync();
-    for (uint32_t f = lane; f < tot_pairs; f += kWave) {
+    for (uint32_t f = lane; f < t_hi - t_lo; f += kWave) {
       const uint32_t i = tile_owner(L.pre, f);
-      const uint32_t jj = f - L.pre[i];
-      const uint32_t bi = L.base[i];
-      const uint8_t* pi = pb + L.loff[i];
-      if (coll_pair(a, tile * kWave + i, bi, L.n[i], jj, bi + pi[2 * jj], bi + pi[2 * jj + 1], event, events))
-        atomicAdd(&L.nulls[i], 1u);
-    }
-    for (uint32_t f = lane; f < tot_carry; f += kWave) {
-      const uint32_t i = tile_owner(L.tpre, f);
-      const uint32_t ni = L.n[i];
-      const uint32_t k = 2u * (ni / 2u) + (f - L.tpre[i]);
-      const uint32_t bi = L.base[i];
-      coll_carry(a, bi + (k < ni ? static_cast<uint32_t>(pb[L.loff[i] + k]) : k), bi + k);
+      const uint32_t li = L.loff[i];
+      if (li == kBigGbx) continue;
+      const uint32_t slot = f - L.pre[i], ni = L.n[i], bi = t_lo + L.pre[i];
+      const uint8_t* pi = pb + li;
+      if (slot < 2u * (ni / 2u)) {
+        const uint32_t jj = slot >> 1, side = slot & 1u;
+        const uint32_t gi = tile * kWave + i, ka = bi + pi[2 * jj], kb = bi + pi[2 * jj + 1];
+        const PairOut o = coll_pair_eval(a, gi, ni, jj, ka, kb, event);
+        if (side == 0u) {
+          if (o.err) set_error(a.ctl, o.err, gi, bi + 2 * jj);
+          events += o.enacted ? 1u : 0u;
+          if (o.nullA || o.nullB) atomicAdd(&L.nulls[i], 1u);
+          coll_write(a, bi + slot, ka, o.A, o.nullA);
+        } else {
+          coll_write(a, bi + slot, kb, o.B, o.nullB);
+        }
+      } else {
+        coll_carry(a, bi + (slot < ni ? static_cast<uint32_t>(pi[slot]) : slot), bi + slot);
+      }
     }
     wave_sync();
     const uint32_t mynulls = L.nulls[lane];
@@ -625,74 +635,107 @@ __device__ __forceinline__ int axis_step(uint32_t& idx, double& coord, uint32_t 
   return 0;
 }
 
+// bounds and face winds of gridbox g
+struct GbxGeom {
+  uint32_t iz, ix, iy;
+  double zlo, zhi, xlo, xhi, ylo, yhi, wlo, whi, ulo, uhi, vlo, vhi;
+};
+__device__ __forceinline__ GbxGeom gbx_geom(const MotionArgs& a, uint32_t g) {
+  const uint32_t nz = a.grid.nz, nx = a.grid.nx;
+  GbxGeom G;
+  G.iz = g % nz;
+  G.ix = (g / nz) % nx;
+  G.iy = g / (nz * nx);
+  G.zlo = a.grid.zh[G.iz];
+  G.zhi = a.grid.zh[G.iz + 1];
+  G.xlo = a.grid.xh[G.ix];
+  G.xhi = a.grid.xh[G.ix + 1];
+  G.ylo = a.grid.yh[G.iy];
+  G.yhi = a.grid.yh[G.iy + 1];
+  G.wlo = a.st.wlo[g];
+  G.whi = a.st.whi[g];
+  G.ulo = a.st.ulo[g];
+  G.uhi = a.st.uhi[g];
+  G.vlo = a.st.vlo[g];
+  G.vhi = a.st.vhi[g];
+  return G;
+}
+// drop k of gridbox g after one predictor-corrector step: new coordinates, new gridbox (kOOB
+// when it left the domain) and the step taken, dir = (dz+1) + 3(dx+1) + 9(dy+1), or 27 out of the
+// domain.  PredCorr::delta_coord3/1/2 (predcorr.hpp:84-145), cfl_criteria (cfl_criteria.hpp:40-57),
+// superdrop_gbx z, then x, then y (predcorrmotion.hpp:85-106) and CartesianCheckBounds
+// (cartesian_motion.hpp:38-52); `report` records the CFL and bounds errors.
+struct Moved {
+  double c3, c1, c2;
+  uint32_t idx;
+  int dir;
+};
+__device__ __forceinline__ Moved move_drop(const MotionArgs& a, uint32_t g, const GbxGeom& G, uint64_t k, bool report) {
+  const uint32_t nz = a.grid.nz, nx = a.grid.nx, ny = a.grid.ny;
+  const double delt = a.delt;
+  const double c3_0 = a.s.c3[k], c1_0 = a.s.c1[k], c2_0 = a.s.c2[k];
+  double coord3 = c3_0;
+  const double terminal = terminal_velocity(a.tv, a.s.r[k]);
+  double vel3 = interpolation(G.zlo, G.zhi, G.wlo, G.whi, coord3);
+  vel3 -= terminal;
+  coord3 += vel3 * delt;
+  double corrvel3 = interpolation(G.zlo, G.zhi, G.wlo, G.whi, coord3);
+  corrvel3 -= terminal;
+  const double delta3 = (vel3 + corrvel3) * (delt / 2);
+  double coord1 = c1_0;
+  const double vel1 = interpolation(G.xlo, G.xhi, G.ulo, G.uhi, coord1);
+  coord1 += vel1 * delt;
+  const double corrvel1 = interpolation(G.xlo, G.xhi, G.ulo, G.uhi, coord1);
+  const double delta1 = (vel1 + corrvel1) * (delt / 2);
+  double coord2 = c2_0;
+  const double vel2 = interpolation(G.ylo, G.yhi, G.vlo, G.vhi, coord2);
+  coord2 += vel2 * delt;
+  const double corrvel2 = interpolation(G.ylo, G.yhi, G.vlo, G.vhi, coord2);
+  const double delta2 = (vel2 + corrvel2) * (delt / 2);
+  bool cfl = (fabs(delta3) <= fabs(G.zhi - G.zlo));
+  cfl = (fabs(delta1) <= fabs(G.xhi - G.xlo)) && cfl;
+  cfl = (fabs(delta2) <= fabs(G.yhi - G.ylo)) && cfl;
+  if (report && !cfl) set_error(a.ctl, CLEO_ERR_CFL, g, k);
+  Moved M;
+  M.c3 = c3_0 + delta3;
+  M.c1 = c1_0 + delta1;
+  M.c2 = c2_0 + delta2;
+  uint32_t idx = g;
+  const int dz = axis_step(idx, M.c3, G.iz, nz, 1u, a.grid.per[0] != 0, a.grid.zh);
+  int dx = 0, dy = 0;
+  if (idx != kOOB) dx = axis_step(idx, M.c1, G.ix, nx, nz, a.grid.per[1] != 0, a.grid.xh);
+  if (idx != kOOB) dy = axis_step(idx, M.c2, G.iy, ny, nz * nx, a.grid.per[2] != 0, a.grid.yh);
+  M.idx = idx;
+  M.dir = 27;
+  if (idx != kOOB) {
+    const uint32_t jz = idx % nz, jx = (idx / nz) % nx, jy = idx / (nz * nx);
+    const bool bad = (M.c3 < a.grid.zh[jz]) | (M.c3 >= a.grid.zh[jz + 1]) | (M.c1 < a.grid.xh[jx]) |
+                     (M.c1 >= a.grid.xh[jx + 1]) | (M.c2 < a.grid.yh[jy]) | (M.c2 >= a.grid.yh[jy + 1]);
+    if (report && bad) set_error(a.ctl, CLEO_ERR_BOUNDS, g, k);
+    M.dir = (dz + 1) + 3 * (dx + 1) + 9 * (dy + 1);
+  }
+  return M;
+}
+
 // MoveSupersInGridboxesFunctor (movesupersindomain.hpp:47-93) with PredCorrMotion over
 // CartesianMaps; also counts, per gridbox, how many drops go to each of the 27 neighbour
 // directions (and out of the domain) for the stable neighbour sort.
 __device__ __forceinline__ uint32_t motion_gbx(const MotionArgs& a, uint32_t g, int lane) {
   const uint32_t base = a.off[g];
   const uint32_t nfull = a.off[g + 1] - base;
-  const uint32_t nz = a.grid.nz, nx = a.grid.nx, ny = a.grid.ny;
-  const uint32_t iz = g % nz, ix = (g / nz) % nx, iy = g / (nz * nx);
-  const double zlo = a.grid.zh[iz], zhi = a.grid.zh[iz + 1];
-  const double xlo = a.grid.xh[ix], xhi = a.grid.xh[ix + 1];
-  const double ylo = a.grid.yh[iy], yhi = a.grid.yh[iy + 1];
-  const double wlo = a.st.wlo[g], whi = a.st.whi[g];
-  const double ulo = a.st.ulo[g], uhi = a.st.uhi[g];
-  const double vlo = a.st.vlo[g], vhi = a.st.vhi[g];
-  const double delt = a.delt;
+  const GbxGeom G = gbx_geom(a, g);
   uint32_t mycount = 0;
   for (uint32_t c = 0; c < nfull; c += kWave) {
     const uint32_t k = base + c + lane;
     const bool act = c + lane < nfull;
     int dir = 0;
     if (act) {
-      // PredCorr::delta_coord3/1/2 (predcorr.hpp:84-145)
-      double coord3 = a.s.c3[k];
-      const double terminal = terminal_velocity(a.tv, a.s.r[k]);
-      double vel3 = interpolation(zlo, zhi, wlo, whi, coord3);
-      vel3 -= terminal;
-      coord3 += vel3 * delt;
-      double corrvel3 = interpolation(zlo, zhi, wlo, whi, coord3);
-      corrvel3 -= terminal;
-      const double delta3 = (vel3 + corrvel3) * (delt / 2);
-      double coord1 = a.s.c1[k];
-      const double vel1 = interpolation(xlo, xhi, ulo, uhi, coord1);
-      coord1 += vel1 * delt;
-      const double corrvel1 = interpolation(xlo, xhi, ulo, uhi, coord1);
-      const double delta1 = (vel1 + corrvel1) * (delt / 2);
-      double coord2 = a.s.c2[k];
-      const double vel2 = interpolation(ylo, yhi, vlo, vhi, coord2);
-      coord2 += vel2 * delt;
-      const double corrvel2 = interpolation(ylo, yhi, vlo, vhi, coord2);
-      const double delta2 = (vel2 + corrvel2) * (delt / 2);
-      // cfl_criteria (cfl_criteria.hpp:40-57)
-      bool cfl = (fabs(delta3) <= fabs(zhi - zlo));
-      cfl = (fabs(delta1) <= fabs(xhi - xlo)) && cfl;
-      cfl = (fabs(delta2) <= fabs(yhi - ylo)) && cfl;
-      if (!cfl) set_error(a.ctl, CLEO_ERR_CFL, g, k);
-      double c3 = a.s.c3[k] + delta3;
-      double c1 = a.s.c1[k] + delta1;
-      double c2 = a.s.c2[k] + delta2;
-      // superdrop_gbx (predcorrmotion.hpp:85-106): z, then x, then y
-      uint32_t idx = g;
-      const int dz = axis_step(idx, c3, iz, nz, 1u, a.grid.per[0] != 0, a.grid.zh);
-      int dx = 0, dy = 0;
-      if (idx != kOOB) dx = axis_step(idx, c1, ix, nx, nz, a.grid.per[1] != 0, a.grid.xh);
-      if (idx != kOOB) dy = axis_step(idx, c2, iy, ny, nz * nx, a.grid.per[2] != 0, a.grid.yh);
-      a.s.c3[k] = c3;
-      a.s.c1[k] = c1;
-      a.s.c2[k] = c2;
-      a.s.gbx[k] = idx;
-      if (idx != kOOB) {
-        // CartesianCheckBounds (cartesian_motion.hpp:38-52)
-        const uint32_t jz = idx % nz, jx = (idx / nz) % nx, jy = idx / (nz * nx);
-        const bool bad = (c3 < a.grid.zh[jz]) | (c3 >= a.grid.zh[jz + 1]) | (c1 < a.grid.xh[jx]) |
-                         (c1 >= a.grid.xh[jx + 1]) | (c2 < a.grid.yh[jy]) | (c2 >= a.grid.yh[jy + 1]);
-        if (bad) set_error(a.ctl, CLEO_ERR_BOUNDS, g, k);
-        dir = (dz + 1) + 3 * (dx + 1) + 9 * (dy + 1);
-      } else {
-        dir = 27;
-      }
+      const Moved M = move_drop(a, g, G, k, true);
+      a.s.c3[k] = M.c3;
+      a.s.c1[k] = M.c1;
+      a.s.c2[k] = M.c2;
+      a.s.gbx[k] = M.idx;
+      dir = M.dir;
     }
     uint64_t mask = __ballot(act);
     while (mask) {
@@ -851,6 +894,72 @@ __global__ void k_nsort_oob(NsortArgs a, uint64_t ntot) {
     a.dst.msol[d] = a.src.msol[p];
     a.dst.id[d] = a.src.id[p];
   }
+}
+
+// Second pass of the fused motion + sort (the first is k_motion_drops<false>): the drops of
+// gridbox g move again, with the arithmetic of the counting pass, and go with their new coordinates
+// and gridbox straight to their place in the sorted store.  The place is the start of the
+// (destination, step) run left in the count table by k_nsort_tot plus the drop's rank among this
+// gridbox's drops that took the same step -- the stable order of the scatter above.
+__device__ __forceinline__ void motion_scatter_gbx(const MotionArgs& a, const NsortArgs& n, uint32_t g, int lane) {
+  const uint32_t base = a.off[g];
+  const uint32_t nfull = a.off[g + 1] - base;
+  const GbxGeom G = gbx_geom(a, g);
+  uint32_t run = 0;   // lane L < 28: next position of the drops of step L
+  uint32_t have = 0;  // bit L: the run of step L has its start
+  for (uint32_t c = 0; c < nfull; c += kWave) {
+    const uint32_t k = base + c + lane;
+    const bool act = c + lane < nfull;
+    Moved M{0.0, 0.0, 0.0, 0u, 0};
+    if (act) M = move_drop(a, g, G, k, false);
+    const int dir = act ? M.dir : -1;
+    // the steps present in this chunk; the runs they open get their starts in one round of loads
+    uint32_t present = 0;
+    uint64_t mask = __ballot(act);
+    while (mask) {
+      const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+      const int v = static_cast<int>(readlane_u32(static_cast<uint32_t>(dir), leader));
+      present |= 1u << v;
+      mask &= ~__ballot(act && dir == v);
+    }
+    const uint32_t need = present & ~have;
+    if (lane < 28 && ((need >> lane) & 1u)) {
+      if (lane == 27) {
+        run = n.base[a.grid.ngbx] + n.oobpre[g];
+      } else {
+        const uint32_t d = neighbour_of(a.grid, g, lane);
+        run = n.base[d] + n.cnt[28ull * d + lane];
+      }
+    }
+    have |= need;
+    uint32_t dest = 0;
+    mask = __ballot(act);
+    while (mask) {
+      const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+      const int v = static_cast<int>(readlane_u32(static_cast<uint32_t>(dir), leader));
+      const uint64_t m = __ballot(act && dir == v);
+      const uint32_t start = readlane_u32(run, v);
+      if ((m >> lane) & 1ull) dest = start + static_cast<uint32_t>(__popcll(m & lanemask_lt()));
+      if (lane == v) run += static_cast<uint32_t>(__popcll(m));
+      mask &= ~m;
+    }
+    if (act && dest >= n.ntot) {  // counts inconsistent with the moves: never write out of range
+      set_error(n.ctl, CLEO_ERR_SORT_NONLOCAL, g, k);
+    } else if (act) {
+      n.dst.gbx[dest] = M.idx;
+      n.dst.c3[dest] = M.c3;
+      n.dst.c1[dest] = M.c1;
+      n.dst.c2[dest] = M.c2;
+      n.dst.xi[dest] = a.s.xi[k];
+      n.dst.r[dest] = a.s.r[k];
+      n.dst.msol[dest] = a.s.msol[k];
+      n.dst.id[dest] = a.s.id[k];
+    }
+  }
+}
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_motion_scatter(MotionArgs a, NsortArgs n) {
+  const int lane = lane_id();
+  for (uint32_t g = wave_index(); g < a.grid.ngbx; g += wave_count()) motion_scatter_gbx(a, n, g, lane);
 }
 
 // counts of the re-sort after collisions: active drops stay, nulls leave the domain
@@ -1061,6 +1170,8 @@ static inline unsigned nblocks(uint64_t n, uint64_t per) {
   return static_cast<unsigned>(b == 0 ? 1 : b);
 }
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v);
+template <bool kWrite>
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_motion_drops(MotionArgs a);
 
 // events around the main kernel of a stage (stage timing), recorded only while kev_on is set
 static inline void kev_mark(cleo_ctx* c, int i) {
@@ -1184,11 +1295,11 @@ void scan_u32(cleo_ctx* c, const uint32_t* in, uint64_t len, uint32_t stride, ui
   scan_rec(c, in, len, stride, offset, out, flag, c->scan_tmp);
 }
 
-void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool from_nact) {
-  const uint32_t ngbx = c->grid.ngbx;
+// argument block of the neighbour-sort kernels reading `src` and writing `dst`
+static NsortArgs nsort_args(cleo_ctx* c, const SoA& src, const SoA& dst, const uint32_t* flag) {
   NsortArgs a{};
-  a.src = c->buf[src];
-  a.dst = c->buf[dst];
+  a.src = src;
+  a.dst = dst;
   a.off = c->off;
   a.base = c->off_alt;
   a.cnt = c->cnt;
@@ -1199,21 +1310,53 @@ void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool fr
   a.ctl = c->ctl;
   a.flag = flag;
   a.ntot = c->ntot;
-  if (from_nact)
-    hipLaunchKernelGGL(k_cnt_from_nact, dim3(nblocks(28ull * ngbx, 256)), dim3(256), 0, c->stream, c->off, c->nact, ngbx,
-                       c->cnt, flag);
+  return a;
+}
+// totals per destination, start offsets in the count table, new refs and the out-of-domain prefix
+static void nsort_tables(cleo_ctx* c, const NsortArgs& a, const uint32_t* flag) {
+  const uint32_t ngbx = c->grid.ngbx;
   hipLaunchKernelGGL(k_nsort_tot, dim3(nblocks(ngbx, 256)), dim3(256), 0, c->stream, a);
   scan_u32(c, c->tot, ngbx, 1u, 0u, c->off_alt, flag);
   scan_u32(c, c->cnt, ngbx, 28u, 27u, c->oobpre, flag);
+}
+// the sorted store in buffer `dst` becomes current: refs and active counts follow
+static void nsort_commit(cleo_ctx* c, int dst) {
+  const uint32_t ngbx = c->grid.ngbx;
+  c->cur = dst;
+  hipLaunchKernelGGL(k_copy_u32, dim3(stride_blocks(ngbx + 1ull)), dim3(256), 0, c->stream, c->off_alt, c->off,
+                     ngbx + 1ull, static_cast<const uint32_t*>(nullptr));
+  hipLaunchKernelGGL(k_set_nact, dim3(nblocks(ngbx, 256)), dim3(256), 0, c->stream, c->off, ngbx, c->nact,
+                     static_cast<const uint32_t*>(nullptr));
+}
+
+void neighbour_sort_to(cleo_ctx* c, const SoA& src, const SoA& dst, int commit_dst) {
+  const uint32_t ngbx = c->grid.ngbx;
+  NsortArgs a = nsort_args(c, src, dst, nullptr);
+  nsort_tables(c, a, nullptr);
+  hipLaunchKernelGGL(k_nsort_scatter, dim3(gbx_blocks(ngbx, a.wpb, c->tiles.motion_grid_waves)), dim3(64 * a.wpb), 0,
+                     c->stream, a);
+  hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, a, c->ntot);
+  hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, static_cast<const uint32_t*>(nullptr));
+  if (commit_dst >= 0) nsort_commit(c, commit_dst);
+}
+
+void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool from_nact) {
+  if (flag == nullptr && !from_nact) {
+    neighbour_sort_to(c, c->buf[src], c->buf[dst], dst);
+    return;
+  }
+  const uint32_t ngbx = c->grid.ngbx;
+  NsortArgs a = nsort_args(c, c->buf[src], c->buf[dst], flag);
+  if (from_nact)
+    hipLaunchKernelGGL(k_cnt_from_nact, dim3(nblocks(28ull * ngbx, 256)), dim3(256), 0, c->stream, c->off, c->nact, ngbx,
+                       c->cnt, flag);
+  nsort_tables(c, a, flag);
   hipLaunchKernelGGL(k_nsort_scatter, dim3(gbx_blocks(ngbx, a.wpb, c->tiles.motion_grid_waves)), dim3(64 * a.wpb), 0,
                      c->stream, a);
   hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, a, c->ntot);
   hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, flag);
   if (flag == nullptr) {
-    c->cur = dst;
-    hipLaunchKernelGGL(k_copy_u32, dim3(stride_blocks(ngbx + 1ull)), dim3(256), 0, c->stream, c->off_alt, c->off,
-                       ngbx + 1ull, flag);
-    hipLaunchKernelGGL(k_set_nact, dim3(nblocks(ngbx, 256)), dim3(256), 0, c->stream, c->off, ngbx, c->nact, flag);
+    nsort_commit(c, dst);
   } else {
     hipLaunchKernelGGL(k_copy_soa, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, c->buf[dst], c->buf[src],
                        0ull, c->ntot, flag);
@@ -1222,6 +1365,28 @@ void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool fr
     hipLaunchKernelGGL(k_set_nact, dim3(nblocks(ngbx, 256)), dim3(256), 0, c->stream, c->off, ngbx, c->nact, flag);
     hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c->stream, c->ctl);
   }
+}
+
+// Motion + neighbour sort in two passes over the drops: count the moves, build the tables, then
+// move again and write each drop to its sorted place in `dst` (src is left unchanged).
+// commit_dst >= 0: that buffer becomes current.
+void motion_sort_fused(cleo_ctx* c, const MotionArgs& args, const SoA& dst, int commit_dst) {
+  const uint32_t ngbx = c->grid.ngbx;
+  MotionArgs a = args;
+  hipLaunchKernelGGL(k_fill_u32, dim3(stride_blocks(28ull * ngbx)), dim3(256), 0, c->stream, a.cnt, 28ull * ngbx, 0u);
+  const unsigned nb = drop_blocks(c->ntot, c->tiles.motion_grid_waves);
+  a.stat = stat_slots(c, 4ull * nb);
+  hipLaunchKernelGGL(k_motion_drops<false>, dim3(nb), dim3(256), 0, c->stream, a);
+  launch_stats_reduce(c, 4u * nb, ST_MOTION_DROPS, 1);
+  NsortArgs n = nsort_args(c, a.s, dst, nullptr);
+  nsort_tables(c, n, nullptr);
+  kev_mark(c, 0);
+  hipLaunchKernelGGL(k_motion_scatter, dim3(gbx_blocks(ngbx, n.wpb, c->tiles.motion_grid_waves)), dim3(64 * n.wpb), 0,
+                     c->stream, a, n);
+  kev_mark(c, 1);
+  hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, n, c->ntot);
+  hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, static_cast<const uint32_t*>(nullptr));
+  if (commit_dst >= 0) nsort_commit(c, commit_dst);
 }
 
 void generic_sort(cleo_ctx* c) {
@@ -1294,48 +1459,99 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_prep(CondFlatArg
   }
 }
 
-// per active drop: Koehler factors, Newton solve, radius update, mass change * xi into dm[k]
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_drops(CondFlatArgs a) {
-  const uint32_t ndom = a.off[a.ngbx];
-  uint32_t iters = 0, subs = 0, restarts = 0, solved = 0;
-  for (uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < ndom;
-       k += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const uint32_t g = a.s.gbx[k];
-    if (g < a.ngbx && k < a.off[g] + a.nact[g]) {
-      const double s_ratio = a.gs[2 * g], ffactor = a.gs[2 * g + 1];
-      const double temp = a.st.temp[g];
-      const double r = a.s.r[k], msol = a.s.msol[k];
-      const uint64_t xi = a.s.xi[k];
-      bool bad = false;
-      const double old_m = condensate_mass(r, msol, a.sol, &bad);
-      double akoh, bkoh;
-      kohler_factors(msol, temp, a.sol, akoh, bkoh);
-      const double warm = a.warm ? a.warm[k] : __longlong_as_double(0x7FF8000000000000ll);
-      SolveOut o{0, false, false, false};
-      const double newr = solve_condensation(a.cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
-      const double rnew = fmax(newr, dryradius(msol, a.sol));
-      a.s.r[k] = rnew;
-      a.dm[k] = (condensate_mass(rnew, msol, a.sol, &bad) - old_m) * static_cast<double>(xi);
-      if (bad) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, g, k);
-      if (o.maxed) set_error(a.ctl, CLEO_ERR_NEWTON_MAXITERS, g, k);
-      if (a.niters) a.niters[k] = o.iters;
-      iters += o.iters;
-      subs += o.substep ? 1u : 0u;
-      restarts += o.restarted ? 1u : 0u;
-      solved += 1;
-    }
-  }
-  iters = wave_sum_u32(iters);
-  subs = wave_sum_u32(subs);
-  restarts = wave_sum_u32(restarts);
-  solved = wave_sum_u32(solved);
+// One active drop k of gridbox g: Koehler factors, Newton solve, radius update, mass change * xi
+// into dm[k] (condensation.cpp:44-125).  With cfg.budget the solve may give up (returns false,
+// nothing written) so that the drop is solved again, without a budget, by k_cond_deferred.
+struct CondStats {
+  uint32_t solved, iters, subs, restarts;
+};
+__device__ __forceinline__ bool cond_drop(const CondFlatArgs& a, const NewtonCfg& cfg, uint32_t g, uint64_t k,
+                                          CondStats& st) {
+  const double s_ratio = a.gs[2 * g], ffactor = a.gs[2 * g + 1];
+  const double temp = a.st.temp[g];
+  const double r = a.s.r[k], msol = a.s.msol[k];
+  double akoh, bkoh;
+  kohler_factors(msol, temp, a.sol, akoh, bkoh);
+  const double warm = a.warm ? a.warm[k] : __longlong_as_double(0x7FF8000000000000ll);
+  SolveOut o{0, false, false, false, false};
+  const double newr = solve_condensation(cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
+  if (o.deferred) return false;
+  bool bad = false;
+  const double old_m = condensate_mass(r, msol, a.sol, &bad);
+  const double rnew = fmax(newr, dryradius(msol, a.sol));
+  a.s.r[k] = rnew;
+  a.dm[k] = (condensate_mass(rnew, msol, a.sol, &bad) - old_m) * static_cast<double>(a.s.xi[k]);
+  if (bad) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, g, k);
+  if (o.maxed) set_error(a.ctl, CLEO_ERR_NEWTON_MAXITERS, g, k);
+  if (a.niters) a.niters[k] = o.iters;
+  st.iters += o.iters;
+  st.subs += o.substep ? 1u : 0u;
+  st.restarts += o.restarted ? 1u : 0u;
+  st.solved += 1;
+  return true;
+}
+__device__ __forceinline__ void put_cond_stats(const StatSlots& sl, CondStats st) {
+  st.solved = wave_sum_u32(st.solved);
+  st.iters = wave_sum_u32(st.iters);
+  st.subs = wave_sum_u32(st.subs);
+  st.restarts = wave_sum_u32(st.restarts);
   if (lane_id() == 0) {
     const uint32_t slot = wave_index();
-    put_stat(a.stat, 0, slot, solved);
-    put_stat(a.stat, 1, slot, iters);
-    put_stat(a.stat, 2, slot, subs);
-    put_stat(a.stat, 3, slot, restarts);
+    put_stat(sl, 0, slot, st.solved);
+    put_stat(sl, 1, slot, st.iters);
+    put_stat(sl, 2, slot, st.subs);
+    put_stat(sl, 3, slot, st.restarts);
   }
+}
+
+// per active drop, one thread each.  Drops whose solve outlives cfg.budget iterations (a few per
+// cent; their lanes would hold the other 63 of the wave) go to this wave's segment of the
+// deferred list, up to `dcap` of them, and are solved by k_cond_deferred.
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_drops(CondFlatArgs a) {
+  const uint32_t ndom = a.off[a.ngbx];
+  const int lane = lane_id();
+  CondStats st{0, 0, 0, 0};
+  uint32_t ndef = 0;
+  uint32_t* mylist = a.dlist + static_cast<uint64_t>(wave_index()) * a.dcap;
+  NewtonCfg nobudget = a.cfg;
+  nobudget.budget = 0;
+  for (uint64_t k0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x; k0 < ndom;
+       k0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t k = k0 + threadIdx.x;
+    bool deferred = false;
+    if (k < ndom) {
+      const uint32_t g = a.s.gbx[k];
+      if (g < a.ngbx && k < a.off[g] + a.nact[g]) deferred = !cond_drop(a, a.cfg, g, k, st);
+    }
+    const uint64_t m = __ballot(deferred);
+    if (m) {
+      const uint32_t idx = ndef + static_cast<uint32_t>(__popcll(m & lanemask_lt()));
+      if (deferred) {
+        if (idx < a.dcap)
+          mylist[idx] = static_cast<uint32_t>(k);
+        else
+          cond_drop(a, nobudget, a.s.gbx[k], k, st);  // the segment is full: solve it here
+      }
+      ndef = min(a.dcap, ndef + static_cast<uint32_t>(__popcll(m)));
+    }
+  }
+  if (a.dcount && lane == 0) a.dcount[wave_index()] = ndef;
+  put_cond_stats(a.stat, st);
+}
+
+// the deferred drops of wave w's segment, solved without a budget by wave w of this launch
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_deferred(CondFlatArgs a) {
+  const int lane = lane_id();
+  CondStats st{0, 0, 0, 0};
+  NewtonCfg nobudget = a.cfg;
+  nobudget.budget = 0;
+  const uint32_t n = a.dcount[wave_index()];
+  const uint32_t* mylist = a.dlist + static_cast<uint64_t>(wave_index()) * a.dcap;
+  for (uint32_t i = lane; i < n; i += kWave) {
+    const uint32_t k = mylist[i];
+    cond_drop(a, nobudget, a.s.gbx[k], k, st);
+  }
+  put_cond_stats(a.stat, st);
 }
 
 // ---- condensation with a per-wave task ring ------------------------------------------------------
@@ -1600,66 +1816,23 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
 
 // MoveSupersInGridboxesFunctor per drop; direction counts aggregated per wave, then atomics
 // one drop: move it and return its count-table key: 28*d + step for destination d, 28*g + 27 when
-// it leaves the domain
-__device__ __forceinline__ uint32_t motion_drop(const MotionArgs& a, uint64_t k) {
-  uint32_t key;
-  {
-    const uint32_t g = a.s.gbx[k];
-    const uint32_t nz = a.grid.nz, nx = a.grid.nx, ny = a.grid.ny;
-    const uint32_t iz = g % nz, ix = (g / nz) % nx, iy = g / (nz * nx);
-    const double zlo = a.grid.zh[iz], zhi = a.grid.zh[iz + 1];
-    const double xlo = a.grid.xh[ix], xhi = a.grid.xh[ix + 1];
-    const double ylo = a.grid.yh[iy], yhi = a.grid.yh[iy + 1];
-    const double wlo = a.st.wlo[g], whi = a.st.whi[g];
-    const double ulo = a.st.ulo[g], uhi = a.st.uhi[g];
-    const double vlo = a.st.vlo[g], vhi = a.st.vhi[g];
-    const double delt = a.delt;
-    const double c3_0 = a.s.c3[k], c1_0 = a.s.c1[k], c2_0 = a.s.c2[k];
-    double coord3 = c3_0;
-    const double terminal = terminal_velocity(a.tv, a.s.r[k]);
-    double vel3 = interpolation(zlo, zhi, wlo, whi, coord3);
-    vel3 -= terminal;
-    coord3 += vel3 * delt;
-    double corrvel3 = interpolation(zlo, zhi, wlo, whi, coord3);
-    corrvel3 -= terminal;
-    const double delta3 = (vel3 + corrvel3) * (delt / 2);
-    double coord1 = c1_0;
-    const double vel1 = interpolation(xlo, xhi, ulo, uhi, coord1);
-    coord1 += vel1 * delt;
-    const double corrvel1 = interpolation(xlo, xhi, ulo, uhi, coord1);
-    const double delta1 = (vel1 + corrvel1) * (delt / 2);
-    double coord2 = c2_0;
-    const double vel2 = interpolation(ylo, yhi, vlo, vhi, coord2);
-    coord2 += vel2 * delt;
-    const double corrvel2 = interpolation(ylo, yhi, vlo, vhi, coord2);
-    const double delta2 = (vel2 + corrvel2) * (delt / 2);
-    bool cfl = (fabs(delta3) <= fabs(zhi - zlo));
-    cfl = (fabs(delta1) <= fabs(xhi - xlo)) && cfl;
-    cfl = (fabs(delta2) <= fabs(yhi - ylo)) && cfl;
-    if (!cfl) set_error(a.ctl, CLEO_ERR_CFL, g, k);
-    double c3 = c3_0 + delta3, c1 = c1_0 + delta1, c2 = c2_0 + delta2;
-    uint32_t idx = g;
-    const int dz = axis_step(idx, c3, iz, nz, 1u, a.grid.per[0] != 0, a.grid.zh);
-    int dx = 0, dy = 0;
-    if (idx != kOOB) dx = axis_step(idx, c1, ix, nx, nz, a.grid.per[1] != 0, a.grid.xh);
-    if (idx != kOOB) dy = axis_step(idx, c2, iy, ny, nz * nx, a.grid.per[2] != 0, a.grid.yh);
-    a.s.c3[k] = c3;
-    a.s.c1[k] = c1;
-    a.s.c2[k] = c2;
-    a.s.gbx[k] = idx;
-    int dir = 27;
-    if (idx != kOOB) {
-      const uint32_t jz = idx % nz, jx = (idx / nz) % nx, jy = idx / (nz * nx);
-      const bool bad = (c3 < a.grid.zh[jz]) | (c3 >= a.grid.zh[jz + 1]) | (c1 < a.grid.xh[jx]) |
-                       (c1 >= a.grid.xh[jx + 1]) | (c2 < a.grid.yh[jy]) | (c2 >= a.grid.yh[jy + 1]);
-      if (bad) set_error(a.ctl, CLEO_ERR_BOUNDS, g, k);
-      dir = (dz + 1) + 3 * (dx + 1) + 9 * (dy + 1);
-    }
-    key = idx != kOOB ? 28u * idx + static_cast<uint32_t>(dir) : 28u * g + 27u;
+// it leaves the domain (`write`: the new coordinates and gridbox go back to the store)
+__device__ __forceinline__ uint32_t motion_drop(const MotionArgs& a, uint64_t k, bool write) {
+  const uint32_t g = a.s.gbx[k];
+  const GbxGeom G = gbx_geom(a, g);
+  const Moved M = move_drop(a, g, G, k, true);
+  if (write) {
+    a.s.c3[k] = M.c3;
+    a.s.c1[k] = M.c1;
+    a.s.c2[k] = M.c2;
+    a.s.gbx[k] = M.idx;
   }
-  return key;
+  return M.idx != kOOB ? 28u * M.idx + static_cast<uint32_t>(M.dir) : 28u * g + 27u;
 }
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_motion_flat(MotionArgs a) {
+// thread per drop: the motion step and the destination-major counts (without kWrite this is the
+// counting pass of the fused motion + sort: nothing of the drops is stored)
+template <bool kWrite>
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_motion_drops(MotionArgs a) {
   const int lane = lane_id();
   const uint32_t ndom = a.off[a.grid.ngbx];
   uint32_t nmoved = 0;
@@ -1668,7 +1841,7 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_motion_flat(MotionArg
        k0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     const uint64_t k = k0 + threadIdx.x;
     const bool act = k < ndom;
-    const uint32_t key = act ? motion_drop(a, k) : 0u;
+    const uint32_t key = act ? motion_drop(a, k, kWrite) : 0u;
     uint64_t mask = __ballot(act);
     nmoved += static_cast<uint32_t>(__popcll(mask));
     while (mask) {
@@ -1700,8 +1873,19 @@ void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& args) {
   } else {
     const unsigned nb = drop_blocks(c->ntot, gw);
     a.stat = stat_slots(c, 4ull * nb);  // every wave of k_cond_drops writes its slot
+    const uint32_t budget = c->tiles.cond_defer;
+    a.cfg.budget = budget;
+    // per-wave segments of the deferred list: a quarter of a wave's drops, at least 16
+    const uint64_t per_wave = (c->ntot + 4ull * nb - 1) / (4ull * nb);
+    a.dcap = budget ? static_cast<uint32_t>(per_wave / 4 + 16) : 0u;
+    a.dlist = c->dlist;
+    a.dcount = budget ? c->dcount : nullptr;
     kev_mark(c, 0);
     hipLaunchKernelGGL(k_cond_drops, dim3(nb), dim3(256), 0, c->stream, a);
+    if (budget) {
+      launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
+      hipLaunchKernelGGL(k_cond_deferred, dim3(nb), dim3(256), 0, c->stream, a);
+    }
     kev_mark(c, 1);
     launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
   }
@@ -1716,7 +1900,7 @@ void launch_motion_flat(cleo_ctx* c, const MotionArgs& args) {
   hipLaunchKernelGGL(k_fill_u32, dim3(stride_blocks(28ull * a.grid.ngbx)), dim3(256), 0, c->stream, a.cnt,
                      28ull * a.grid.ngbx, 0u);
   kev_mark(c, 0);
-  hipLaunchKernelGGL(k_motion_flat, dim3(nb), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(k_motion_drops<true>, dim3(nb), dim3(256), 0, c->stream, a);
   kev_mark(c, 1);
   launch_stats_reduce(c, 4u * nb, ST_MOTION_DROPS, 1);
 }

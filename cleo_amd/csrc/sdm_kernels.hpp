@@ -10,7 +10,8 @@ constexpr int kStatSlotsMax = 4;       // counters per wave of one kernel
 // tiled collisions: byte permutations of up to kTileMaxN drops, kTilePermBytes of them per wave
 constexpr uint32_t kTileMaxN = 255;
 constexpr uint32_t kTilePermBytes = 8192;
-constexpr uint32_t kCollTileLdsBytes = kTilePermBytes + 4u * (2u * 65u + 4u * 64u);
+constexpr uint32_t kCollTileLdsBytes = kTilePermBytes + 4u * (65u + 3u * 64u);
+constexpr uint32_t kBigGbx = 0xFFFFFFFFu;  // tile LDS marker of a gridbox on the per-gridbox path
 // condensation task ring: prepared drops per wave, and waves per CU of the default grid
 constexpr uint32_t kRing = 128;
 constexpr uint32_t kCondRingBytes = kRing * (11u * 8u + 8u + 2u * 4u);
@@ -111,6 +112,9 @@ struct CondFlatArgs {
   double* gs;  // 2 per gridbox: saturation ratio, diffusion factor
   double* dm;  // per drop: condensed mass * xi
   StatSlots stat;  // ST_COND_DROPS .. ST_WARM_RESTARTS
+  uint32_t* dlist;   // deferred drops, `dcap` per wave of k_cond_drops
+  uint32_t* dcount;  // deferred drops per wave (null: no deferral)
+  uint32_t dcap;
 };
 
 // launchers (sdm_kernels.hip); all enqueue on ctx->stream
@@ -133,6 +137,10 @@ size_t scan_tmp_words(uint64_t len);
 // stable neighbour sort of buf[src] into buf[dst] after motion (flag == nullptr) or after
 // collisions (flag = &ctl->resort_flag, data ends back in buf[src] and refs in c->off)
 void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool from_nact);
+// the neighbour sort of `src` (after motion) into `dst`; commit_dst >= 0: that buffer becomes current
+void neighbour_sort_to(cleo_ctx* c, const SoA& src, const SoA& dst, int commit_dst);
+// motion + neighbour sort fused into two passes (motion_flat 2): src = args.s is left unchanged
+void motion_sort_fused(cleo_ctx* c, const MotionArgs& args, const SoA& dst, int commit_dst);
 // generic stable sort of buf[cur] into buf[1-cur]; swaps cur; sets off and nact
 void generic_sort(cleo_ctx* c);
 void launch_copy_oob_region(cleo_ctx* c, int src, int dst);

@@ -269,6 +269,8 @@ typedef struct {
   uint32_t coll_grid_waves;        /*   0: one wave per gridbox (one thread per drop) */
   uint32_t motion_grid_waves;
   uint32_t coll_flat;              /* 1: 64 gridboxes per wave, pairs flattened over the lanes */
+  uint32_t cond_defer;             /* cond_flat 1: Newton iterations after which a drop is set
+                                      aside and solved by a second launch (0: never) */
 } cleo_tiles;
 int cleo_set_tiles(cleo_ctx* ctx, const cleo_tiles* tiles);
 int cleo_get_tiles(cleo_ctx* ctx, cleo_tiles* tiles);

@@ -43,13 +43,13 @@ def test_upload_sort_random_order(gpu, oracle):
     assert np.array_equal(g.refs(), o.refs())
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant,defer", [(0, 0), (1, 0), (1, 3), (2, 0)])
 @pytest.mark.parametrize("ndims,mean", [((4, 4, 4), 122), ((2, 3, 2), 7)])
-def test_condensation(gpu, oracle, ndims, mean, variant):
-    _say(f"case: condensation ndims={ndims} mean={mean} variant={variant}")
+def test_condensation(gpu, oracle, ndims, mean, variant, defer):
+    _say(f"case: condensation ndims={ndims} mean={mean} variant={variant} defer={defer}")
     pop = P.condensation(SEED, ndims=ndims, mean=mean)
     g, o = _both(gpu, oracle, pop)
-    t = abi.Tiles(1, 1, 1, variant, 0, 4 if variant == 2 else 0, 0, 0, 0)
+    t = abi.Tiles(1, 1, 1, variant, 0, 4 if variant == 2 else 0, 0, 0, 0, defer)
     assert gpu.lib.cleo_set_tiles(g.h, abi.ctypes_byref(t)) == 0
     g.record_niters(True)
     o.record_niters(True)
@@ -147,12 +147,15 @@ def test_collisions_tiled_mixed_sizes(gpu, oracle, ndims, mean):
         assert sg[k] == so[k], k
 
 
-def test_motion(gpu, oracle):
-    _say("case: predictor-corrector motion + neighbour sort, 6 steps")
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_motion(gpu, oracle, variant):
+    _say(f"case: predictor-corrector motion + neighbour sort, 6 steps, motion variant {variant}")
     pop = P.coupled(SEED, ndims=(6, 5, 4), spacing_m=25.0, mean=50, wmax_mps=6.0)
     pop.state["vvel_lo"][:] = 1.5
     pop.state["vvel_hi"][:] = -2.0
     g, o = _both(gpu, oracle, pop)
+    t = abi.Tiles(1, 1, 1, 0, variant, 0, 0, 0, 0, 0)
+    assert gpu.lib.cleo_set_tiles(g.h, abi.ctypes_byref(t)) == 0
     p = motion_params()
     for step in range(6):
         assert g.motion(p) == 0
@@ -221,12 +224,16 @@ def test_tiles_do_not_change_results(gpu):
     outs, stats = [], []
     # the small grids make every wave loop over several gridboxes (or drop chunks); cond_flat 2 is
     # the task-ring condensation (with 8 or 1 waves its rings drain many drops each)
-    rows = [(wpb, flat, gw, flat) for wpb, flat, gw in ((1, 0, 0), (2, 0, 0), (4, 0, 0), (1, 1, 0), (1, 0, 64),
-                                                           (2, 1, 64), (4, 0, 8), (1, 1, 4), (4, 1, 0), (2, 1, 1))]
-    rows += [(1, 2, 0, 1), (1, 2, 8, 0), (2, 2, 4, 1)]
-    for wpb, cflat, gw, flat in rows:
+    rows = [(wpb, flat, gw, flat, 0) for wpb, flat, gw in ((1, 0, 0), (2, 0, 0), (4, 0, 0), (1, 1, 0), (1, 0, 64),
+                                                              (2, 1, 64), (4, 0, 8), (1, 1, 4), (4, 1, 0), (2, 1, 1))]
+    rows += [(1, 2, 0, 1, 0), (1, 2, 8, 0, 0), (2, 2, 4, 1, 0)]
+    # deferred slow drops: budgets that set many drops aside, and a grid whose segments overflow
+    rows += [(1, 1, 0, 1, 2), (1, 1, 8, 0, 3), (1, 1, 1, 1, 6)]
+    # fused motion + sort (motion variant 2), with one wave per gridbox and with looping waves
+    rows += [(1, 1, 0, 2, 0), (2, 2, 16, 2, 0), (1, 1, 3, 2, 6)]
+    for wpb, cflat, gw, flat, defer in rows:
         d = make_domain(gpu, pop)
-        t = abi.Tiles(wpb, wpb, wpb, cflat, flat, gw, gw, gw, flat)
+        t = abi.Tiles(wpb, wpb, wpb, cflat, flat, gw, gw, gw, min(flat, 1), defer)
         assert gpu.lib.cleo_set_tiles(d.h, abi.ctypes_byref(t)) == 0
         d.set_sdm(s)
         d.rng_counter(11)
@@ -240,10 +247,14 @@ def test_tiles_do_not_change_results(gpu):
         assert st == stats[0]
 
 
-def test_multiblock_scans_coupled(gpu, oracle):
-    _say("case: 4096 gridboxes (multi-block scans), 2 coupled steps")
+@pytest.mark.parametrize("tiles", [None, (1, 1, 1, 1, 2, 8192, 0, 8192, 1, 6)])
+def test_multiblock_scans_coupled(gpu, oracle, tiles):
+    _say(f"case: 4096 gridboxes (multi-block scans), 2 coupled steps, tiles {tiles}")
     pop = P.coupled(SEED, ndims=(16, 16, 16), mean=64)
     g, o = _both(gpu, oracle, pop)
+    if tiles:
+        t = abi.Tiles(*tiles)
+        assert gpu.lib.cleo_set_tiles(g.h, abi.ctypes_byref(t)) == 0
     s = sdm_params(cond=cond_params(), coll=coll_params(), motion=motion_params())
     for d in (g, o):
         d.set_sdm(s)
