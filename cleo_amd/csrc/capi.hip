@@ -126,7 +126,10 @@ int enqueue_condensation(cleo_ctx* c, const cleo_cond_params& p, bool fuse_qcond
   if (p.interval == 0) return fail(c, CLEO_E_ARG, "condensation interval 0");
   if (!(p.minsubdelt_s > 0.0)) return fail(c, CLEO_E_ARG, "MINSUBTSTEP must be positive");
   CondArgs a = cond_args(c, p, fuse_qcond);
-  if (p.warm_start) {
+  // the thread-per-drop sequence evaluates the learned guesses itself (after the saturation
+  // ratios are known); the others read them from a separate guess pass
+  const bool inline_mlp = p.warm_start && c->tiles.cond_flat == 1;
+  if (p.warm_start && !inline_mlp) {
     const int rc = mlp_enqueue_guess(c, p, &a.warm);
     if (rc != CLEO_OK) return rc;
     if (c->rec_guess && c->guess)
@@ -135,6 +138,14 @@ int enqueue_condensation(cleo_ctx* c, const cleo_cond_params& p, bool fuse_qcond
   if (c->tiles.cond_flat) {
     CondFlatArgs f{a.s, a.off, a.nact, a.st, a.sol, a.cfg, a.ngbx, a.alter, a.fuse_qcond, a.ctl, a.niters,
                    a.warm, a.warm_limit, c->gs, c->dm};
+    if (inline_mlp) {
+      const int rc = mlp_check_ready(c);
+      if (rc != CLEO_OK) return rc;
+      f.mlp_inline = 1;
+      f.W = mlp_weights(c);
+      f.warm_buf = mlp_warm_buffer(c);
+      f.guess_out = (c->rec_guess && c->guess) ? c->guess : nullptr;
+    }
     launch_condensation_flat(c, f);
   } else {
     launch_condensation(c, a);

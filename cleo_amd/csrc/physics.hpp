@@ -285,8 +285,9 @@ struct SolveOut {
   bool deferred;  // the budget ran out: nothing of this solve is kept
 };
 // solve_condensation (impliciteuler.cpp:35-54). `warm` is a learned guess for z = r^2 or NaN.
-// Learned path: the unchanged two-phase Newton from `warm` with `warm_limit` as until-converged
-// limit; if that fails, the reference path from the plain guess.
+// Learned path (no reference counterpart): the reference's Newton update and convergence test
+// iterated from `warm` until converged, at most `warm_limit` iterations; if it has not converged
+// by then, the reference path from the plain guess.
 __device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, double s_ratio, double akoh,
                                                      double bkoh, double ffactor, double rprev, double warm,
                                                      uint32_t warm_limit, SolveOut& o) {
@@ -298,23 +299,15 @@ __device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, doubl
   double rsqrd;
   if (ucrit1 || ucrit2) {
     if (warm == warm) {
-      NewtonState s0;
-      s0.z = warm;
-      ode_g_dg(c, cfg.delt, rprev, warm, s0.g, s0.dg);
-      NewtonState s = s0;
-      bool conv = newton_iterate(cfg, c, cfg.delt, rprev, s, o.iters);
-      conv = newton_iterate(cfg, c, cfg.delt, rprev, s, o.iters);
-      if (conv) {
+      NewtonState s;
+      s.z = warm;
+      ode_g_dg(c, cfg.delt, rprev, warm, s.g, s.dg);
+      if (newton_until(cfg, c, warm_limit, cfg.delt, rprev, s, o.iters, o.deferred)) {
         rsqrd = s.z;
       } else {
-        s = s0;
-        if (newton_until(cfg, c, warm_limit, cfg.delt, rprev, s, o.iters, o.deferred)) {
-          rsqrd = s.z;
-        } else {
-          if (o.deferred) return 0.0;
-          o.restarted = true;
-          rsqrd = newton_integrate(cfg, c, cfg.delt, rprev, ziter, o.iters, o.maxed, o.deferred);
-        }
+        if (o.deferred) return 0.0;
+        o.restarted = true;
+        rsqrd = newton_integrate(cfg, c, cfg.delt, rprev, ziter, o.iters, o.maxed, o.deferred);
       }
     } else {
       rsqrd = newton_integrate(cfg, c, cfg.delt, rprev, ziter, o.iters, o.maxed, o.deferred);

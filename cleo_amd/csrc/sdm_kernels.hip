@@ -200,11 +200,13 @@ __device__ __forceinline__ PairOut coll_pair_eval(const CollArgs& a, uint32_t g,
   const double scale_p = n * (n - 1.0) / (2.0 * npairs);
   const double VOLUME = a.vol[g] * dimless::VOL0;
   PairOut o;
-  o.A = DropAttrs{a.src.xi[ka], a.src.r[ka], a.src.msol[ka]};
-  o.B = DropAttrs{a.src.xi[kb], a.src.r[kb], a.src.msol[kb]};
-  const bool a_first = !(o.A.xi < o.B.xi);  // assign_drops (collisions.hpp:94-101)
-  DropAttrs& d1 = a_first ? o.A : o.B;
-  DropAttrs& d2 = a_first ? o.B : o.A;
+  const DropAttrs A{a.src.xi[ka], a.src.r[ka], a.src.msol[ka]};
+  const DropAttrs B{a.src.xi[kb], a.src.r[kb], a.src.msol[kb]};
+  const bool a_first = !(A.xi < B.xi);  // assign_drops (collisions.hpp:94-101)
+  // d1 is the drop with the larger multiplicity; selected field by field (a reference picked by a
+  // lane-varying flag would put both drops in scratch memory)
+  DropAttrs d1{a_first ? A.xi : B.xi, a_first ? A.r : B.r, a_first ? A.msol : B.msol};
+  DropAttrs d2{a_first ? B.xi : A.xi, a_first ? B.r : A.r, a_first ? B.msol : A.msol};
   const double prob_jk = pair_probability(a.k, d1.r, d2.r, VOLUME);
   const double prob = scale_p * static_cast<double>(d1.xi) * prob_jk;
   double phi_coll, phi_out;
@@ -223,6 +225,8 @@ __device__ __forceinline__ PairOut coll_pair_eval(const CollArgs& a, uint32_t g,
   o.enacted = (res >> 1) & 1u;
   o.nullA = (res & 1u) && a_first;
   o.nullB = (res & 1u) && !a_first;
+  o.A = DropAttrs{a_first ? d1.xi : d2.xi, a_first ? d1.r : d2.r, a_first ? d1.msol : d2.msol};
+  o.B = DropAttrs{a_first ? d2.xi : d1.xi, a_first ? d2.r : d1.r, a_first ? d2.msol : d1.msol};
   return o;
 }
 // drop of `src` at ks written to `dst` at kd with attributes D (other fields carried)
@@ -379,7 +383,7 @@ __device__ __forceinline__ void coll_gbx(const CollArgs& a, int lane, uint32_t g
   acc[2] += nulls;
 }
 
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_collisions(CollArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_collisions(CollArgs a) {
   extern __shared__ uint32_t lds_dyn[];
   const int lane = lane_id();
   const int w = static_cast<int>(threadIdx.x >> 6);
@@ -439,7 +443,7 @@ __device__ __forceinline__ uint32_t tile_owner(const uint32_t* pre, uint32_t f) 
   return lo;
 }
 
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_coll_tiles(CollArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_coll_tiles(CollArgs a) {
   extern __shared__ uint32_t lds_dyn[];
   const int lane = lane_id();
   CollTileLds& L = reinterpret_cast<CollTileLds*>(lds_dyn)[threadIdx.x >> 6];
@@ -486,12 +490,23 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_coll_tiles(CollArgs a
     L.n[lane] = n;
     L.nulls[lane] = 0u;
     wave_sync();
-    for (uint32_t f = lane; f < t_hi - t_lo; f += kWave) {
+    // chunks start on multiples of 64 drops, so that every store instruction covers whole cache
+    // lines (a partly written line costs a read-modify-write in HBM); only the tile's two edge
+    // chunks are partial
+    const uint32_t a_lo = t_lo & ~static_cast<uint32_t>(kWave - 1);
+    for (uint32_t pos = a_lo + lane; pos < t_hi; pos += kWave) {
+      if (pos < t_lo) continue;
+      const uint32_t f = pos - t_lo;
       const uint32_t i = tile_owner(L.pre, f);
       const uint32_t li = L.loff[i];
       if (li == kBigGbx) continue;
       const uint32_t slot = f - L.pre[i], ni = L.n[i], bi = t_lo + L.pre[i];
       const uint8_t* pi = pb + li;
+      // what this lane writes: its side of a pair, or a carried drop; one store sequence for all
+      // lanes so that each store instruction covers a contiguous run
+      uint32_t ks;
+      DropAttrs D;
+      bool isnull = false;
       if (slot < 2u * (ni / 2u)) {
         const uint32_t jj = slot >> 1, side = slot & 1u;
         const uint32_t gi = tile * kWave + i, ka = bi + pi[2 * jj], kb = bi + pi[2 * jj + 1];
@@ -500,13 +515,18 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_coll_tiles(CollArgs a
           if (o.err) set_error(a.ctl, o.err, gi, bi + 2 * jj);
           events += o.enacted ? 1u : 0u;
           if (o.nullA || o.nullB) atomicAdd(&L.nulls[i], 1u);
-          coll_write(a, bi + slot, ka, o.A, o.nullA);
-        } else {
-          coll_write(a, bi + slot, kb, o.B, o.nullB);
         }
+        // field by field: selecting a whole struct by a lane-varying flag puts it in scratch
+        ks = side ? kb : ka;
+        D.xi = side ? o.B.xi : o.A.xi;
+        D.r = side ? o.B.r : o.A.r;
+        D.msol = side ? o.B.msol : o.A.msol;
+        isnull = side ? o.nullB : o.nullA;
       } else {
-        coll_carry(a, bi + (slot < ni ? static_cast<uint32_t>(pi[slot]) : slot), bi + slot);
+        ks = bi + (slot < ni ? static_cast<uint32_t>(pi[slot]) : slot);
+        D = DropAttrs{a.src.xi[ks], a.src.r[ks], a.src.msol[ks]};
       }
+      coll_write(a, bi + slot, ks, D, isnull);
     }
     wave_sync();
     const uint32_t mynulls = L.nulls[lane];
@@ -754,7 +774,7 @@ __device__ __forceinline__ uint32_t motion_gbx(const MotionArgs& a, uint32_t g, 
     a.cnt[28ull * neighbour_of(a.grid, g, lane) + lane] = mycount;
   return nfull;
 }
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_motion(MotionArgs a) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_motion(MotionArgs a) {
   const int lane = lane_id();
   uint32_t moved = 0;
   for (uint32_t g = wave_index(); g < a.grid.ngbx; g += wave_count()) moved += motion_gbx(a, g, lane);
@@ -1466,13 +1486,12 @@ struct CondStats {
   uint32_t solved, iters, subs, restarts;
 };
 __device__ __forceinline__ bool cond_drop(const CondFlatArgs& a, const NewtonCfg& cfg, uint32_t g, uint64_t k,
-                                          CondStats& st) {
+                                          double warm, CondStats& st) {
   const double s_ratio = a.gs[2 * g], ffactor = a.gs[2 * g + 1];
   const double temp = a.st.temp[g];
   const double r = a.s.r[k], msol = a.s.msol[k];
   double akoh, bkoh;
   kohler_factors(msol, temp, a.sol, akoh, bkoh);
-  const double warm = a.warm ? a.warm[k] : __longlong_as_double(0x7FF8000000000000ll);
   SolveOut o{0, false, false, false, false};
   const double newr = solve_condensation(cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
   if (o.deferred) return false;
@@ -1504,6 +1523,48 @@ __device__ __forceinline__ void put_cond_stats(const StatSlots& sl, CondStats st
   }
 }
 
+// Learned guess z = r^2 exp(2 y) of the wave's 64 drops (NaN where `valid` is false), the network
+// evaluated by the whole wave (every lane must call it); the same arithmetic as k_mlp_guess.
+__device__ __forceinline__ double inline_guess(const CondFlatArgs& a, InferLds& L, bool valid, uint32_t g,
+                                               uint64_t k) {
+  float f[kIn] = {0, 0, 0, 0, 0, 0};
+  double r = 0.0;
+  if (valid) {
+    r = a.s.r[k];
+    drop_features(r, a.s.msol[k], a.gs[2 * g], a.st.temp[g], a.st.press[g], a.cfg.delt, f);
+  }
+  load_normalised(L, a.W.p, f, valid);
+  const float yn = mlp_infer_wave(a.W, L);
+  wave_sync();  // L is reused by the next chunk
+  double z = __longlong_as_double(0x7FF8000000000000ll);
+  if (valid) {
+    const double y = static_cast<double>(yn) * a.W.p[P_YSTD] + a.W.p[P_YMEAN];
+    z = r * r * exp(2.0 * y);
+  }
+  if (a.guess_out && valid) a.guess_out[k] = z;
+  return z;
+}
+
+// the learned guess of every in-domain drop into a.warm_buf (NaN for inactive drops), 64 drops per
+// network pass; it runs after k_cond_prep, whose saturation ratio it reads
+__global__ void __launch_bounds__(256) k_cond_guess(CondFlatArgs a) {
+  __shared__ InferLds Lall[4];
+  InferLds& L = Lall[threadIdx.x >> 6];
+  const uint32_t ndom = a.off[a.ngbx];
+  for (uint64_t k0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x; k0 < ndom;
+       k0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t k = k0 + threadIdx.x;
+    bool active = false;
+    uint32_t g = 0;
+    if (k < ndom) {
+      g = a.s.gbx[k];
+      active = g < a.ngbx && k < a.off[g] + a.nact[g];
+    }
+    const double z = inline_guess(a, L, active, g, k);
+    if (k < ndom) a.warm_buf[k] = z;
+  }
+}
+
 // per active drop, one thread each.  Drops whose solve outlives cfg.budget iterations (a few per
 // cent; their lanes would hold the other 63 of the wave) go to this wave's segment of the
 // deferred list, up to `dcap` of them, and are solved by k_cond_deferred.
@@ -1518,11 +1579,16 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_drops(CondFlatAr
   for (uint64_t k0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x; k0 < ndom;
        k0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     const uint64_t k = k0 + threadIdx.x;
-    bool deferred = false;
+    bool active = false;
+    uint32_t g = 0;
     if (k < ndom) {
-      const uint32_t g = a.s.gbx[k];
-      if (g < a.ngbx && k < a.off[g] + a.nact[g]) deferred = !cond_drop(a, a.cfg, g, k, st);
+      g = a.s.gbx[k];
+      active = g < a.ngbx && k < a.off[g] + a.nact[g];
     }
+    double warm = __longlong_as_double(0x7FF8000000000000ll);
+    if (active && a.warm) warm = a.warm[k];
+    bool deferred = false;
+    if (active) deferred = !cond_drop(a, a.cfg, g, k, warm, st);
     const uint64_t m = __ballot(deferred);
     if (m) {
       const uint32_t idx = ndef + static_cast<uint32_t>(__popcll(m & lanemask_lt()));
@@ -1530,7 +1596,7 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_drops(CondFlatAr
         if (idx < a.dcap)
           mylist[idx] = static_cast<uint32_t>(k);
         else
-          cond_drop(a, nobudget, a.s.gbx[k], k, st);  // the segment is full: solve it here
+          cond_drop(a, nobudget, g, k, warm, st);  // the segment is full: solve it here
       }
       ndef = min(a.dcap, ndef + static_cast<uint32_t>(__popcll(m)));
     }
@@ -1549,7 +1615,8 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_deferred(CondFla
   const uint32_t* mylist = a.dlist + static_cast<uint64_t>(wave_index()) * a.dcap;
   for (uint32_t i = lane; i < n; i += kWave) {
     const uint32_t k = mylist[i];
-    cond_drop(a, nobudget, a.s.gbx[k], k, st);
+    const double warm = a.warm ? a.warm[k] : __longlong_as_double(0x7FF8000000000000ll);
+    cond_drop(a, nobudget, a.s.gbx[k], k, warm, st);
   }
   put_cond_stats(a.stat, st);
 }
@@ -1696,7 +1763,13 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_cond_queue(CondFlatAr
         z0 = z = zn;
         g0 = gz = g1;
         dg0 = dgz = dg1;
-        phase = PH_FIRST;
+        if (mode == MODE_WARM) {  // from the learned guess: iterate until converged at once
+          phase = PH_UNTIL;
+          niter = 1;
+          segfail = niter > limit;
+        } else {
+          phase = PH_FIRST;
+        }
       } else {
         ++iters;
         const bool conv = fabs(g1 - gz) < cfg.rtol * fabs(g1) + cfg.atol;
@@ -1880,6 +1953,10 @@ void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& args) {
     a.dcap = budget ? static_cast<uint32_t>(per_wave / 4 + 16) : 0u;
     a.dlist = c->dlist;
     a.dcount = budget ? c->dcount : nullptr;
+    if (a.mlp_inline) {  // the learned guesses, from the saturation ratios k_cond_prep left
+      hipLaunchKernelGGL(k_cond_guess, dim3(drop_blocks(c->ntot, gw ? gw : 32768)), dim3(256), 0, c->stream, a);
+      a.warm = a.warm_buf;
+    }
     kev_mark(c, 0);
     hipLaunchKernelGGL(k_cond_drops, dim3(nb), dim3(256), 0, c->stream, a);
     if (budget) {

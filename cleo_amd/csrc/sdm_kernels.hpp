@@ -1,6 +1,7 @@
 // sdm_kernels.hpp -- kernel argument blocks and host-side launchers of sdm_kernels.hip.
 #pragma once
 #include "ctx.hpp"
+#include "mlp_device.hpp"
 
 namespace cleo {
 
@@ -115,6 +116,12 @@ struct CondFlatArgs {
   uint32_t* dlist;   // deferred drops, `dcap` per wave of k_cond_drops
   uint32_t* dcount;  // deferred drops per wave (null: no deferral)
   uint32_t dcap;
+  // mlp_inline: the learned guesses are evaluated by k_cond_guess into warm_buf inside the launch
+  // sequence (after the saturation ratios exist) and read from there
+  int32_t mlp_inline;
+  MLPWeights W;
+  double* warm_buf;
+  double* guess_out;  // non-null: the guesses are recorded here too (array position order)
 };
 
 // launchers (sdm_kernels.hip); all enqueue on ctx->stream

@@ -320,8 +320,9 @@ static inline double critical_timestep(const ODEConsts& c) {
   return 2.5 * c.ffactor_fv / c.akoh * cuberoot;
 }
 /* solve_condensation (impliciteuler.cpp:35-54); warm = NaN for the plain path.
-   Learned-guess variant: Newton from `warm` through the unchanged two-phase loop with
-   `warm_limit` as the until-converged limit; on failure, restart from the plain guess. */
+   Learned-guess variant (no reference counterpart): the reference's Newton update and
+   convergence test iterated from `warm` until converged, at most `warm_limit` iterations; if it
+   has not converged by then, the reference path from the plain guess. */
 static inline double solve_condensation(const NewtonCfg& cfg, double s_ratio, double akoh,
                                         double bkoh, double ffactor, double rprev, double warm,
                                         uint64_t warm_limit, NewtonCount& cnt, const ErrSink& e) {
@@ -333,18 +334,11 @@ static inline double solve_condensation(const NewtonCfg& cfg, double s_ratio, do
   double rsqrd = 0.0;
   if (ucrit1 || ucrit2) {
     if (warm == warm) { /* learned guess */
-      double z = warm;
-      bool conv = false;
-      for (int it = 0; it < 2; ++it) conv = iterate(cfg, c, cfg.delt, rprev, z, cnt);
-      if (conv) {
-        rsqrd = z;
-      } else {
-        bool ok = true;
-        rsqrd = until_converged(cfg, c, warm_limit, cfg.delt, rprev, warm, cnt, ok);
-        if (!ok) {
-          cnt.restarted = true;
-          rsqrd = integrate(cfg, c, cfg.delt, rprev, ziter, cnt, e);
-        }
+      bool ok = true;
+      rsqrd = until_converged(cfg, c, warm_limit, cfg.delt, rprev, warm, cnt, ok);
+      if (!ok) {
+        cnt.restarted = true;
+        rsqrd = integrate(cfg, c, cfg.delt, rprev, ziter, cnt, e);
       }
     } else {
       rsqrd = integrate(cfg, c, cfg.delt, rprev, ziter, cnt, e);

@@ -83,14 +83,14 @@ def test_mlp_gradient_vs_oracle(gpu):
     assert rel < 5e-2 and cos > 0.998
 
 
-@pytest.mark.parametrize("variant", [0, 2])
-def test_training_and_warm_start(gpu, oracle, variant):
+@pytest.mark.parametrize("variant,defer", [(0, 0), (1, 0), (1, 3), (2, 0)])
+def test_training_and_warm_start(gpu, oracle, variant, defer):
     _say(f"case: train on solver samples, warm-started condensation vs oracle with the same guesses, "
-         f"condensation variant {variant}")
+         f"condensation variant {variant} (1: guess evaluated in the drop kernel), defer {defer}")
     pop = P.condensation(SEED, ndims=(8, 8, 8), mean=40)
     pop.state["qvap"] *= 1.005
     d = make_domain(gpu, pop)
-    t = abi.Tiles(1, 1, 1, variant, 0, 0, 0, 0, 0)
+    t = abi.Tiles(1, 1, 1, variant, 0, 0, 0, 0, 0, defer)
     assert gpu.lib.cleo_set_tiles(d.h, abi.ctypes_byref(t)) == 0
     cp = cond_params()
     n = pop.n
