@@ -51,12 +51,15 @@ struct GuessArgs {
 // learned guess z for every active in-domain drop (NaN elsewhere), 64 drops per network pass;
 // blocks of four waves loop over the drops
 __global__ void __launch_bounds__(256) k_mlp_guess(GuessArgs a) {
-  __shared__ InferLds Lall[4];
-  InferLds& L = Lall[threadIdx.x >> 6];
+  __shared__ InferConst K;
+  infer_const_load(K, a.W.p);
+  __syncthreads();
+  const InferFrags F = infer_frags(a.W);
   const uint32_t ndom = a.off[a.ngbx];
-  for (uint64_t k0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x; k0 < ndom;
+  // wave-uniform loop: every lane of a wave runs the network on each chunk of 64 drops
+  for (uint64_t k0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u); k0 < ndom;
        k0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const uint64_t k = k0 + threadIdx.x;
+    const uint64_t k = k0 + lane_id();
     bool valid = false;
     float f[kIn] = {0, 0, 0, 0, 0, 0};
     double r = 0.0;
@@ -71,17 +74,10 @@ __global__ void __launch_bounds__(256) k_mlp_guess(GuessArgs a) {
         drop_features(r, a.s.msol[k], s_ratio, temp, press, a.delt, f);
       }
     }
-    load_normalised(L, a.W.p, f, valid);
-    const float yn = mlp_infer_wave(a.W, L);
-    wave_sync();  // L is reused by the next chunk
-    if (k < ndom) {
-      double z = __longlong_as_double(0x7FF8000000000000ll);
-      if (valid) {
-        const double y = static_cast<double>(yn) * a.W.p[P_YSTD] + a.W.p[P_YMEAN];
-        z = r * r * exp(2.0 * y);
-      }
-      a.warm[k] = z;
-    }
+    float xn[kIn];
+    infer_normalise(K, f, valid, xn);
+    const float yn = mlp_infer(F, K, xn);
+    if (k < ndom) a.warm[k] = valid ? learned_z(K, r, yn) : __longlong_as_double(0x7FF8000000000000ll);
   }
 }
 
@@ -324,19 +320,24 @@ struct ForwardArgs {
   uint64_t n;
   MLPWeights W;
 };
-// normalised-output forward pass (for checks): y[k] = network(x[k]) in target units
-__global__ void __launch_bounds__(64) k_mlp_forward(ForwardArgs a) {
-  __shared__ WaveLds L;
-  const int lane = lane_id();
-  const uint64_t k = static_cast<uint64_t>(blockIdx.x) * 64 + lane;
-  float f[kIn] = {0, 0, 0, 0, 0, 0};
-  const bool valid = k < a.n;
-  if (valid)
-    for (int j = 0; j < kIn; ++j) f[j] = a.x[k * kInPad + j];
-  load_normalised(L, a.W.p, f, valid);
-  f32x16 z1[2][2], z2[2][2];
-  const float yn = mlp_forward_wave(a.W, L, z1, z2);
-  if (valid) a.y[k] = yn * a.W.p[P_YSTD] + a.W.p[P_YMEAN];
+// forward pass (the inference pass of the guess kernels): y[k] = network(x[k]) in target units
+__global__ void __launch_bounds__(256) k_mlp_forward(ForwardArgs a) {
+  __shared__ InferConst K;
+  infer_const_load(K, a.W.p);
+  __syncthreads();
+  const InferFrags F = infer_frags(a.W);
+  for (uint64_t k0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u); k0 < a.n;
+       k0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t k = k0 + lane_id();
+    float f[kIn] = {0, 0, 0, 0, 0, 0};
+    const bool valid = k < a.n;
+    if (valid)
+      for (int j = 0; j < kIn; ++j) f[j] = a.x[k * kInPad + j];
+    float xn[kIn];
+    infer_normalise(K, f, valid, xn);
+    const float yn = mlp_infer(F, K, xn);
+    if (valid) a.y[k] = yn * K.ystd + K.ymean;
+  }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -559,7 +560,8 @@ int cleo_mlp_forward(cleo_ctx* c, const float* x, float* y, uint64_t n) {
   if (rc != CLEO_OK) return rc;
   if (n == 0) return CLEO_OK;
   ForwardArgs a{x, y, n, c->mlp->weights()};
-  hipLaunchKernelGGL(k_mlp_forward, dim3(static_cast<unsigned>((n + 63) / 64)), dim3(64), 0, c->stream, a);
+  const uint64_t nb = std::min<uint64_t>(8192, (n + 255) / 256);
+  hipLaunchKernelGGL(k_mlp_forward, dim3(static_cast<unsigned>(nb)), dim3(256), 0, c->stream, a);
   MLP_TRY(c, hipGetLastError());
   MLP_TRY(c, hipStreamSynchronize(c->stream));
   return CLEO_OK;

@@ -1523,45 +1523,39 @@ __device__ __forceinline__ void put_cond_stats(const StatSlots& sl, CondStats st
   }
 }
 
-// Learned guess z = r^2 exp(2 y) of the wave's 64 drops (NaN where `valid` is false), the network
-// evaluated by the whole wave (every lane must call it); the same arithmetic as k_mlp_guess.
-__device__ __forceinline__ double inline_guess(const CondFlatArgs& a, InferLds& L, bool valid, uint32_t g,
-                                               uint64_t k) {
-  float f[kIn] = {0, 0, 0, 0, 0, 0};
-  double r = 0.0;
-  if (valid) {
-    r = a.s.r[k];
-    drop_features(r, a.s.msol[k], a.gs[2 * g], a.st.temp[g], a.st.press[g], a.cfg.delt, f);
-  }
-  load_normalised(L, a.W.p, f, valid);
-  const float yn = mlp_infer_wave(a.W, L);
-  wave_sync();  // L is reused by the next chunk
-  double z = __longlong_as_double(0x7FF8000000000000ll);
-  if (valid) {
-    const double y = static_cast<double>(yn) * a.W.p[P_YSTD] + a.W.p[P_YMEAN];
-    z = r * r * exp(2.0 * y);
-  }
-  if (a.guess_out && valid) a.guess_out[k] = z;
-  return z;
-}
-
-// the learned guess of every in-domain drop into a.warm_buf (NaN for inactive drops), 64 drops per
-// network pass; it runs after k_cond_prep, whose saturation ratio it reads
+// the learned guess z = r^2 exp(2 y) of every in-domain drop into a.warm_buf (NaN for inactive
+// drops; also into a.guess_out when recording), 64 drops per network pass; it runs after
+// k_cond_prep, whose saturation ratio it reads
 __global__ void __launch_bounds__(256) k_cond_guess(CondFlatArgs a) {
-  __shared__ InferLds Lall[4];
-  InferLds& L = Lall[threadIdx.x >> 6];
+  __shared__ InferConst K;
+  infer_const_load(K, a.W.p);
+  __syncthreads();
+  const InferFrags F = infer_frags(a.W);
   const uint32_t ndom = a.off[a.ngbx];
-  for (uint64_t k0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x; k0 < ndom;
+  // wave-uniform loop: every lane of a wave runs the network on each chunk of 64 drops
+  for (uint64_t k0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u); k0 < ndom;
        k0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const uint64_t k = k0 + threadIdx.x;
-    bool active = false;
+    const uint64_t k = k0 + lane_id();
+    bool valid = false;
     uint32_t g = 0;
+    float f[kIn] = {0, 0, 0, 0, 0, 0};
+    double r = 0.0;
     if (k < ndom) {
       g = a.s.gbx[k];
-      active = g < a.ngbx && k < a.off[g] + a.nact[g];
+      valid = g < a.ngbx && k < a.off[g] + a.nact[g];
+      if (valid) {
+        r = a.s.r[k];
+        drop_features(r, a.s.msol[k], a.gs[2 * g], a.st.temp[g], a.st.press[g], a.cfg.delt, f);
+      }
     }
-    const double z = inline_guess(a, L, active, g, k);
-    if (k < ndom) a.warm_buf[k] = z;
+    float xn[kIn];
+    infer_normalise(K, f, valid, xn);
+    const float yn = mlp_infer(F, K, xn);
+    if (k < ndom) {
+      const double z = valid ? learned_z(K, r, yn) : __longlong_as_double(0x7FF8000000000000ll);
+      a.warm_buf[k] = z;
+      if (a.guess_out && valid) a.guess_out[k] = z;
+    }
   }
 }
 
