@@ -690,12 +690,20 @@ struct Moved {
   uint32_t idx;
   int dir;
 };
-__device__ __forceinline__ Moved move_drop(const MotionArgs& a, uint32_t g, const GbxGeom& G, uint64_t k, bool report) {
+// what the motion step reads of drop k
+struct DropPos {
+  double c3, c1, c2, r;
+};
+__device__ __forceinline__ DropPos drop_pos(const MotionArgs& a, uint64_t k) {
+  return DropPos{a.s.c3[k], a.s.c1[k], a.s.c2[k], a.s.r[k]};
+}
+__device__ __forceinline__ Moved move_drop(const MotionArgs& a, uint32_t g, const GbxGeom& G, const DropPos& P,
+                                           uint64_t k, bool report) {
   const uint32_t nz = a.grid.nz, nx = a.grid.nx, ny = a.grid.ny;
   const double delt = a.delt;
-  const double c3_0 = a.s.c3[k], c1_0 = a.s.c1[k], c2_0 = a.s.c2[k];
+  const double c3_0 = P.c3, c1_0 = P.c1, c2_0 = P.c2;
   double coord3 = c3_0;
-  const double terminal = terminal_velocity(a.tv, a.s.r[k]);
+  const double terminal = terminal_velocity(a.tv, P.r);
   double vel3 = interpolation(G.zlo, G.zhi, G.wlo, G.whi, coord3);
   vel3 -= terminal;
   coord3 += vel3 * delt;
@@ -735,6 +743,9 @@ __device__ __forceinline__ Moved move_drop(const MotionArgs& a, uint32_t g, cons
     M.dir = (dz + 1) + 3 * (dx + 1) + 9 * (dy + 1);
   }
   return M;
+}
+__device__ __forceinline__ Moved move_drop(const MotionArgs& a, uint32_t g, const GbxGeom& G, uint64_t k, bool report) {
+  return move_drop(a, g, G, drop_pos(a, k), k, report);
 }
 
 // MoveSupersInGridboxesFunctor (movesupersindomain.hpp:47-93) with PredCorrMotion over
@@ -1191,7 +1202,12 @@ static inline unsigned nblocks(uint64_t n, uint64_t per) {
 }
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v);
 template <bool kWrite>
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_motion_drops(MotionArgs a);
+static void launch_motion_drops(cleo_ctx* c, const MotionArgs& a, unsigned nb);
+// blocks of the motion drop loop (two drops per thread), capped at `grid_waves` waves when non-zero
+static inline unsigned motion_blocks(uint64_t n, uint32_t grid_waves) {
+  const unsigned b = nblocks(n, 512);
+  return grid_waves ? std::min<unsigned>(b, std::max<unsigned>(1u, grid_waves / 4)) : b;
+}
 
 // events around the main kernel of a stage (stage timing), recorded only while kev_on is set
 static inline void kev_mark(cleo_ctx* c, int i) {
@@ -1394,9 +1410,9 @@ void motion_sort_fused(cleo_ctx* c, const MotionArgs& args, const SoA& dst, int 
   const uint32_t ngbx = c->grid.ngbx;
   MotionArgs a = args;
   hipLaunchKernelGGL(k_fill_u32, dim3(stride_blocks(28ull * ngbx)), dim3(256), 0, c->stream, a.cnt, 28ull * ngbx, 0u);
-  const unsigned nb = drop_blocks(c->ntot, c->tiles.motion_grid_waves);
+  const unsigned nb = motion_blocks(c->ntot, c->tiles.motion_grid_waves);
   a.stat = stat_slots(c, 4ull * nb);
-  hipLaunchKernelGGL(k_motion_drops<false>, dim3(nb), dim3(256), 0, c->stream, a);
+  launch_motion_drops<false>(c, a, nb);
   launch_stats_reduce(c, 4u * nb, ST_MOTION_DROPS, 1);
   NsortArgs n = nsort_args(c, a.s, dst, nullptr);
   nsort_tables(c, n, nullptr);
@@ -1881,45 +1897,83 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
     p[i] = v;
 }
 
-// MoveSupersInGridboxesFunctor per drop; direction counts aggregated per wave, then atomics
-// one drop: move it and return its count-table key: 28*d + step for destination d, 28*g + 27 when
-// it leaves the domain (`write`: the new coordinates and gridbox go back to the store)
-__device__ __forceinline__ uint32_t motion_drop(const MotionArgs& a, uint64_t k, bool write) {
-  const uint32_t g = a.s.gbx[k];
-  const GbxGeom G = gbx_geom(a, g);
-  const Moved M = move_drop(a, g, G, k, true);
-  if (write) {
-    a.s.c3[k] = M.c3;
-    a.s.c1[k] = M.c1;
-    a.s.c2[k] = M.c2;
-    a.s.gbx[k] = M.idx;
-  }
+// count-table key of a moved drop: 28*d + step for destination d, 28*g + 27 when it left the domain
+__device__ __forceinline__ uint32_t motion_key(const Moved& M, uint32_t g) {
   return M.idx != kOOB ? 28u * M.idx + static_cast<uint32_t>(M.dir) : 28u * g + 27u;
 }
-// thread per drop: the motion step and the destination-major counts (without kWrite this is the
-// counting pass of the fused motion + sort: nothing of the drops is stored)
-template <bool kWrite>
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_motion_drops(MotionArgs a) {
+// adds the wave's drops per distinct key to the count table (every lane of the wave calls it)
+__device__ __forceinline__ void count_keys(uint32_t* cnt, uint32_t key, bool act, int lane) {
+  uint64_t mask = __ballot(act);
+  while (mask) {
+    const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+    const uint32_t v = readlane_u32(key, leader);
+    const uint64_t m = __ballot(act && key == v);
+    if (lane == leader) atomicAdd(&cnt[v], static_cast<uint32_t>(__popcll(m)));
+    mask &= ~m;
+  }
+}
+// largest coordinate-bound table (zh, xh, yh together) that the motion kernels stage in LDS
+constexpr uint32_t kMotionLdsBounds = 2048;
+
+// MoveSupersInGridboxesFunctor per drop, two drops per thread and iteration with every load of
+// both issued before the arithmetic; the destination-major counts are aggregated per wave, then
+// added with atomics.  kLds: the coordinate bounds are read from an LDS copy (dynamic shared
+// memory of (nz + nx + ny + 3) doubles).  Without kWrite this is the counting pass of the fused
+// motion + sort: nothing of the drops is stored.
+template <bool kWrite, bool kLds>
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_motion_drops(MotionArgs a) {
+  extern __shared__ double bounds_lds[];
   const int lane = lane_id();
+  MotionArgs b = a;
+  if constexpr (kLds) {
+    const uint32_t nzh = a.grid.nz + 1, nxh = a.grid.nx + 1, nyh = a.grid.ny + 1;
+    for (uint32_t i = threadIdx.x; i < nzh + nxh + nyh; i += blockDim.x)
+      bounds_lds[i] = i < nzh ? a.grid.zh[i] : (i < nzh + nxh ? a.grid.xh[i - nzh] : a.grid.yh[i - nzh - nxh]);
+    __syncthreads();
+    b.grid.zh = bounds_lds;
+    b.grid.xh = bounds_lds + nzh;
+    b.grid.yh = bounds_lds + nzh + nxh;
+  }
   const uint32_t ndom = a.off[a.grid.ngbx];
   uint32_t nmoved = 0;
-  // block-uniform chunks keep every wave in the loop together for the ballots below
-  for (uint64_t k0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x; k0 < ndom;
-       k0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const uint64_t k = k0 + threadIdx.x;
-    const bool act = k < ndom;
-    const uint32_t key = act ? motion_drop(a, k, kWrite) : 0u;
-    uint64_t mask = __ballot(act);
-    nmoved += static_cast<uint32_t>(__popcll(mask));
-    while (mask) {
-      const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-      const uint32_t v = readlane_u32(key, leader);
-      const uint64_t m = __ballot(act && key == v);
-      if (lane == leader) atomicAdd(&a.cnt[v], static_cast<uint32_t>(__popcll(m)));
-      mask &= ~m;
+  // block-uniform chunks of 512 drops keep every wave in the loop together for the ballots
+  for (uint64_t k0 = 2ull * blockIdx.x * blockDim.x; k0 < ndom; k0 += 2ull * gridDim.x * blockDim.x) {
+    const uint64_t ka = k0 + threadIdx.x, kb = ka + blockDim.x;
+    const bool acta = ka < ndom, actb = kb < ndom;
+    // past the end a lane reads drop k0 (in the domain) and discards the result
+    const uint64_t la = acta ? ka : k0, lb = actb ? kb : k0;
+    const uint32_t ga = a.s.gbx[la], gb = a.s.gbx[lb];
+    const DropPos pa = drop_pos(a, la), pb = drop_pos(a, lb);
+    const GbxGeom Ga = gbx_geom(b, ga), Gb = gbx_geom(b, gb);
+    const Moved Ma = move_drop(b, ga, Ga, pa, la, acta);
+    const Moved Mb = move_drop(b, gb, Gb, pb, lb, actb);
+    if (kWrite && acta) {
+      a.s.c3[ka] = Ma.c3;
+      a.s.c1[ka] = Ma.c1;
+      a.s.c2[ka] = Ma.c2;
+      a.s.gbx[ka] = Ma.idx;
     }
+    if (kWrite && actb) {
+      a.s.c3[kb] = Mb.c3;
+      a.s.c1[kb] = Mb.c1;
+      a.s.c2[kb] = Mb.c2;
+      a.s.gbx[kb] = Mb.idx;
+    }
+    nmoved += static_cast<uint32_t>(__popcll(__ballot(acta))) + static_cast<uint32_t>(__popcll(__ballot(actb)));
+    count_keys(a.cnt, motion_key(Ma, ga), acta, lane);
+    count_keys(a.cnt, motion_key(Mb, gb), actb, lane);
   }
   if (lane == 0) put_stat(a.stat, 0, wave_index(), nmoved);
+}
+
+// launches k_motion_drops with the bounds in LDS when they fit
+template <bool kWrite>
+static void launch_motion_drops(cleo_ctx* c, const MotionArgs& a, unsigned nb) {
+  const uint32_t nb_ = a.grid.nz + a.grid.nx + a.grid.ny + 3;
+  if (nb_ <= kMotionLdsBounds)
+    hipLaunchKernelGGL((k_motion_drops<kWrite, true>), dim3(nb), dim3(256), nb_ * sizeof(double), c->stream, a);
+  else
+    hipLaunchKernelGGL((k_motion_drops<kWrite, false>), dim3(nb), dim3(256), 0, c->stream, a);
 }
 
 void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& args) {
@@ -1966,12 +2020,12 @@ void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& args) {
 void launch_motion_flat(cleo_ctx* c, const MotionArgs& args) {
   if (args.grid.ngbx == 0) return;
   MotionArgs a = args;
-  const unsigned nb = drop_blocks(c->ntot, c->tiles.motion_grid_waves);
-  a.stat = stat_slots(c, 4ull * nb);  // every wave of k_motion_flat writes its slot
+  const unsigned nb = motion_blocks(c->ntot, c->tiles.motion_grid_waves);
+  a.stat = stat_slots(c, 4ull * nb);  // every wave of k_motion_drops writes its slot
   hipLaunchKernelGGL(k_fill_u32, dim3(stride_blocks(28ull * a.grid.ngbx)), dim3(256), 0, c->stream, a.cnt,
                      28ull * a.grid.ngbx, 0u);
   kev_mark(c, 0);
-  hipLaunchKernelGGL(k_motion_drops<true>, dim3(nb), dim3(256), 0, c->stream, a);
+  launch_motion_drops<true>(c, a, nb);
   kev_mark(c, 1);
   launch_stats_reduce(c, 4u * nb, ST_MOTION_DROPS, 1);
 }
