@@ -344,6 +344,7 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   // deferred list: waves of a drop launch times (a quarter of their drops + 16)
   HIP_TRY(c, dalloc(&c->dcount, c->sp_slots));
   HIP_TRY(c, dalloc(&c->dlist, maxn / 4 + 20 * c->sp_slots + 64));
+  HIP_TRY(c, dalloc(&c->perm, std::max<uint64_t>(maxn, 1)));
   c->big_scratch_n = 2 * maxn;
   HIP_TRY(c, dalloc(&c->ctl, 1));
   HIP_TRY(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_ctl), sizeof(DevCtl), hipHostMallocDefault));
@@ -372,7 +373,8 @@ int cleo_destroy(cleo_ctx* c) {
                   static_cast<void*>(c->niters), static_cast<void*>(c->guess), static_cast<void*>(c->rs_key[0]),
                   static_cast<void*>(c->rs_key[1]), static_cast<void*>(c->rs_idx[0]), static_cast<void*>(c->rs_idx[1]),
                   static_cast<void*>(c->rs_cnt), static_cast<void*>(c->gs), static_cast<void*>(c->dm),
-                  static_cast<void*>(c->sp), static_cast<void*>(c->dlist), static_cast<void*>(c->dcount)})
+                  static_cast<void*>(c->sp), static_cast<void*>(c->dlist), static_cast<void*>(c->dcount),
+                  static_cast<void*>(c->perm)})
     if (p) hipFree(p);
   if (c->h_ctl) hipHostFree(c->h_ctl);
   for (hipEvent_t& e : c->ev)

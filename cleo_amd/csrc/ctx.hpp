@@ -106,6 +106,8 @@ struct cleo_ctx {
   // deferred-drop list of the thread-per-drop condensation (segments per wave) and their counts
   uint32_t* dlist = nullptr;
   uint32_t* dcount = nullptr;
+  // neighbour sort: source position of every destination position (maxn)
+  uint32_t* perm = nullptr;
   uint64_t sp_slots = 0;  // slots per counter: max(ngbx, waves of a thread-per-drop launch)
   // per-drop Newton iteration record
   bool rec_niters = false;

@@ -822,86 +822,122 @@ __global__ void k_nsort_tot(NsortArgs a) {
 
 // Stable scatter (sortsupers.hpp:63-76,167-193 under Serial): drop p of source gridbox g with
 // key d lands at  base[d] + (drops reaching d from gridboxes < g) + (earlier drops of g with key d);
-// the middle term is row d of the count table after k_nsort_tot, at the step g -> d.
-__device__ __forceinline__ void nsort_scatter_gbx(const NsortArgs& a, uint32_t g, int lane) {
+// the middle term is row d of the count table after k_nsort_tot, at the step g -> d.  The 27 run
+// starts of g (one per step) and the start of its out-of-domain run are loaded by lanes 0..27
+// together with the keys, so the ranking below touches no memory.  The destination of each drop is
+// recorded as perm[dest] = p; k_nsort_gather then moves the records, so that each cache line of the
+// sorted store is written whole by one wave (runs scattered from many source gridboxes into one
+// line cost HBM read-modify-writes).
+__device__ __forceinline__ void nsort_rank_gbx(const NsortArgs& a, uint32_t g, int lane) {
   const uint32_t sbase = a.off[g];
   const uint32_t nfull = a.off[g + 1] - sbase;
   const uint32_t ngbx = a.grid.ngbx;
-  const uint32_t gz = g % a.grid.nz, gx = (g / a.grid.nz) % a.grid.nx, gy = g / (a.grid.nz * a.grid.nx);
-  uint32_t tkey = 0xFFFFFFFEu, trun = 0;  // lane i < 28 holds table slot i
-  int ntab = 0;
+  const int nz = static_cast<int>(a.grid.nz), nx = static_cast<int>(a.grid.nx), ny = static_cast<int>(a.grid.ny);
+  const int gz = static_cast<int>(g % a.grid.nz), gx = static_cast<int>((g / a.grid.nz) % a.grid.nx),
+            gy = static_cast<int>(g / (a.grid.nz * a.grid.nx));
+  // lane s < 27: next place of this gridbox's drops that take step s; lane 27: of those leaving
+  uint32_t trun = 0;
+  if (lane < 27) {
+    const uint32_t d = neighbour_of(a.grid, g, lane);
+    if (d != kOOB) trun = a.base[d] + a.cnt[28ull * d + lane];
+  } else if (lane == 27) {
+    trun = a.base[ngbx] + a.oobpre[g];
+  }
+  auto near = [](int q, int r, int nd, bool per) {
+    const int dd = q - r;
+    if (dd >= -1 && dd <= 1) return true;
+    return per && (dd == nd - 1 || dd == 1 - nd);
+  };
+  auto step = [](int q, int r) {
+    const int dd = q - r;
+    return dd > 1 ? -1 : (dd < -1 ? 1 : dd);
+  };
   for (uint32_t c = 0; c < nfull; c += kWave) {
     const uint32_t p = sbase + c + lane;
     const bool act = c + lane < nfull;
-    uint32_t key = 0;
-    if (act) {
-      key = a.src.gbx[p];
-      if (key >= ngbx) key = kOOB;
+    const uint32_t key = act ? a.src.gbx[p] : 0u;
+    // the step g -> key (27: out of the domain); a key that is no neighbour of g is an error
+    int sv = 27;
+    bool ok = act;
+    if (act && key < ngbx) {
+      const int vz = static_cast<int>(key % a.grid.nz), vx = static_cast<int>((key / a.grid.nz) % a.grid.nx),
+                vy = static_cast<int>(key / (a.grid.nz * a.grid.nx));
+      if (near(vz, gz, nz, a.grid.per[0]) && near(vx, gx, nx, a.grid.per[1]) && near(vy, gy, ny, a.grid.per[2])) {
+        sv = (step(vz, gz) + 1) + 3 * (step(vx, gx) + 1) + 9 * (step(vy, gy) + 1);
+      } else {
+        set_error(a.ctl, CLEO_ERR_SORT_NONLOCAL, g, p);
+        ok = false;
+      }
     }
     uint32_t dest = 0;
-    uint64_t mask = __ballot(act);
+    uint64_t mask = __ballot(ok);
     while (mask) {
       const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-      const uint32_t v = readlane_u32(key, leader);
-      const uint64_t m = __ballot(act && key == v);
-      const uint64_t hit = __ballot(lane < ntab && tkey == v);
-      uint32_t start;
-      int slot;
-      if (hit) {
-        slot = __ffsll(static_cast<unsigned long long>(hit)) - 1;
-        start = readlane_u32(trun, slot);
-      } else {
-        slot = ntab++;
-        if (v == kOOB) {
-          start = a.base[ngbx] + a.oobpre[g];
-        } else {
-          // v must be a neighbour of g
-          const int vz = static_cast<int>(v % a.grid.nz), vx = static_cast<int>((v / a.grid.nz) % a.grid.nx),
-                    vy = static_cast<int>(v / (a.grid.nz * a.grid.nx));
-          auto near = [](int q, int r, int nd, bool per) {
-            const int dd = q - r;
-            if (dd >= -1 && dd <= 1) return true;
-            return per && (dd == nd - 1 || dd == 1 - nd);
-          };
-          if (!(near(vz, static_cast<int>(gz), static_cast<int>(a.grid.nz), a.grid.per[0]) &&
-                near(vx, static_cast<int>(gx), static_cast<int>(a.grid.nx), a.grid.per[1]) &&
-                near(vy, static_cast<int>(gy), static_cast<int>(a.grid.ny), a.grid.per[2]))) {
-            if (lane == 0) set_error(a.ctl, CLEO_ERR_SORT_NONLOCAL, g, sbase + c + leader);
-          }
-          // the step g -> v, then the start k_nsort_tot left in row v
-          auto step = [](int q, int r, int nd) {
-            const int dd = q - r;
-            return dd > 1 ? -1 : (dd < -1 ? 1 : dd);
-          };
-          const int sv = (step(vz, static_cast<int>(gz), static_cast<int>(a.grid.nz)) + 1) +
-                         3 * (step(vx, static_cast<int>(gx), static_cast<int>(a.grid.nx)) + 1) +
-                         9 * (step(vy, static_cast<int>(gy), static_cast<int>(a.grid.ny)) + 1);
-          start = a.base[v] + a.cnt[28ull * v + sv];
-        }
-        if (lane == slot) tkey = v;
-      }
+      const int v = static_cast<int>(readlane_u32(static_cast<uint32_t>(sv), leader));
+      const uint64_t m = __ballot(ok && sv == v);
+      const uint32_t start = readlane_u32(trun, v);
       if ((m >> lane) & 1ull) dest = start + static_cast<uint32_t>(__popcll(m & lanemask_lt()));
-      if (lane == slot) trun = start + static_cast<uint32_t>(__popcll(m));
+      if (lane == v) trun = start + static_cast<uint32_t>(__popcll(m));
       mask &= ~m;
     }
-    if (act && dest >= a.ntot) {  // counts inconsistent with keys: never write out of range
+    if (ok && dest >= a.ntot) {  // counts inconsistent with keys: never write out of range
       set_error(a.ctl, CLEO_ERR_SORT_NONLOCAL, g, p);
-    } else if (act) {
-      a.dst.gbx[dest] = a.src.gbx[p];
-      a.dst.c3[dest] = a.src.c3[p];
-      a.dst.c1[dest] = a.src.c1[p];
-      a.dst.c2[dest] = a.src.c2[p];
-      a.dst.xi[dest] = a.src.xi[p];
-      a.dst.r[dest] = a.src.r[p];
-      a.dst.msol[dest] = a.src.msol[p];
-      a.dst.id[dest] = a.src.id[p];
+    } else if (ok) {
+      a.perm[dest] = p;
     }
   }
 }
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(8) k_nsort_scatter(NsortArgs a) {
+// first pass of the neighbour sort: the destination of every drop of the domain, as perm[dest] = p
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(8) k_nsort_perm(NsortArgs a) {
   if (guard_off(a.flag)) return;
   const int lane = lane_id();
-  for (uint32_t g = wave_index(); g < a.grid.ngbx; g += wave_count()) nsort_scatter_gbx(a, g, lane);
+  for (uint32_t g = wave_index(); g < a.grid.ngbx; g += wave_count()) nsort_rank_gbx(a, g, lane);
+}
+
+// Second pass: destination position q takes the record at source position perm[q].  Consecutive
+// threads write consecutive positions, so each cache line of the sorted store is written whole by
+// one wave; the reads follow the runs of the source order.  Two positions per thread, all loads of
+// both issued before the stores.  A position whose perm entry is not a source position of the
+// domain (counts inconsistent with the keys) is reported and skipped, so no read leaves the store.
+__global__ void __launch_bounds__(256) k_nsort_gather(NsortArgs a) {
+  if (guard_off(a.flag)) return;
+  const uint64_t nsrc = a.off[a.grid.ngbx];
+  for (uint64_t q0 = 2ull * blockIdx.x * blockDim.x; q0 < nsrc; q0 += 2ull * gridDim.x * blockDim.x) {
+    const uint64_t qa = q0 + threadIdx.x, qb = qa + blockDim.x;
+    const uint32_t pa = qa < nsrc ? a.perm[qa] : 0u, pb = qb < nsrc ? a.perm[qb] : 0u;
+    const bool oka = qa < nsrc && pa < nsrc, okb = qb < nsrc && pb < nsrc;
+    if (qa < nsrc && !oka) set_error(a.ctl, CLEO_ERR_SORT_NONLOCAL, kOOB, qa);
+    if (qb < nsrc && !okb) set_error(a.ctl, CLEO_ERR_SORT_NONLOCAL, kOOB, qb);
+    const uint32_t sa = oka ? pa : 0u, sb = okb ? pb : 0u;
+    const uint32_t ga = a.src.gbx[sa], gb = a.src.gbx[sb];
+    const double c3a = a.src.c3[sa], c3b = a.src.c3[sb];
+    const double c1a = a.src.c1[sa], c1b = a.src.c1[sb];
+    const double c2a = a.src.c2[sa], c2b = a.src.c2[sb];
+    const uint64_t xia = a.src.xi[sa], xib = a.src.xi[sb];
+    const double ra = a.src.r[sa], rb = a.src.r[sb];
+    const double ma = a.src.msol[sa], mb = a.src.msol[sb];
+    const uint64_t ida = a.src.id[sa], idb = a.src.id[sb];
+    if (oka) {
+      a.dst.gbx[qa] = ga;
+      a.dst.c3[qa] = c3a;
+      a.dst.c1[qa] = c1a;
+      a.dst.c2[qa] = c2a;
+      a.dst.xi[qa] = xia;
+      a.dst.r[qa] = ra;
+      a.dst.msol[qa] = ma;
+      a.dst.id[qa] = ida;
+    }
+    if (okb) {
+      a.dst.gbx[qb] = gb;
+      a.dst.c3[qb] = c3b;
+      a.dst.c1[qb] = c1b;
+      a.dst.c2[qb] = c2b;
+      a.dst.xi[qb] = xib;
+      a.dst.r[qb] = rb;
+      a.dst.msol[qb] = mb;
+      a.dst.id[qb] = idb;
+    }
+  }
 }
 
 // drops already out of the domain stay behind the newly out-of-domain ones, in order
@@ -1346,7 +1382,16 @@ static NsortArgs nsort_args(cleo_ctx* c, const SoA& src, const SoA& dst, const u
   a.ctl = c->ctl;
   a.flag = flag;
   a.ntot = c->ntot;
+  a.perm = c->perm;
   return a;
+}
+// the drops of the domain to their sorted places: destinations, then the records
+static void nsort_move(cleo_ctx* c, const NsortArgs& a) {
+  const uint32_t ngbx = c->grid.ngbx;
+  hipLaunchKernelGGL(k_nsort_perm, dim3(gbx_blocks(ngbx, a.wpb, c->tiles.motion_grid_waves)), dim3(64 * a.wpb), 0,
+                     c->stream, a);
+  hipLaunchKernelGGL(k_nsort_gather, dim3(motion_blocks(c->ntot, c->tiles.motion_grid_waves)), dim3(256), 0,
+                     c->stream, a);
 }
 // totals per destination, start offsets in the count table, new refs and the out-of-domain prefix
 static void nsort_tables(cleo_ctx* c, const NsortArgs& a, const uint32_t* flag) {
@@ -1369,8 +1414,7 @@ void neighbour_sort_to(cleo_ctx* c, const SoA& src, const SoA& dst, int commit_d
   const uint32_t ngbx = c->grid.ngbx;
   NsortArgs a = nsort_args(c, src, dst, nullptr);
   nsort_tables(c, a, nullptr);
-  hipLaunchKernelGGL(k_nsort_scatter, dim3(gbx_blocks(ngbx, a.wpb, c->tiles.motion_grid_waves)), dim3(64 * a.wpb), 0,
-                     c->stream, a);
+  nsort_move(c, a);
   hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, a, c->ntot);
   hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, static_cast<const uint32_t*>(nullptr));
   if (commit_dst >= 0) nsort_commit(c, commit_dst);
@@ -1387,8 +1431,7 @@ void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool fr
     hipLaunchKernelGGL(k_cnt_from_nact, dim3(nblocks(28ull * ngbx, 256)), dim3(256), 0, c->stream, c->off, c->nact, ngbx,
                        c->cnt, flag);
   nsort_tables(c, a, flag);
-  hipLaunchKernelGGL(k_nsort_scatter, dim3(gbx_blocks(ngbx, a.wpb, c->tiles.motion_grid_waves)), dim3(64 * a.wpb), 0,
-                     c->stream, a);
+  nsort_move(c, a);
   hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, a, c->ntot);
   hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, flag);
   if (flag == nullptr) {

@@ -93,6 +93,7 @@ struct NsortArgs {
   DevCtl* ctl;
   const uint32_t* flag;    // non-null: skip unless *flag
   uint64_t ntot;
+  uint32_t* perm;          // source position of each destination position of the domain's drops
 };
 
 // thread-per-drop condensation: prep (per gridbox) -> drops -> state (per gridbox)
