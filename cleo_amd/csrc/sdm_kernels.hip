@@ -1390,8 +1390,8 @@ static void nsort_move(cleo_ctx* c, const NsortArgs& a) {
   const uint32_t ngbx = c->grid.ngbx;
   hipLaunchKernelGGL(k_nsort_perm, dim3(gbx_blocks(ngbx, a.wpb, c->tiles.motion_grid_waves)), dim3(64 * a.wpb), 0,
                      c->stream, a);
-  hipLaunchKernelGGL(k_nsort_gather, dim3(motion_blocks(c->ntot, c->tiles.motion_grid_waves)), dim3(256), 0,
-                     c->stream, a);
+  // streaming copy: one pair of positions per thread (measured faster than a capped grid)
+  hipLaunchKernelGGL(k_nsort_gather, dim3(motion_blocks(c->ntot, 0)), dim3(256), 0, c->stream, a);
 }
 // totals per destination, start offsets in the count table, new refs and the out-of-domain prefix
 static void nsort_tables(cleo_ctx* c, const NsortArgs& a, const uint32_t* flag) {
