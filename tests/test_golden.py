@@ -1,0 +1,114 @@
+"""The golden fixtures of tests/golden/ (written by tests/golden/make_golden.py with the CPU oracle).
+
+- The oracle reproduces every fixture bit for bit, so a change of its arithmetic shows here.
+- The HIP path meets the parity bar against the same fixtures, with no oracle build needed:
+  - xi, sdgbxindex, sdid, the drop order and the gridbox refs are exact;
+  - radius, msol, coordinates and State agree within 1e-12 relative (qcond relative to qvap + qcond);
+  - per-drop Newton iteration counts are equal for at least 99.9 % of drops.
+
+Cases: three coupled steps with the Philox counter stream; two CoalBuRe events driven by an
+injected stream; one condensation pass."""
+import os
+
+import numpy as np
+import pytest
+
+from cleo_amd import abi
+from tests.golden.make_golden import COLL, COND, MOTION, STATE_KEYS, SUPER_KEYS, unpack_population
+from tests.helpers import (assert_state_match, assert_supers_match, coll_params, cond_params, make_domain,
+                           motion_params, sdm_params)
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = ("coupled_4x4x3", "collisions_2x2x2", "condensation_3x3x2")
+# tilings of the HIP path: the defaults (one wave per gridbox) and the bench workload's choice
+TILINGS = {"default": None, "flat": (1, 1, 1, 1, 1, 32768, 0, 32768, 1, 8)}
+
+
+def _load(name):
+    return np.load(os.path.join(HERE, f"{name}.npz"), allow_pickle=False)
+
+
+def _expected(f, prefix):
+    sup = {k: f[f"{prefix}{k}"] for k in SUPER_KEYS}
+    sup["ndomain"] = int(f[f"{prefix}ndomain"])
+    st = {k: f[f"{prefix}state_{k}"] for k in STATE_KEYS}
+    return sup, st, f[f"{prefix}refs"]
+
+
+def _got(d):
+    s = d.download()
+    sup = {k: s[k][: d.ntot] for k in SUPER_KEYS}
+    sup["ndomain"] = s["ndomain"]
+    return sup, d.state(), d.refs()
+
+
+def run_case(backend, name, f, tiles=None):
+    """Runs the fixture's case on `backend`; returns [(label, got, expected), ...] and the Newton
+    iteration counts (condensation case) as (got, expected)."""
+    pop = unpack_population(f)
+    d = make_domain(backend, pop)
+    if tiles is not None:
+        t = abi.Tiles(*tiles)
+        assert backend.lib.cleo_set_tiles(d.h, abi.ctypes_byref(t)) == 0
+    res, nit = [], None
+    if name.startswith("coupled"):
+        d.set_sdm(sdm_params(cond=cond_params(**COND), coll=coll_params(**COLL), motion=motion_params(**MOTION)))
+        d.rng_counter(int(f["rng_seed"]))
+        for step in range(3):
+            assert d.run_step(step * 100, (step + 1) * 100) == 0
+        res.append(("coupled", _got(d), _expected(f, "out_")))
+        st = d.stats()
+        got = np.array([st[k] for k in ("cond_drops", "coll_pairs", "coll_events", "nulls", "motion_drops")])
+        assert np.array_equal(got, f["stats"]), (got, f["stats"])
+    elif name.startswith("collisions"):
+        for ev in range(2):
+            d.rng_injected(f[f"js{ev}"], f[f"phis{ev}"])
+            assert d.collisions(coll_params(**COLL)) == 0
+            res.append((f"event {ev}", _got(d), _expected(f, f"ev{ev}_")))
+    else:
+        d.record_niters(True)
+        assert d.condensation(cond_params(**COND)) == 0
+        res.append(("condensation", _got(d), _expected(f, "out_")))
+        nit = (d.niters(), f["niters"])
+    assert d.error()[0] == 0, d.error()
+    d.close()
+    return res, nit
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_reproduces_golden(oracle, name):
+    f = _load(name)
+    res, nit = run_case(oracle, name, f)
+    for label, (gs, gst, gref), (es, est, eref) in res:
+        assert_supers_match(gs, es, exact_fp=True, what=f"{name} {label}")
+        for k in STATE_KEYS:
+            assert np.array_equal(gst[k], est[k]), f"{name} {label} state {k}"
+        assert np.array_equal(gref, eref)
+    if nit is not None:
+        assert np.array_equal(nit[0], nit[1])
+
+
+def test_golden_fixtures_are_small_and_complete():
+    for name in CASES:
+        path = os.path.join(HERE, f"{name}.npz")
+        assert os.path.getsize(path) < 256 * 1024
+        f = _load(name)
+        assert len(f["in_xi"]) > 300
+        pop = unpack_population(f)
+        assert pop.n == len(f["in_xi"]) and pop.ngbx == int(np.prod(f["ndims"]))
+        assert set(pop.supers) == set(SUPER_KEYS)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tiling", list(TILINGS))
+@pytest.mark.parametrize("name", CASES)
+def test_gpu_matches_golden(gpu, name, tiling):
+    print(f"case: golden fixture {name}, tiling {tiling}", flush=True)
+    f = _load(name)
+    res, nit = run_case(gpu, name, f, TILINGS[tiling])
+    for label, (gs, gst, gref), (es, est, eref) in res:
+        assert_supers_match(gs, es, what=f"{name} {label}")
+        assert_state_match(gst, est, what=f"{name} {label}")
+        assert np.array_equal(gref, eref)
+    if nit is not None:
+        assert np.mean(nit[0] == nit[1]) >= 0.999
