@@ -264,7 +264,9 @@ typedef struct {
   uint32_t cond_flat;              /* 1: one thread per drop + per-gridbox prep/sum kernels;
                                       2: the same with each wave's drops fed to its lanes from a
                                          task ring (lanes do not wait for slower drops) */
-  uint32_t motion_flat;            /* 1: one thread per drop, direction counts by atomics */
+  uint32_t motion_flat;            /* 1: one thread per drop (two per thread), direction counts by
+                                         atomics; 2: motion and sort fused (a counting pass, then
+                                         each drop moved again straight to its sorted place) */
   uint32_t cond_grid_waves;        /* waves per launch; each loops over gridboxes (or drops) */
   uint32_t coll_grid_waves;        /*   0: one wave per gridbox (one thread per drop) */
   uint32_t motion_grid_waves;
