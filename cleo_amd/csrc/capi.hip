@@ -219,11 +219,13 @@ int enqueue_motion(cleo_ctx* c, const cleo_motion_params& p, bool qcond) {
   if (c->tiles.motion_flat == 2) {
     motion_sort_fused(c, a, c->buf[1 - c->cur], 1 - c->cur);
   } else {
-    if (c->tiles.motion_flat)
+    if (c->tiles.motion_flat) {
+      a.step = c->dstep;  // the sort reads each drop's step instead of decoding its new key
       launch_motion_flat(c, a);
-    else
+    } else {
       launch_motion(c, a);
-    neighbour_sort(c, c->cur, 1 - c->cur, nullptr, false);
+    }
+    neighbour_sort(c, c->cur, 1 - c->cur, nullptr, false, a.step);
   }
   if (qcond) {
     QcondArgs q{c->buf[c->cur], c->off, c->st, c->solute, c->grid.ngbx, wpb_or1(c->tiles.cond_waves_per_block), c->ctl};
@@ -345,6 +347,7 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   HIP_TRY(c, dalloc(&c->dcount, c->sp_slots));
   HIP_TRY(c, dalloc(&c->dlist, maxn / 4 + 20 * c->sp_slots + 64));
   HIP_TRY(c, dalloc(&c->perm, std::max<uint64_t>(maxn, 1)));
+  HIP_TRY(c, dalloc(&c->dstep, std::max<uint64_t>(maxn, 1)));
   c->big_scratch_n = 2 * maxn;
   HIP_TRY(c, dalloc(&c->ctl, 1));
   HIP_TRY(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_ctl), sizeof(DevCtl), hipHostMallocDefault));
@@ -374,7 +377,7 @@ int cleo_destroy(cleo_ctx* c) {
                   static_cast<void*>(c->rs_key[1]), static_cast<void*>(c->rs_idx[0]), static_cast<void*>(c->rs_idx[1]),
                   static_cast<void*>(c->rs_cnt), static_cast<void*>(c->gs), static_cast<void*>(c->dm),
                   static_cast<void*>(c->sp), static_cast<void*>(c->dlist), static_cast<void*>(c->dcount),
-                  static_cast<void*>(c->perm)})
+                  static_cast<void*>(c->perm), static_cast<void*>(c->dstep)})
     if (p) hipFree(p);
   if (c->h_ctl) hipHostFree(c->h_ctl);
   for (hipEvent_t& e : c->ev)
@@ -906,13 +909,15 @@ static int time_stage(cleo_ctx* c, int kern, const cleo_tiles& t, int reps, bool
       if (t.motion_flat == 2) {
         motion_sort_fused(c, a, sorted, -1);
       } else {
-        if (t.motion_flat)
+        if (t.motion_flat) {
+          a.step = c->dstep;
           launch_motion_flat(c, a);
-        else
+        } else {
           launch_motion(c, a);
+        }
         const bool on = c->kev_on;
         c->kev_on = false;  // the main kernel of the unfused stage is the motion kernel
-        neighbour_sort_to(c, c->buf[spare], sorted, -1);
+        neighbour_sort_to(c, c->buf[spare], sorted, -1, a.step);
         c->kev_on = on;
       }
     }

@@ -106,8 +106,10 @@ struct cleo_ctx {
   // deferred-drop list of the thread-per-drop condensation (segments per wave) and their counts
   uint32_t* dlist = nullptr;
   uint32_t* dcount = nullptr;
-  // neighbour sort: source position of every destination position (maxn)
+  // neighbour sort: source position of every destination position (maxn), and the step each drop
+  // took in the last thread-per-drop motion (maxn bytes, 27 = left the domain)
   uint32_t* perm = nullptr;
+  uint8_t* dstep = nullptr;
   uint64_t sp_slots = 0;  // slots per counter: max(ngbx, waves of a thread-per-drop launch)
   // per-drop Newton iteration record
   bool rec_niters = false;

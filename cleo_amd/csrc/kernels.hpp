@@ -14,6 +14,16 @@ __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) -
 // smaller than one wave per gridbox amortises the dispatch of short waves
 __device__ __forceinline__ uint32_t wave_index() { return blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); }
 __device__ __forceinline__ uint32_t wave_count() { return gridDim.x * (blockDim.x >> 6); }
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b to XCD b % 8).  This bijection
+// of [0, gridDim.x) gives each XCD's workgroups one contiguous range of logical indices, in
+// dispatch order, so that work on neighbouring indices shares that XCD's L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t per = nb / kXcds, rem = nb % kXcds, x = b % kXcds;
+  return x * per + min(x, rem) + b / kXcds;
+}
+__device__ __forceinline__ uint32_t xcd_wave_index() { return xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6); }
 
 // LDS / memory ordering between lanes of one wavefront (no workgroup barrier needed)
 __device__ __forceinline__ void wave_sync() {
@@ -48,6 +58,11 @@ __device__ __forceinline__ double ordered_add(double acc, double v, int m) {
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
   return v;
 }
 

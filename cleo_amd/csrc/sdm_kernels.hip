@@ -824,7 +824,8 @@ __global__ void k_nsort_tot(NsortArgs a) {
 // key d lands at  base[d] + (drops reaching d from gridboxes < g) + (earlier drops of g with key d);
 // the middle term is row d of the count table after k_nsort_tot, at the step g -> d.  The 27 run
 // starts of g (one per step) and the start of its out-of-domain run are loaded by lanes 0..27
-// together with the keys, so the ranking below touches no memory.  The destination of each drop is
+// together with the keys (one round trip; loading only the steps that occur costs a second one),
+// so the ranking below touches no memory.  The destination of each drop is
 // recorded as perm[dest] = p; k_nsort_gather then moves the records, so that each cache line of the
 // sorted store is written whole by one wave (runs scattered from many source gridboxes into one
 // line cost HBM read-modify-writes).
@@ -835,11 +836,24 @@ __device__ __forceinline__ void nsort_rank_gbx(const NsortArgs& a, uint32_t g, i
   const int nz = static_cast<int>(a.grid.nz), nx = static_cast<int>(a.grid.nx), ny = static_cast<int>(a.grid.ny);
   const int gz = static_cast<int>(g % a.grid.nz), gx = static_cast<int>((g / a.grid.nz) % a.grid.nx),
             gy = static_cast<int>(g / (a.grid.nz * a.grid.nx));
-  // lane s < 27: next place of this gridbox's drops that take step s; lane 27: of those leaving
+  // lane s < 27: next place of this gridbox's drops that take step s; lane 27: of those leaving.
+  // The neighbour reached by step s = (dz+1) + 3(dx+1) + 9(dy+1), wrapped on periodic axes
+  // (the same gridbox as neighbour_of, without a division per lane).
   uint32_t trun = 0;
   if (lane < 27) {
-    const uint32_t d = neighbour_of(a.grid, g, lane);
-    if (d != kOOB) trun = a.base[d] + a.cnt[28ull * d + lane];
+    auto axis = [](int i, int d, int n, bool per) {
+      int j = i + d;
+      if (j < 0) j = per ? j + n : -1;
+      if (j >= n) j = per ? j - n : -1;
+      return j;
+    };
+    const int jz = axis(gz, lane % 3 - 1, nz, a.grid.per[0] != 0);
+    const int jx = axis(gx, (lane / 3) % 3 - 1, nx, a.grid.per[1] != 0);
+    const int jy = axis(gy, lane / 9 - 1, ny, a.grid.per[2] != 0);
+    if (jz >= 0 && jx >= 0 && jy >= 0) {
+      const uint32_t d = static_cast<uint32_t>(jz + nz * (jx + nx * jy));
+      trun = a.base[d] + a.cnt[28ull * d + lane];
+    }
   } else if (lane == 27) {
     trun = a.base[ngbx] + a.oobpre[g];
   }
@@ -855,11 +869,20 @@ __device__ __forceinline__ void nsort_rank_gbx(const NsortArgs& a, uint32_t g, i
   for (uint32_t c = 0; c < nfull; c += kWave) {
     const uint32_t p = sbase + c + lane;
     const bool act = c + lane < nfull;
-    const uint32_t key = act ? a.src.gbx[p] : 0u;
-    // the step g -> key (27: out of the domain); a key that is no neighbour of g is an error
+    // the step g -> key (27: out of the domain): recorded by motion, or found from the key; a key
+    // that is no neighbour of g is an error
     int sv = 27;
     bool ok = act;
-    if (act && key < ngbx) {
+    const uint32_t key = (act && !a.step) ? a.src.gbx[p] : 0u;
+    if (act && a.step) {
+      sv = a.step[p];
+      if (sv > 27) {
+        set_error(a.ctl, CLEO_ERR_SORT_NONLOCAL, g, p);
+        ok = false;
+      }
+    } else if (act && key == g) {
+      sv = 13;
+    } else if (act && key < ngbx) {
       const int vz = static_cast<int>(key % a.grid.nz), vx = static_cast<int>((key / a.grid.nz) % a.grid.nx),
                 vy = static_cast<int>(key / (a.grid.nz * a.grid.nx));
       if (near(vz, gz, nz, a.grid.per[0]) && near(vx, gx, nx, a.grid.per[1]) && near(vy, gy, ny, a.grid.per[2])) {
@@ -887,11 +910,13 @@ __device__ __forceinline__ void nsort_rank_gbx(const NsortArgs& a, uint32_t g, i
     }
   }
 }
-// first pass of the neighbour sort: the destination of every drop of the domain, as perm[dest] = p
+// first pass of the neighbour sort: the destination of every drop of the domain, as perm[dest] = p.
+// Gridboxes go to waves through xcd_wave_index(), so each XCD sweeps one contiguous range and the
+// count-table rows that neighbouring gridboxes share are fetched into one L2.
 __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(8) k_nsort_perm(NsortArgs a) {
   if (guard_off(a.flag)) return;
   const int lane = lane_id();
-  for (uint32_t g = wave_index(); g < a.grid.ngbx; g += wave_count()) nsort_rank_gbx(a, g, lane);
+  for (uint32_t g = xcd_wave_index(); g < a.grid.ngbx; g += wave_count()) nsort_rank_gbx(a, g, lane);
 }
 
 // Second pass: destination position q takes the record at source position perm[q].  Consecutive
@@ -1410,9 +1435,10 @@ static void nsort_commit(cleo_ctx* c, int dst) {
                      static_cast<const uint32_t*>(nullptr));
 }
 
-void neighbour_sort_to(cleo_ctx* c, const SoA& src, const SoA& dst, int commit_dst) {
+void neighbour_sort_to(cleo_ctx* c, const SoA& src, const SoA& dst, int commit_dst, const uint8_t* step) {
   const uint32_t ngbx = c->grid.ngbx;
   NsortArgs a = nsort_args(c, src, dst, nullptr);
+  a.step = step;
   nsort_tables(c, a, nullptr);
   nsort_move(c, a);
   hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, a, c->ntot);
@@ -1420,9 +1446,9 @@ void neighbour_sort_to(cleo_ctx* c, const SoA& src, const SoA& dst, int commit_d
   if (commit_dst >= 0) nsort_commit(c, commit_dst);
 }
 
-void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool from_nact) {
+void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool from_nact, const uint8_t* step) {
   if (flag == nullptr && !from_nact) {
-    neighbour_sort_to(c, c->buf[src], c->buf[dst], dst);
+    neighbour_sort_to(c, c->buf[src], c->buf[dst], dst, step);
     return;
   }
   const uint32_t ngbx = c->grid.ngbx;
@@ -1995,12 +2021,14 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_motion_drops(MotionAr
       a.s.c1[ka] = Ma.c1;
       a.s.c2[ka] = Ma.c2;
       a.s.gbx[ka] = Ma.idx;
+      if (a.step) a.step[ka] = static_cast<uint8_t>(Ma.dir);
     }
     if (kWrite && actb) {
       a.s.c3[kb] = Mb.c3;
       a.s.c1[kb] = Mb.c1;
       a.s.c2[kb] = Mb.c2;
       a.s.gbx[kb] = Mb.idx;
+      if (a.step) a.step[kb] = static_cast<uint8_t>(Mb.dir);
     }
     nmoved += static_cast<uint32_t>(__popcll(__ballot(acta))) + static_cast<uint32_t>(__popcll(__ballot(actb)));
     count_keys(a.cnt, motion_key(Ma, ga), acta, lane);

@@ -79,6 +79,7 @@ struct MotionArgs {
   uint32_t* cnt;
   DevCtl* ctl;
   StatSlots stat;  // ST_MOTION_DROPS
+  uint8_t* step;   // non-null: the step each drop takes (27: out of the domain), position order
 };
 
 struct NsortArgs {
@@ -94,6 +95,7 @@ struct NsortArgs {
   const uint32_t* flag;    // non-null: skip unless *flag
   uint64_t ntot;
   uint32_t* perm;          // source position of each destination position of the domain's drops
+  const uint8_t* step;     // non-null: each drop's step from motion, read instead of its key
 };
 
 // thread-per-drop condensation: prep (per gridbox) -> drops -> state (per gridbox)
@@ -144,9 +146,10 @@ void scan_u32(cleo_ctx* c, const uint32_t* in, uint64_t len, uint32_t stride, ui
 size_t scan_tmp_words(uint64_t len);
 // stable neighbour sort of buf[src] into buf[dst] after motion (flag == nullptr) or after
 // collisions (flag = &ctl->resort_flag, data ends back in buf[src] and refs in c->off)
-void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool from_nact);
+void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool from_nact,
+                    const uint8_t* step = nullptr);
 // the neighbour sort of `src` (after motion) into `dst`; commit_dst >= 0: that buffer becomes current
-void neighbour_sort_to(cleo_ctx* c, const SoA& src, const SoA& dst, int commit_dst);
+void neighbour_sort_to(cleo_ctx* c, const SoA& src, const SoA& dst, int commit_dst, const uint8_t* step = nullptr);
 // motion + neighbour sort fused into two passes (motion_flat 2): src = args.s is left unchanged
 void motion_sort_fused(cleo_ctx* c, const MotionArgs& args, const SoA& dst, int commit_dst);
 // generic stable sort of buf[cur] into buf[1-cur]; swaps cur; sets off and nact
