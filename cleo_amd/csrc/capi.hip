@@ -125,6 +125,7 @@ CondArgs cond_args(cleo_ctx* c, const cleo_cond_params& p, bool fuse_qcond) {
 int enqueue_condensation(cleo_ctx* c, const cleo_cond_params& p, bool fuse_qcond) {
   if (p.interval == 0) return fail(c, CLEO_E_ARG, "condensation interval 0");
   if (!(p.minsubdelt_s > 0.0)) return fail(c, CLEO_E_ARG, "MINSUBTSTEP must be positive");
+  materialize_order(c);  // the per-gridbox sums run in drop order
   CondArgs a = cond_args(c, p, fuse_qcond);
   // the thread-per-drop sequence evaluates the learned guesses itself (after the saturation
   // ratios are known); the others read them from a separate guess pass
@@ -169,7 +170,8 @@ int prepare_injected(cleo_ctx* c) {
   return CLEO_OK;
 }
 
-// one collision event over every gridbox: buf[cur] -> buf[1-cur], cur flips
+// one collision event over every gridbox: the drops stay in buf[cur], their shuffled order goes to
+// c->order (a pending order of an earlier event is applied first)
 int enqueue_collision_event(cleo_ctx* c, const cleo_coll_params& p) {
   if (p.interval == 0) return fail(c, CLEO_E_ARG, "collision interval 0");
   if (c->rng_mode == 0) return fail(c, CLEO_E_STATE, "no random-number stream set");
@@ -178,9 +180,10 @@ int enqueue_collision_event(cleo_ctx* c, const cleo_coll_params& p) {
     const int rc = prepare_injected(c);
     if (rc != CLEO_OK) return rc;
   }
+  materialize_order(c);
   CollArgs a{};
-  a.src = c->buf[c->cur];
-  a.dst = c->buf[1 - c->cur];
+  a.s = c->buf[c->cur];
+  a.order = c->order;
   a.off = c->off;
   a.nact = c->nact;
   a.vol = c->st.vol;
@@ -196,9 +199,8 @@ int enqueue_collision_event(cleo_ctx* c, const cleo_coll_params& p) {
   a.inj_op = c->inj_op;
   a.big = c->big_scratch;
   launch_collisions(c, a);
-  launch_copy_oob_region(c, c->cur, 1 - c->cur);
   launch_advance_event(c);
-  c->cur = 1 - c->cur;
+  c->order_live = true;
   if (inj) c->rng_mode = 0;  // consumed
   return launch_check(c);
 }
@@ -217,6 +219,8 @@ int enqueue_motion(cleo_ctx* c, const cleo_motion_params& p, bool qcond) {
   a.cnt = c->cnt;
   a.ctl = c->ctl;
   if (c->tiles.motion_flat == 2) {
+    materialize_order(c);  // the fused pass ranks the drops in store order
+    a.s = c->buf[c->cur];
     motion_sort_fused(c, a, c->buf[1 - c->cur], 1 - c->cur);
   } else {
     if (c->tiles.motion_flat) {
@@ -236,7 +240,7 @@ int enqueue_motion(cleo_ctx* c, const cleo_motion_params& p, bool qcond) {
 
 // SDMMethods::run_step(t_mdl, t_mdl_next) (sdmmethods.hpp:319-332): host loop over SDM sub-steps;
 // every gridbox follows the same schedule, so each process runs as one launch over all gridboxes.
-int enqueue_run_step(cleo_ctx* c, uint32_t t_mdl, uint32_t t_mdl_next, int* swaps) {
+int enqueue_run_step(cleo_ctx* c, uint32_t t_mdl, uint32_t t_mdl_next) {
   const cleo_sdm_params& p = c->sdm;
   const bool mot = p.enable_motion != 0, cond = p.enable_cond != 0, coll = p.enable_colls != 0;
   const uint32_t UMAX = 0xFFFFFFFFu;
@@ -250,7 +254,6 @@ int enqueue_run_step(cleo_ctx* c, uint32_t t_mdl, uint32_t t_mdl_next, int* swap
       const int rc = enqueue_motion(c, p.motion, false);
       if (rc != CLEO_OK) return rc;
       pending_qcond = true;
-      if (swaps) *swaps += 1;
     }
     bool any_coll = false;
     for (uint32_t subt = t_sdm; subt < t_sdm_next;) {
@@ -271,7 +274,6 @@ int enqueue_run_step(cleo_ctx* c, uint32_t t_mdl, uint32_t t_mdl_next, int* swap
         const int rc = enqueue_collision_event(c, p.coll);
         if (rc != CLEO_OK) return rc;
         any_coll = true;
-        if (swaps) *swaps += 1;
       }
       const uint32_t a = cond ? next_of(subt, p.cond.interval) : UMAX;
       const uint32_t b = coll ? next_of(subt, p.coll.interval) : UMAX;
@@ -348,6 +350,7 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   HIP_TRY(c, dalloc(&c->dlist, maxn / 4 + 20 * c->sp_slots + 64));
   HIP_TRY(c, dalloc(&c->perm, std::max<uint64_t>(maxn, 1)));
   HIP_TRY(c, dalloc(&c->dstep, std::max<uint64_t>(maxn, 1)));
+  HIP_TRY(c, dalloc(&c->order, std::max<uint64_t>(maxn, 1)));
   c->big_scratch_n = 2 * maxn;
   HIP_TRY(c, dalloc(&c->ctl, 1));
   HIP_TRY(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_ctl), sizeof(DevCtl), hipHostMallocDefault));
@@ -377,7 +380,7 @@ int cleo_destroy(cleo_ctx* c) {
                   static_cast<void*>(c->rs_key[1]), static_cast<void*>(c->rs_idx[0]), static_cast<void*>(c->rs_idx[1]),
                   static_cast<void*>(c->rs_cnt), static_cast<void*>(c->gs), static_cast<void*>(c->dm),
                   static_cast<void*>(c->sp), static_cast<void*>(c->dlist), static_cast<void*>(c->dcount),
-                  static_cast<void*>(c->perm), static_cast<void*>(c->dstep)})
+                  static_cast<void*>(c->perm), static_cast<void*>(c->dstep), static_cast<void*>(c->order)})
     if (p) hipFree(p);
   if (c->h_ctl) hipHostFree(c->h_ctl);
   for (hipEvent_t& e : c->ev)
@@ -403,6 +406,7 @@ int cleo_upload_supers(cleo_ctx* c, const cleo_supers_soa* h, uint64_t ntot) {
   if (!c || !h) return CLEO_E_ARG;
   if (ntot > c->maxn) return fail(c, CLEO_E_ARG, "ntot exceeds the context capacity");
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->order_live = false;  // the uploaded drops replace the store and its pending order
   SoA& d = c->buf[c->cur];
   HIP_TRY(c, ctx_copy(c, d.gbx, h->sdgbxindex, ntot * 4, hipMemcpyHostToDevice));
   HIP_TRY(c, ctx_copy(c, d.c3, h->coord3, ntot * 8, hipMemcpyHostToDevice));
@@ -427,6 +431,7 @@ int cleo_upload_supers_device(cleo_ctx* c, const cleo_supers_soa* h, uint64_t nt
   if (!c || !h) return CLEO_E_ARG;
   if (ntot > c->maxn) return fail(c, CLEO_E_ARG, "ntot exceeds the context capacity");
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->order_live = false;  // the uploaded drops replace the store and its pending order
   SoA& d = c->buf[c->cur];
   HIP_TRY(c, ctx_copy(c, d.gbx, h->sdgbxindex, ntot * 4, hipMemcpyDeviceToDevice));
   HIP_TRY(c, ctx_copy(c, d.c3, h->coord3, ntot * 8, hipMemcpyDeviceToDevice));
@@ -470,6 +475,7 @@ int cleo_upload_supers_aos(cleo_ctx* c, const void* aos, uint64_t ntot) {
 
 int cleo_download_supers(cleo_ctx* c, cleo_supers_soa* h, uint64_t* ntot, uint64_t* ndomain) {
   if (!c || !h) return CLEO_E_ARG;
+  materialize_order(c);  // the drops in their order
   const int rc = sync_check(c);
   const uint64_t n = c->ntot;
   const SoA& d = c->buf[c->cur];
@@ -601,6 +607,7 @@ int cleo_set_sdm(cleo_ctx* c, const cleo_sdm_params* p) {
 
 int cleo_sort(cleo_ctx* c) {
   if (!c) return CLEO_E_ARG;
+  materialize_order(c);  // the generic sort is stable in the drops' order
   const int rc = ensure_radix(c, c->ntot);
   if (rc != CLEO_OK) return rc;
   generic_sort(c);
@@ -644,7 +651,7 @@ int cleo_run_step(cleo_ctx* c, uint32_t t_mdl, uint32_t t_mdl_next) {
   if (c->sdm.enable_cond &&
       step2dimlesstime(c->sdm.cond.interval) < c->sdm.cond.minsubdelt_s / dimless::TIME0)
     return set_substep_error(c);
-  return enqueue_run_step(c, t_mdl, t_mdl_next, nullptr);
+  return enqueue_run_step(c, t_mdl, t_mdl_next);
 }
 
 int cleo_run_steps_graph(cleo_ctx* c, uint32_t t_mdl, uint32_t span, uint32_t nsteps) {
@@ -661,31 +668,56 @@ int cleo_run_steps_graph(cleo_ctx* c, uint32_t t_mdl, uint32_t span, uint32_t ns
     if (rc != CLEO_OK) return rc;
   }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  // capture one step; if it flips the double buffer an odd number of times, capture two
-  const int cur0 = c->cur;
-  int swaps = 0;
+  // The launches a step enqueues depend on the store's state when it starts (which buffer is
+  // current, whether a collision order is pending), so every replay must start in the state the
+  // previous one ended in.  One step is captured, or two when one step changes the state; if two
+  // still do, the start state is set to the end state outside the graph and the capture repeated.
   hipGraph_t graph = nullptr;
-  HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_run_step(c, t_mdl, t_mdl + span, &swaps);
   uint32_t per_graph = 1;
-  if (rc == CLEO_OK && (swaps % 2) != 0) {
-    rc = enqueue_run_step(c, t_mdl + span, t_mdl + 2 * span, &swaps);
-    per_graph = 2;
-  }
-  hipError_t e = hipStreamEndCapture(c->stream, &graph);
-  if (rc != CLEO_OK) {
-    if (graph) hipGraphDestroy(graph);
+  int cur0 = c->cur;
+  bool live0 = c->order_live;
+  int rc = CLEO_OK;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    cur0 = c->cur;
+    live0 = c->order_live;
+    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_run_step(c, t_mdl, t_mdl + span);
+    per_graph = 1;
+    if (rc == CLEO_OK && (c->cur != cur0 || c->order_live != live0)) {
+      rc = enqueue_run_step(c, t_mdl + span, t_mdl + 2 * span);
+      per_graph = 2;
+    }
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    const int cur1 = c->cur;
+    const bool live1 = c->order_live;
     c->cur = cur0;
-    return rc;
+    c->order_live = live0;
+    if (rc != CLEO_OK || e != hipSuccess) {
+      if (graph) hipGraphDestroy(graph);
+      if (rc != CLEO_OK) return rc;
+      return fail(c, CLEO_E_HIP, std::string("capture: ") + hipGetErrorString(e));
+    }
+    if (cur1 == cur0 && live1 == live0) break;
+    hipGraphDestroy(graph);
+    graph = nullptr;
+    if (live1 && !live0) {
+      pending_identity_order(c);  // the same drops in the same order, as a pending order
+    } else if (!live1 && live0) {
+      materialize_order(c);
+    } else {
+      return fail(c, CLEO_E_STATE, "the step leaves the store in a state no replay can start from");
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
   }
-  if (e != hipSuccess) return fail(c, CLEO_E_HIP, std::string("capture: ") + hipGetErrorString(e));
+  if (!graph) return fail(c, CLEO_E_STATE, "no consistent step graph");
   hipGraphExec_t exec = nullptr;
-  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   if (e != hipSuccess) {
     hipGraphDestroy(graph);
     return fail(c, CLEO_E_HIP, std::string("instantiate: ") + hipGetErrorString(e));
   }
-  c->cur = cur0;  // every replay starts and ends on the same buffer
+  c->cur = cur0;  // every replay starts and ends in the same state
+  c->order_live = live0;
   const uint32_t nreplay = nsteps / per_graph;
   for (uint32_t i = 0; i < nreplay; ++i) {
     e = hipGraphLaunch(exec, c->stream);
@@ -695,7 +727,7 @@ int cleo_run_steps_graph(cleo_ctx* c, uint32_t t_mdl, uint32_t span, uint32_t ns
   hipGraphDestroy(graph);
   if (e != hipSuccess) return fail(c, CLEO_E_HIP, std::string("graph launch: ") + hipGetErrorString(e));
   if (nsteps % per_graph) rc = enqueue_run_step(c, t_mdl + nreplay * per_graph * span,
-                                               t_mdl + (nreplay * per_graph + 1) * span, nullptr);
+                                               t_mdl + (nreplay * per_graph + 1) * span);
   return rc;
 }
 
@@ -815,12 +847,14 @@ int cleo_set_tiles(cleo_ctx* c, const cleo_tiles* t) {
 }
 
 // Runs one stage `reps` times with tiling `t` on a copy of the resident drops in the spare buffer
-// (collisions read the current buffer with timing_only set); State fields that condensation
-// writes, the active counts and the control block are restored afterwards.  Reports the fastest
-// and the mean duration, of the whole stage or (kernel_only) of its main kernel alone.
+// (a pending collision order is applied first; the collision order array is scratch here); State
+// fields that condensation writes, the active counts and the control block are restored
+// afterwards.  Reports the fastest and the mean duration, of the whole stage or (kernel_only) of
+// its main kernel alone.
 static int time_stage(cleo_ctx* c, int kern, const cleo_tiles& t, int reps, bool kernel_only, float* tmin,
                       float* tmean) {
   const uint32_t ngbx = c->grid.ngbx;
+  materialize_order(c);
   const int cur = c->cur, spare = 1 - c->cur;
   for (int s = 0; s < 2; ++s) {
     if (!c->ev[14 + s]) HIP_TRY(c, hipEventCreate(&c->ev[14 + s]));
@@ -878,8 +912,8 @@ static int time_stage(cleo_ctx* c, int kern, const cleo_tiles& t, int reps, bool
       }
     } else if (kern == 1) {
       CollArgs a{};
-      a.src = c->buf[cur];
-      a.dst = c->buf[spare];
+      a.s = c->buf[spare];
+      a.order = c->order;
       a.off = c->off;
       a.nact = c->nact;
       a.vol = c->st.vol;
@@ -893,7 +927,6 @@ static int time_stage(cleo_ctx* c, int kern, const cleo_tiles& t, int reps, bool
       a.inj_oj = c->inj_oj;
       a.inj_op = c->inj_op;
       a.big = c->big_scratch;
-      a.timing_only = 1;
       launch_collisions(c, a);
     } else {
       MotionArgs a{};

@@ -110,6 +110,12 @@ struct cleo_ctx {
   // took in the last thread-per-drop motion (maxn bytes, 27 = left the domain)
   uint32_t* perm = nullptr;
   uint8_t* dstep = nullptr;
+  // Collisions leave the drops in place and record the shuffled order: order[q] is the position
+  // of the drop at slot q of the domain's logical order.  While order_live is set the store's
+  // physical order may differ from its logical order; the next sort applies the order and
+  // materialize_order() applies it for callers that need the drops in order.
+  uint32_t* order = nullptr;
+  bool order_live = false;
   uint64_t sp_slots = 0;  // slots per counter: max(ngbx, waves of a thread-per-drop launch)
   // per-drop Newton iteration record
   bool rec_niters = false;

@@ -479,6 +479,7 @@ int cleo_mlp_make_samples(cleo_ctx* c, const cleo_cond_params* p, float* x_out, 
   const uint64_t n = c->ntot;
   *nwritten = 0;
   if (n == 0) return CLEO_OK;
+  materialize_order(c);  // samples in drop order
   float *x = nullptr, *y = nullptr;
   uint32_t *flag = nullptr, *pos = nullptr;
   MLP_TRY(c, hipMalloc(reinterpret_cast<void**>(&x), n * kInPad * 4));

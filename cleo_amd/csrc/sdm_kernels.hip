@@ -162,10 +162,12 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_qcond(QcondArgs a) {
 }
 
 // ============================ collisions ======================================================
-// DoCollisions::do_collisions for one gridbox per wave (collisions.hpp:140-278):
-// Fisher-Yates on a u32 index permutation (all lanes draw the indices first; the n-1 swaps run
-// in registers for n <= 128, else one lane walks them in LDS / HBM scratch), pairs (2jj, 2jj+1) one per lane, enactment, then the drops are written to
-// `dst` in shuffled order with nulls moved behind the active drops (stable partition).
+// DoCollisions::do_collisions (collisions.hpp:140-278): Fisher-Yates shuffle of a gridbox's drops,
+// pairs (2jj, 2jj+1) of the shuffled order, enactment, nulls moved behind the active drops
+// (stable partition).  The drops stay where they are: the shuffled order is recorded in
+// CollArgs::order (slot -> position) and only the attributes of enacted pairs are written, in
+// place.  The reference moves every drop to its shuffled place; here the next sort moves each drop
+// once, to its sorted place in that order.
 // j_t of Fisher-Yates swap t < n-1 of gridbox g: uniform in [0, iter], iter = n-1-t
 // (shuffle.hpp:80-87), from the counter stream or the injected indices
 __device__ __forceinline__ uint32_t fy_index(const CollArgs& a, uint32_t g, uint32_t base, uint32_t n, uint32_t t,
@@ -186,7 +188,7 @@ __device__ __forceinline__ uint32_t fy_index(const CollArgs& a, uint32_t g, uint
   return j;
 }
 
-// One collision pair (2jj, 2jj+1) of gridbox g (n active drops) from the drops at ka, kb of `src`
+// One collision pair (2jj, 2jj+1) of gridbox g (n active drops) from the drops at ka, kb
 // (collisions.hpp:180-230): probability and enactment on (xi, r, msol) of both drops.
 struct PairOut {
   DropAttrs A, B;  // the drops at ka and kb after the enactment
@@ -200,8 +202,8 @@ __device__ __forceinline__ PairOut coll_pair_eval(const CollArgs& a, uint32_t g,
   const double scale_p = n * (n - 1.0) / (2.0 * npairs);
   const double VOLUME = a.vol[g] * dimless::VOL0;
   PairOut o;
-  const DropAttrs A{a.src.xi[ka], a.src.r[ka], a.src.msol[ka]};
-  const DropAttrs B{a.src.xi[kb], a.src.r[kb], a.src.msol[kb]};
+  const DropAttrs A{a.s.xi[ka], a.s.r[ka], a.s.msol[ka]};
+  const DropAttrs B{a.s.xi[kb], a.s.r[kb], a.s.msol[kb]};
   const bool a_first = !(A.xi < B.xi);  // assign_drops (collisions.hpp:94-101)
   // d1 is the drop with the larger multiplicity; selected field by field (a reference picked by a
   // lane-varying flag would put both drops in scratch memory)
@@ -229,87 +231,60 @@ __device__ __forceinline__ PairOut coll_pair_eval(const CollArgs& a, uint32_t g,
   o.B = DropAttrs{a_first ? d2.xi : d1.xi, a_first ? d2.r : d1.r, a_first ? d2.msol : d1.msol};
   return o;
 }
-// drop of `src` at ks written to `dst` at kd with attributes D (other fields carried)
-__device__ __forceinline__ void coll_write(const CollArgs& a, uint32_t kd, uint32_t ks, const DropAttrs& D, bool isnull) {
-  a.dst.gbx[kd] = isnull ? kOOB : a.src.gbx[ks];
-  a.dst.c3[kd] = a.src.c3[ks];
-  a.dst.c1[kd] = a.src.c1[ks];
-  a.dst.c2[kd] = a.src.c2[ks];
-  a.dst.xi[kd] = D.xi;
-  a.dst.r[kd] = D.r;
-  a.dst.msol[kd] = D.msol;
-  a.dst.id[kd] = a.src.id[ks];
+// the attributes of drop k after an enacted pair, in place; a null drop gets the out-of-domain key
+__device__ __forceinline__ void coll_store(const CollArgs& a, uint32_t k, const DropAttrs& D, bool isnull) {
+  a.s.xi[k] = D.xi;
+  a.s.r[k] = D.r;
+  a.s.msol[k] = D.msol;
+  if (isnull) a.s.gbx[k] = kOOB;
 }
-// the pair written to its shuffled positions base+2jj, base+2jj+1 of `dst`; true when one of the
-// two became null; `events` counts enactments
+// pair jj of the drops at ka, kb: the slots 2jj, 2jj+1 of the logical order, and the new attributes
+// when it was enacted; true when one of the two became null; `events` counts enactments
 __device__ __forceinline__ bool coll_pair(const CollArgs& a, uint32_t g, uint32_t base, uint32_t n, uint32_t jj,
                                           uint32_t ka, uint32_t kb, uint64_t event, uint32_t& events) {
   const PairOut o = coll_pair_eval(a, g, n, jj, ka, kb, event);
   if (o.err) set_error(a.ctl, o.err, g, base + 2 * jj);
-  events += o.enacted ? 1u : 0u;
-  coll_write(a, base + 2 * jj, ka, o.A, o.nullA);
-  coll_write(a, base + 2 * jj + 1, kb, o.B, o.nullB);
+  a.order[base + 2 * jj] = ka;
+  a.order[base + 2 * jj + 1] = kb;
+  if (o.enacted) {
+    events += 1u;
+    coll_store(a, ka, o.A, o.nullA);
+    coll_store(a, kb, o.B, o.nullB);
+  }
   return o.nullA || o.nullB;
 }
-// drop k of a gridbox that takes no part in a pair (the unpaired last drop and the inactive
-// tail) is carried unchanged from src[ks] to dst[kd]
-__device__ __forceinline__ void coll_carry(const CollArgs& a, uint32_t ks, uint32_t kd) {
-  a.dst.gbx[kd] = a.src.gbx[ks];
-  a.dst.c3[kd] = a.src.c3[ks];
-  a.dst.c1[kd] = a.src.c1[ks];
-  a.dst.c2[kd] = a.src.c2[ks];
-  a.dst.xi[kd] = a.src.xi[ks];
-  a.dst.r[kd] = a.src.r[ks];
-  a.dst.msol[kd] = a.src.msol[ks];
-  a.dst.id[kd] = a.src.id[ks];
-}
 
-// remove_null_supers (collisions.hpp:240-248) for gridbox g after its pairs were written to
-// `dst`: the reference's sort_team is restated as a stable partition (active drops first, nulls
-// behind, each in order), through the now unused `src` range of the gridbox.  Only this wave
-// reads back what it stored, so a workgroup-scope fence orders it (an agent-scope fence would
+// remove_null_supers (collisions.hpp:240-248) for gridbox g after its pairs: the reference's
+// sort_team is restated as a stable partition of the gridbox's logical order (active drops first,
+// nulls behind, each in order).  The partitioned order is staged in the gridbox's HBM scratch
+// (2 u32 per drop from big + 2 base, free once the shuffle is done) and copied back.  Only this
+// wave reads what it stored, so a workgroup-scope fence orders it (an agent-scope fence would
 // write back the XCD's whole L2).
-__device__ __forceinline__ void coll_remove_nulls(const CollArgs& a, uint32_t g, uint32_t base, uint32_t n,
-                                                  uint32_t nulls, int lane) {
+__device__ __forceinline__ void coll_order_nulls(const CollArgs& a, uint32_t g, uint32_t base, uint32_t n,
+                                                 uint32_t nulls, int lane) {
   __threadfence_block();
   wave_sync();
+  uint32_t* tmp = a.big + 2ull * base;
   const uint32_t nn = n - nulls;
   uint32_t run_nn = 0, run_null = 0;
   for (uint32_t c = 0; c < n; c += kWave) {
     const uint32_t k = c + lane;
     const bool act = k < n;
-    const bool isnull = act && a.dst.xi[base + k] == 0;
+    const uint32_t pk = act ? a.order[base + k] : 0u;
+    const bool isnull = act && a.s.xi[pk] == 0;
     const uint64_t mnull = __ballot(isnull);
     const uint64_t mnn = __ballot(act && !isnull);
     if (act) {
       const uint32_t pos = isnull ? nn + run_null + __popcll(mnull & lanemask_lt())
                                   : run_nn + __popcll(mnn & lanemask_lt());
-      const uint32_t ks = base + k, kd = base + pos;
-      a.src.gbx[kd] = a.dst.gbx[ks];
-      a.src.c3[kd] = a.dst.c3[ks];
-      a.src.c1[kd] = a.dst.c1[ks];
-      a.src.c2[kd] = a.dst.c2[ks];
-      a.src.xi[kd] = a.dst.xi[ks];
-      a.src.r[kd] = a.dst.r[ks];
-      a.src.msol[kd] = a.dst.msol[ks];
-      a.src.id[kd] = a.dst.id[ks];
+      tmp[pos] = pk;
     }
     run_null += __popcll(mnull);
     run_nn += __popcll(mnn);
   }
   __threadfence_block();
   wave_sync();
-  for (uint32_t k = lane; k < n; k += kWave) {
-    const uint32_t kk = base + k;
-    a.dst.gbx[kk] = a.src.gbx[kk];
-    a.dst.c3[kk] = a.src.c3[kk];
-    a.dst.c1[kk] = a.src.c1[kk];
-    a.dst.c2[kk] = a.src.c2[kk];
-    a.dst.xi[kk] = a.src.xi[kk];
-    a.dst.r[kk] = a.src.r[kk];
-    a.dst.msol[kk] = a.src.msol[kk];
-    a.dst.id[kk] = a.src.id[kk];
-  }
+  for (uint32_t k = lane; k < n; k += kWave) a.order[base + k] = tmp[k];
   if (lane == 0) {
     a.nact[g] = nn;
     a.ctl->resort_flag = 1u;
@@ -375,9 +350,9 @@ __device__ __forceinline__ void coll_gbx(const CollArgs& a, int lane, uint32_t g
     if (jj < npairs) nul = coll_pair(a, g, base, n, jj, base + perm[2 * jj], base + perm[2 * jj + 1], event, events);
     nulls += static_cast<uint32_t>(__popcll(__ballot(nul)));
   }
-  // unpaired last drop and the inactive tail (nulls of earlier sub-steps) are carried unchanged
-  for (uint32_t k = 2 * npairs + lane; k < nfull; k += kWave) coll_carry(a, base + (k < n ? perm[k] : k), base + k);
-  if (nulls > 0 && !a.timing_only) coll_remove_nulls(a, g, base, n, nulls, lane);
+  // unpaired last drop and the inactive tail (nulls of earlier sub-steps) keep their places
+  for (uint32_t k = 2 * npairs + lane; k < nfull; k += kWave) a.order[base + k] = base + (k < n ? perm[k] : k);
+  if (nulls > 0) coll_order_nulls(a, g, base, n, nulls, lane);
   acc[0] += npairs;
   acc[1] += wave_sum_u32(events);
   acc[2] += nulls;
@@ -407,13 +382,14 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_collisions(CollArgs a
 // ---- tiled collisions: 64 gridboxes per wave ---------------------------------------------------
 // Lane i shuffles gridbox 64*tile + i: its permutation (one byte per drop) lives in the wave's LDS
 // tile and the swaps of all 64 gridboxes run side by side, with the same index sequence as the
-// per-gridbox path.  The tile's pairs and carried drops are then flattened over the lanes (a lane
-// finds its gridbox by binary search over the prefix of per-gridbox counts), so every lane works
+// per-gridbox path.  The tile's pairs are then flattened over the lanes (a lane finds its gridbox
+// by binary search over the prefix of per-gridbox pair counts), so every lane evaluates a pair
 // whatever the gridbox sizes.  Gridboxes above kTileMaxN drops, or beyond the tile's LDS, take the
 // per-gridbox path through HBM scratch.
 struct CollTileLds {
   uint32_t perm32[kTilePermBytes / 4];  // byte permutations, gridbox i from byte loff[i]
-  uint32_t pre[kWave + 1];              // first drop of gridbox i relative to the tile's first
+  uint32_t ppre[kWave + 1];             // first pair of gridbox i among the tile's pairs
+  uint32_t base[kWave];                 // first drop of gridbox i
   uint32_t loff[kWave];                 // kBigGbx: the gridbox takes the per-gridbox path
   uint32_t n[kWave];
   uint32_t nulls[kWave];
@@ -479,67 +455,40 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_coll_tiles(CollArgs a
       }
     }
     const uint32_t np = small ? n / 2 : 0u;
-    // the tile's drops [t_lo, t_hi) in destination order: lane L writes position t_lo + f, f = L,
-    // L + 64, ..., so every store of the wave is one contiguous run; both lanes of a pair evaluate
-    // it and each writes its own drop
-    const uint32_t t_lo = readlane_u32(base, 0);
-    const uint32_t t_hi = a.off[min(tile * kWave + kWave, a.ngbx)];
-    L.pre[lane] = valid ? base - t_lo : t_hi - t_lo;
-    if (lane == kWave - 1) L.pre[kWave] = t_hi - t_lo;
+    uint32_t ptot;
+    const uint32_t pfirst = wave_excl_scan(np, lane, ptot);
+    L.ppre[lane] = pfirst;
+    if (lane == kWave - 1) L.ppre[kWave] = ptot;
+    L.base[lane] = base;
     L.loff[lane] = small ? loff : kBigGbx;
     L.n[lane] = n;
     L.nulls[lane] = 0u;
     wave_sync();
-    // chunks start on multiples of 64 drops, so that every store instruction covers whole cache
-    // lines (a partly written line costs a read-modify-write in HBM); only the tile's two edge
-    // chunks are partial
-    const uint32_t a_lo = t_lo & ~static_cast<uint32_t>(kWave - 1);
-    for (uint32_t pos = a_lo + lane; pos < t_hi; pos += kWave) {
-      if (pos < t_lo) continue;
-      const uint32_t f = pos - t_lo;
-      const uint32_t i = tile_owner(L.pre, f);
-      const uint32_t li = L.loff[i];
-      if (li == kBigGbx) continue;
-      const uint32_t slot = f - L.pre[i], ni = L.n[i], bi = t_lo + L.pre[i];
-      const uint8_t* pi = pb + li;
-      // what this lane writes: its side of a pair, or a carried drop; one store sequence for all
-      // lanes so that each store instruction covers a contiguous run
-      uint32_t ks;
-      DropAttrs D;
-      bool isnull = false;
-      if (slot < 2u * (ni / 2u)) {
-        const uint32_t jj = slot >> 1, side = slot & 1u;
-        const uint32_t gi = tile * kWave + i, ka = bi + pi[2 * jj], kb = bi + pi[2 * jj + 1];
-        const PairOut o = coll_pair_eval(a, gi, ni, jj, ka, kb, event);
-        if (side == 0u) {
-          if (o.err) set_error(a.ctl, o.err, gi, bi + 2 * jj);
-          events += o.enacted ? 1u : 0u;
-          if (o.nullA || o.nullB) atomicAdd(&L.nulls[i], 1u);
-        }
-        // field by field: selecting a whole struct by a lane-varying flag puts it in scratch
-        ks = side ? kb : ka;
-        D.xi = side ? o.B.xi : o.A.xi;
-        D.r = side ? o.B.r : o.A.r;
-        D.msol = side ? o.B.msol : o.A.msol;
-        isnull = side ? o.nullB : o.nullA;
-      } else {
-        ks = bi + (slot < ni ? static_cast<uint32_t>(pi[slot]) : slot);
-        D = DropAttrs{a.src.xi[ks], a.src.r[ks], a.src.msol[ks]};
-      }
-      coll_write(a, bi + slot, ks, D, isnull);
+    // the tile's pairs flattened over the lanes (pair f belongs to the gridbox i with
+    // ppre[i] <= f < ppre[i+1]): each pair is evaluated once, by one lane, which records its two
+    // slots of the logical order and stores the drops' new attributes when it was enacted
+    for (uint32_t f = lane; f < ptot; f += kWave) {
+      const uint32_t i = tile_owner(L.ppre, f);
+      const uint32_t jj = f - L.ppre[i], bi = L.base[i];
+      const uint8_t* pi = pb + L.loff[i];
+      if (coll_pair(a, tile * kWave + i, bi, L.n[i], jj, bi + pi[2 * jj], bi + pi[2 * jj + 1], event, events))
+        atomicAdd(&L.nulls[i], 1u);
+    }
+    // the unpaired last drop and the inactive tail of the lane's own gridbox keep their places
+    if (small) {
+      const uint8_t* p = pb + loff;
+      for (uint32_t k = 2u * np; k < nfull; ++k) a.order[base + k] = base + (k < n ? static_cast<uint32_t>(p[k]) : k);
     }
     wave_sync();
     const uint32_t mynulls = L.nulls[lane];
     pairs += np;
     nulls += mynulls;
-    if (!a.timing_only) {
-      uint64_t m = __ballot(mynulls > 0u);
-      while (m) {
-        const int i = __ffsll(static_cast<unsigned long long>(m)) - 1;
-        m &= m - 1;
-        coll_remove_nulls(a, tile * kWave + static_cast<uint32_t>(i), readlane_u32(base, i), readlane_u32(n, i),
-                          readlane_u32(mynulls, i), lane);
-      }
+    uint64_t m = __ballot(mynulls > 0u);
+    while (m) {
+      const int i = __ffsll(static_cast<unsigned long long>(m)) - 1;
+      m &= m - 1;
+      coll_order_nulls(a, tile * kWave + static_cast<uint32_t>(i), readlane_u32(base, i), readlane_u32(n, i),
+                       readlane_u32(mynulls, i), lane);
     }
     uint64_t mb = __ballot(valid && !small);
     while (mb) {
@@ -866,23 +815,31 @@ __device__ __forceinline__ void nsort_rank_gbx(const NsortArgs& a, uint32_t g, i
     const int dd = q - r;
     return dd > 1 ? -1 : (dd < -1 ? 1 : dd);
   };
+  const uint32_t nsrc = a.off[ngbx];
   for (uint32_t c = 0; c < nfull; c += kWave) {
-    const uint32_t p = sbase + c + lane;
+    // slot q of the gridbox's logical order holds the drop at position p (the order collisions
+    // left, or the store's own order)
+    const uint32_t q = sbase + c + lane;
     const bool act = c + lane < nfull;
+    const uint32_t p = (act && a.order) ? a.order[q] : q;
     // the step g -> key (27: out of the domain): recorded by motion, or found from the key; a key
-    // that is no neighbour of g is an error
+    // that is no neighbour of g, or a position outside the domain, is an error
     int sv = 27;
     bool ok = act;
-    const uint32_t key = (act && !a.step) ? a.src.gbx[p] : 0u;
-    if (act && a.step) {
+    if (act && p >= nsrc) {
+      set_error(a.ctl, CLEO_ERR_SORT_NONLOCAL, g, q);
+      ok = false;
+    }
+    const uint32_t key = (ok && !a.step) ? a.src.gbx[p] : 0u;
+    if (ok && a.step) {
       sv = a.step[p];
       if (sv > 27) {
         set_error(a.ctl, CLEO_ERR_SORT_NONLOCAL, g, p);
         ok = false;
       }
-    } else if (act && key == g) {
+    } else if (ok && key == g) {
       sv = 13;
-    } else if (act && key < ngbx) {
+    } else if (ok && key < ngbx) {
       const int vz = static_cast<int>(key % a.grid.nz), vx = static_cast<int>((key / a.grid.nz) % a.grid.nx),
                 vy = static_cast<int>(key / (a.grid.nz * a.grid.nx));
       if (near(vz, gz, nz, a.grid.per[0]) && near(vx, gx, nx, a.grid.per[1]) && near(vy, gy, ny, a.grid.per[2])) {
@@ -1262,6 +1219,7 @@ static inline unsigned nblocks(uint64_t n, uint64_t per) {
   return static_cast<unsigned>(b == 0 ? 1 : b);
 }
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v);
+__global__ void k_iota_u32(uint32_t* p, uint64_t n, const uint32_t* flag);
 template <bool kWrite>
 static void launch_motion_drops(cleo_ctx* c, const MotionArgs& a, unsigned nb);
 // blocks of the motion drop loop (two drops per thread), capped at `grid_waves` waves when non-zero
@@ -1408,6 +1366,7 @@ static NsortArgs nsort_args(cleo_ctx* c, const SoA& src, const SoA& dst, const u
   a.flag = flag;
   a.ntot = c->ntot;
   a.perm = c->perm;
+  a.order = c->order_live ? c->order : nullptr;
   return a;
 }
 // the drops of the domain to their sorted places: destinations, then the records
@@ -1429,6 +1388,7 @@ static void nsort_tables(cleo_ctx* c, const NsortArgs& a, const uint32_t* flag) 
 static void nsort_commit(cleo_ctx* c, int dst) {
   const uint32_t ngbx = c->grid.ngbx;
   c->cur = dst;
+  c->order_live = false;  // the sorted store is in its logical order
   hipLaunchKernelGGL(k_copy_u32, dim3(stride_blocks(ngbx + 1ull)), dim3(256), 0, c->stream, c->off_alt, c->off,
                      ngbx + 1ull, static_cast<const uint32_t*>(nullptr));
   hipLaunchKernelGGL(k_set_nact, dim3(nblocks(ngbx, 256)), dim3(256), 0, c->stream, c->off, ngbx, c->nact,
@@ -1465,11 +1425,34 @@ void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool fr
   } else {
     hipLaunchKernelGGL(k_copy_soa, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, c->buf[dst], c->buf[src],
                        0ull, c->ntot, flag);
+    // the copied-back store is in its logical order: if the sort ran, the order becomes the identity
+    if (c->order_live)
+      hipLaunchKernelGGL(k_iota_u32, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, c->order, c->ntot, flag);
     hipLaunchKernelGGL(k_copy_u32, dim3(stride_blocks(ngbx + 1ull)), dim3(256), 0, c->stream, c->off_alt, c->off,
                        ngbx + 1ull, flag);
     hipLaunchKernelGGL(k_set_nact, dim3(nblocks(ngbx, 256)), dim3(256), 0, c->stream, c->off, ngbx, c->nact, flag);
     hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c->stream, c->ctl);
   }
+}
+
+// A pending collision order applied: the domain's drops are gathered into the other buffer in
+// their logical order (the out-of-domain drops are copied as they are) and that buffer becomes
+// current.  Nothing happens when the store is already in order.
+// the store as it is, with the identity as its pending order
+void pending_identity_order(cleo_ctx* c) {
+  hipLaunchKernelGGL(k_iota_u32, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, c->order, c->ntot,
+                     static_cast<const uint32_t*>(nullptr));
+  c->order_live = true;
+}
+void materialize_order(cleo_ctx* c) {
+  if (!c->order_live) return;
+  const int src = c->cur, dst = 1 - c->cur;
+  NsortArgs a = nsort_args(c, c->buf[src], c->buf[dst], nullptr);
+  a.perm = c->order;
+  hipLaunchKernelGGL(k_nsort_gather, dim3(motion_blocks(c->ntot, 0)), dim3(256), 0, c->stream, a);
+  launch_copy_oob_region(c, src, dst);
+  c->cur = dst;
+  c->order_live = false;
 }
 
 // Motion + neighbour sort in two passes over the drops: count the moves, build the tables, then
@@ -1958,6 +1941,14 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_state(CondFlatAr
       a.st.qc[g] = qc;
     }
   }
+}
+
+// p[i] = i for i < n, skipped unless *flag when flag is non-null
+__global__ void k_iota_u32(uint32_t* p, uint64_t n, const uint32_t* flag) {
+  if (flag && *flag == 0u) return;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = static_cast<uint32_t>(i);
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {

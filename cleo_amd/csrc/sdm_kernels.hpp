@@ -11,7 +11,7 @@ constexpr int kStatSlotsMax = 4;       // counters per wave of one kernel
 // tiled collisions: byte permutations of up to kTileMaxN drops, kTilePermBytes of them per wave
 constexpr uint32_t kTileMaxN = 255;
 constexpr uint32_t kTilePermBytes = 8192;
-constexpr uint32_t kCollTileLdsBytes = kTilePermBytes + 4u * (65u + 3u * 64u);
+constexpr uint32_t kCollTileLdsBytes = kTilePermBytes + 4u * (65u + 4u * 64u);
 constexpr uint32_t kBigGbx = 0xFFFFFFFFu;  // tile LDS marker of a gridbox on the per-gridbox path
 // condensation task ring: prepared drops per wave, and waves per CU of the default grid
 constexpr uint32_t kRing = 128;
@@ -49,7 +49,8 @@ struct QcondArgs {
 };
 
 struct CollArgs {
-  SoA src, dst;
+  SoA s;            // the store: drops keep their places, changed attributes are written in place
+  uint32_t* order;  // out: position of the drop at each slot of the shuffled (logical) order
   const uint32_t* off;
   uint32_t* nact;
   const double* vol;
@@ -64,8 +65,7 @@ struct CollArgs {
   const uint32_t* inj_oj;
   const uint32_t* inj_op;
   uint32_t* big;
-  int32_t timing_only;  // autotuner runs: no null partition, `src` and `nact` stay untouched
-  StatSlots stat;       // ST_COLL_PAIRS .. ST_NULLS
+  StatSlots stat;   // ST_COLL_PAIRS .. ST_NULLS
 };
 
 struct MotionArgs {
@@ -96,6 +96,7 @@ struct NsortArgs {
   uint64_t ntot;
   uint32_t* perm;          // source position of each destination position of the domain's drops
   const uint8_t* step;     // non-null: each drop's step from motion, read instead of its key
+  const uint32_t* order;   // non-null: position of the drop at each slot of the logical order
 };
 
 // thread-per-drop condensation: prep (per gridbox) -> drops -> state (per gridbox)
@@ -155,5 +156,9 @@ void motion_sort_fused(cleo_ctx* c, const MotionArgs& args, const SoA& dst, int 
 // generic stable sort of buf[cur] into buf[1-cur]; swaps cur; sets off and nact
 void generic_sort(cleo_ctx* c);
 void launch_copy_oob_region(cleo_ctx* c, int src, int dst);
+// applies a pending collision order to the store (buf[cur] then holds the drops in order)
+void materialize_order(cleo_ctx* c);
+// the identity as the pending order (the store unchanged)
+void pending_identity_order(cleo_ctx* c);
 
 }  // namespace cleo
