@@ -34,15 +34,19 @@ BYTES = {
     # read sdgbxindex, r, msol, xi; write r, dm | saturation ratio and diffusion factor, temp, refs
     "k_cond_drops": (44, 2 * 8 + 8 + 8),
     "k_cond_queue": (44, 2 * 8 + 8 + 8),
-    # read 60-byte record; write 60-byte record in shuffled order | refs, nact, volume
-    "k_collisions": (120, 8 + 4 + 8),
-    "k_coll_tiles": (120, 8 + 4 + 8),
+    # read xi, r, msol of every paired drop; write its slot of the shuffled order (the drops stay
+    # in place; enacted pairs, a few per cent, also write their attributes) | refs, nact, volume
+    "k_collisions": (28, 8 + 4 + 8),
+    "k_coll_tiles": (28, 8 + 4 + 8),
     # read coords, radius, sdgbxindex; write coords, sdgbxindex | bounds, winds, counts
     "k_motion": (64, 6 * 8 + 6 * 8 + 28 * 4),
     "k_motion_flat": (64, 6 * 8 + 6 * 8 + 28 * 4),
     # fused motion + sort, moving pass: read coords, radius, xi, msol, id; write the 60-byte record
     # at its sorted place | bounds, winds, refs, run starts
     "k_motion_scatter": (112, 6 * 8 + 6 * 8 + 8 + 4 * 8),
+    # counting pass of the motion in the sort's gather: read coords, radius, sdgbxindex; write the
+    # step | bounds, winds, counts
+    "k_motion_keys": (37, 6 * 8 + 6 * 8 + 28 * 4),
 }
 
 
@@ -67,7 +71,7 @@ def stage_kernel(stage, tiles):
         return {0: "k_condensation", 1: "k_cond_drops", 2: "k_cond_queue"}[tiles.cond_flat]
     if stage == 1:
         return "k_coll_tiles" if tiles.coll_flat else "k_collisions"
-    return {0: "k_motion", 1: "k_motion_flat", 2: "k_motion_scatter"}[tiles.motion_flat]
+    return {0: "k_motion", 1: "k_motion_flat", 2: "k_motion_scatter", 3: "k_motion_keys"}[tiles.motion_flat]
 
 
 def parse():

@@ -222,6 +222,8 @@ int enqueue_motion(cleo_ctx* c, const cleo_motion_params& p, bool qcond) {
     materialize_order(c);  // the fused pass ranks the drops in store order
     a.s = c->buf[c->cur];
     motion_sort_fused(c, a, c->buf[1 - c->cur], 1 - c->cur);
+  } else if (c->tiles.motion_flat == 3) {
+    motion_sort_gather(c, a, c->buf[1 - c->cur], 1 - c->cur);
   } else {
     if (c->tiles.motion_flat) {
       a.step = c->dstep;  // the sort reads each drop's step instead of decoding its new key
@@ -839,8 +841,8 @@ int cleo_set_tiles(cleo_ctx* c, const cleo_tiles* t) {
   if (!c || !t) return CLEO_E_ARG;
   for (uint32_t v : {t->cond_waves_per_block, t->coll_waves_per_block, t->motion_waves_per_block})
     if (v > 4) return fail(c, CLEO_E_ARG, "waves per block must be 0..4");
-  if (t->cond_flat > 2 || t->motion_flat > 2 || t->coll_flat > 1)
-    return fail(c, CLEO_E_ARG, "variant flags: cond_flat and motion_flat 0..2, coll_flat 0 or 1");
+  if (t->cond_flat > 2 || t->motion_flat > 3 || t->coll_flat > 1)
+    return fail(c, CLEO_E_ARG, "variant flags: cond_flat 0..2, motion_flat 0..3, coll_flat 0 or 1");
   if (t->cond_defer == 1) return fail(c, CLEO_E_ARG, "cond_defer must be 0 or at least 2");
   c->tiles = *t;
   return CLEO_OK;
@@ -941,6 +943,8 @@ static int time_stage(cleo_ctx* c, int kern, const cleo_tiles& t, int reps, bool
       a.ctl = c->ctl;
       if (t.motion_flat == 2) {
         motion_sort_fused(c, a, sorted, -1);
+      } else if (t.motion_flat == 3) {
+        motion_sort_gather(c, a, sorted, -1);
       } else {
         if (t.motion_flat) {
           a.step = c->dstep;
@@ -990,8 +994,8 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
   struct Cand {
     uint32_t wpb, flat;
   };
-  constexpr int kNC = 6;
-  const Cand cands[kNC] = {{1, 0}, {2, 0}, {4, 0}, {1, 1}, {4, 1}, {1, 2}};  // {waves per block, variant}
+  constexpr int kNC = 7;
+  const Cand cands[kNC] = {{1, 0}, {2, 0}, {4, 0}, {1, 1}, {4, 1}, {1, 2}, {1, 3}};  // {waves per block, variant}
   const uint32_t grid_waves[3] = {0, 8192, 32768};
   cleo_tiles best_t = c->tiles;
   for (int kern = 0; kern < 3; ++kern) {
@@ -1000,7 +1004,8 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
       const Cand& cd = cands[gi % kNC];
       const uint32_t gw = grid_waves[gi / kNC];
       if (kern != 1 && cd.flat && cd.wpb != 1) continue;  // thread-per-drop kernels ignore wpb
-      if (kern == 1 && cd.flat == 2) continue;              // collisions have no third variant
+      if (kern == 1 && cd.flat >= 2) continue;              // collisions have two variants
+      if (kern == 0 && cd.flat == 3) continue;              // condensation has three
       cleo_tiles t = best_t;
       if (kern == 0) {
         t.cond_waves_per_block = cd.wpb;

@@ -1222,6 +1222,7 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v);
 __global__ void k_iota_u32(uint32_t* p, uint64_t n, const uint32_t* flag);
 template <bool kWrite>
 static void launch_motion_drops(cleo_ctx* c, const MotionArgs& a, unsigned nb);
+static void launch_gather_move(cleo_ctx* c, const NsortArgs& n, const MotionArgs& m);
 // blocks of the motion drop loop (two drops per thread), capped at `grid_waves` waves when non-zero
 static inline unsigned motion_blocks(uint64_t n, uint32_t grid_waves) {
   const unsigned b = nblocks(n, 512);
@@ -1472,6 +1473,33 @@ void motion_sort_fused(cleo_ctx* c, const MotionArgs& args, const SoA& dst, int 
   hipLaunchKernelGGL(k_motion_scatter, dim3(gbx_blocks(ngbx, n.wpb, c->tiles.motion_grid_waves)), dim3(64 * n.wpb), 0,
                      c->stream, a, n);
   kev_mark(c, 1);
+  hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, n, c->ntot);
+  hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, static_cast<const uint32_t*>(nullptr));
+  if (commit_dst >= 0) nsort_commit(c, commit_dst);
+}
+
+// Motion + neighbour sort with the motion arithmetic in the gather (motion_flat 3): a counting
+// pass records each drop's step (nothing of the store changes), the tables and the rank pass
+// follow, and the gather moves each drop again as it writes it to its sorted place in `dst`.
+// commit_dst >= 0: that buffer becomes current.
+void motion_sort_gather(cleo_ctx* c, const MotionArgs& args, const SoA& dst, int commit_dst) {
+  const uint32_t ngbx = c->grid.ngbx;
+  if (ngbx == 0) return;
+  MotionArgs a = args;
+  a.step = c->dstep;
+  hipLaunchKernelGGL(k_fill_u32, dim3(stride_blocks(28ull * ngbx)), dim3(256), 0, c->stream, a.cnt, 28ull * ngbx, 0u);
+  const unsigned nb = motion_blocks(c->ntot, c->tiles.motion_grid_waves);
+  a.stat = stat_slots(c, 4ull * nb);
+  kev_mark(c, 0);
+  launch_motion_drops<false>(c, a, nb);
+  kev_mark(c, 1);
+  launch_stats_reduce(c, 4u * nb, ST_MOTION_DROPS, 1);
+  NsortArgs n = nsort_args(c, a.s, dst, nullptr);
+  n.step = c->dstep;
+  nsort_tables(c, n, nullptr);
+  hipLaunchKernelGGL(k_nsort_perm, dim3(gbx_blocks(ngbx, n.wpb, c->tiles.motion_grid_waves)), dim3(64 * n.wpb), 0,
+                     c->stream, n);
+  launch_gather_move(c, n, a);
   hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, n, c->ntot);
   hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, static_cast<const uint32_t*>(nullptr));
   if (commit_dst >= 0) nsort_commit(c, commit_dst);
@@ -1977,9 +2005,10 @@ constexpr uint32_t kMotionLdsBounds = 2048;
 
 // MoveSupersInGridboxesFunctor per drop, two drops per thread and iteration with every load of
 // both issued before the arithmetic; the destination-major counts are aggregated per wave, then
-// added with atomics.  kLds: the coordinate bounds are read from an LDS copy (dynamic shared
-// memory of (nz + nx + ny + 3) doubles).  Without kWrite this is the counting pass of the fused
-// motion + sort: nothing of the drops is stored.
+// added with atomics, and the step of each drop goes to a.step when that is set.  kLds: the
+// coordinate bounds are read from an LDS copy (dynamic shared memory of (nz + nx + ny + 3)
+// doubles).  Without kWrite this is the counting pass of a fused motion + sort: the drops' new
+// coordinates and gridboxes are not stored (the second pass computes them again).
 template <bool kWrite, bool kLds>
 __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_motion_drops(MotionArgs a) {
   extern __shared__ double bounds_lds[];
@@ -2012,15 +2041,15 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_motion_drops(MotionAr
       a.s.c1[ka] = Ma.c1;
       a.s.c2[ka] = Ma.c2;
       a.s.gbx[ka] = Ma.idx;
-      if (a.step) a.step[ka] = static_cast<uint8_t>(Ma.dir);
     }
     if (kWrite && actb) {
       a.s.c3[kb] = Mb.c3;
       a.s.c1[kb] = Mb.c1;
       a.s.c2[kb] = Mb.c2;
       a.s.gbx[kb] = Mb.idx;
-      if (a.step) a.step[kb] = static_cast<uint8_t>(Mb.dir);
     }
+    if (a.step && acta) a.step[ka] = static_cast<uint8_t>(Ma.dir);
+    if (a.step && actb) a.step[kb] = static_cast<uint8_t>(Mb.dir);
     nmoved += static_cast<uint32_t>(__popcll(__ballot(acta))) + static_cast<uint32_t>(__popcll(__ballot(actb)));
     count_keys(a.cnt, motion_key(Ma, ga), acta, lane);
     count_keys(a.cnt, motion_key(Mb, gb), actb, lane);
@@ -2036,6 +2065,58 @@ static void launch_motion_drops(cleo_ctx* c, const MotionArgs& a, unsigned nb) {
     hipLaunchKernelGGL((k_motion_drops<kWrite, true>), dim3(nb), dim3(256), nb_ * sizeof(double), c->stream, a);
   else
     hipLaunchKernelGGL((k_motion_drops<kWrite, false>), dim3(nb), dim3(256), 0, c->stream, a);
+}
+
+// Motion fused into the sort's gather (motion_flat 3): destination q takes the drop at position
+// perm[q] of the unmoved store, moved by the same predictor-corrector step the counting pass
+// computed for it (the arithmetic is deterministic, so the new gridbox is the key the drop was
+// ranked by); the new coordinates and gridbox go straight to q.  One position per thread and
+// iteration (two would spill); bounds from LDS when kLds.  A perm entry that is not a position of
+// the domain is reported and skipped.
+template <bool kLds>
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_nsort_gather_move(NsortArgs n, MotionArgs m) {
+  extern __shared__ double bounds_lds[];
+  MotionArgs b = m;
+  if constexpr (kLds) {
+    const uint32_t nzh = m.grid.nz + 1, nxh = m.grid.nx + 1, nyh = m.grid.ny + 1;
+    for (uint32_t i = threadIdx.x; i < nzh + nxh + nyh; i += blockDim.x)
+      bounds_lds[i] = i < nzh ? m.grid.zh[i] : (i < nzh + nxh ? m.grid.xh[i - nzh] : m.grid.yh[i - nzh - nxh]);
+    __syncthreads();
+    b.grid.zh = bounds_lds;
+    b.grid.xh = bounds_lds + nzh;
+    b.grid.yh = bounds_lds + nzh + nxh;
+  }
+  const uint64_t nsrc = n.off[n.grid.ngbx];
+  for (uint64_t q = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < nsrc;
+       q += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t p = n.perm[q];
+    if (p >= nsrc) {
+      set_error(n.ctl, CLEO_ERR_SORT_NONLOCAL, kOOB, q);
+      continue;
+    }
+    const uint32_t g = n.src.gbx[p];
+    const DropPos P = drop_pos(b, p);
+    // an in-domain drop's gridbox is below ngbx; the guard keeps a corrupted one from indexing
+    const uint32_t gg = g < n.grid.ngbx ? g : 0u;
+    const GbxGeom G = gbx_geom(b, gg);
+    const Moved M = move_drop(b, gg, G, P, p, false);
+    n.dst.gbx[q] = M.idx;
+    n.dst.c3[q] = M.c3;
+    n.dst.c1[q] = M.c1;
+    n.dst.c2[q] = M.c2;
+    n.dst.xi[q] = n.src.xi[p];
+    n.dst.r[q] = P.r;
+    n.dst.msol[q] = n.src.msol[p];
+    n.dst.id[q] = n.src.id[p];
+  }
+}
+static void launch_gather_move(cleo_ctx* c, const NsortArgs& n, const MotionArgs& m) {
+  const uint32_t nb_ = m.grid.nz + m.grid.nx + m.grid.ny + 3;
+  const unsigned nb = drop_blocks(c->ntot, 0);
+  if (nb_ <= kMotionLdsBounds)
+    hipLaunchKernelGGL(k_nsort_gather_move<true>, dim3(nb), dim3(256), nb_ * sizeof(double), c->stream, n, m);
+  else
+    hipLaunchKernelGGL(k_nsort_gather_move<false>, dim3(nb), dim3(256), 0, c->stream, n, m);
 }
 
 void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& args) {

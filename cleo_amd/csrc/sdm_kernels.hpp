@@ -151,6 +151,9 @@ void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool fr
                     const uint8_t* step = nullptr);
 // the neighbour sort of `src` (after motion) into `dst`; commit_dst >= 0: that buffer becomes current
 void neighbour_sort_to(cleo_ctx* c, const SoA& src, const SoA& dst, int commit_dst, const uint8_t* step = nullptr);
+// motion + neighbour sort with the motion arithmetic in the sort's gather (motion_flat 3): the
+// store args.s is left unchanged, the moved and sorted drops go to dst
+void motion_sort_gather(cleo_ctx* c, const MotionArgs& args, const SoA& dst, int commit_dst);
 // motion + neighbour sort fused into two passes (motion_flat 2): src = args.s is left unchanged
 void motion_sort_fused(cleo_ctx* c, const MotionArgs& args, const SoA& dst, int commit_dst);
 // generic stable sort of buf[cur] into buf[1-cur]; swaps cur; sets off and nact

@@ -266,7 +266,9 @@ typedef struct {
                                          task ring (lanes do not wait for slower drops) */
   uint32_t motion_flat;            /* 1: one thread per drop (two per thread), direction counts by
                                          atomics; 2: motion and sort fused (a counting pass, then
-                                         each drop moved again straight to its sorted place) */
+                                         each drop moved again straight to its sorted place);
+                                      3: a counting pass, then the sort's gather moves each drop
+                                         as it copies it to its sorted place */
   uint32_t cond_grid_waves;        /* waves per launch; each loops over gridboxes (or drops) */
   uint32_t coll_grid_waves;        /*   0: one wave per gridbox (one thread per drop) */
   uint32_t motion_grid_waves;

@@ -21,7 +21,8 @@ from tests.helpers import (assert_state_match, assert_supers_match, coll_params,
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CASES = ("coupled_4x4x3", "collisions_2x2x2", "condensation_3x3x2")
 # tilings of the HIP path: the defaults (one wave per gridbox) and the bench workload's choice
-TILINGS = {"default": None, "flat": (1, 1, 1, 1, 1, 32768, 0, 32768, 1, 8)}
+TILINGS = {"default": None, "flat": (1, 1, 1, 1, 1, 32768, 0, 32768, 1, 8),
+           "gather": (1, 1, 1, 1, 3, 32768, 0, 32768, 1, 8)}
 
 
 def _load(name):

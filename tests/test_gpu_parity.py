@@ -147,7 +147,7 @@ def test_collisions_tiled_mixed_sizes(gpu, oracle, ndims, mean):
         assert sg[k] == so[k], k
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_motion(gpu, oracle, variant):
     _say(f"case: predictor-corrector motion + neighbour sort, 6 steps, motion variant {variant}")
     pop = P.coupled(SEED, ndims=(6, 5, 4), spacing_m=25.0, mean=50, wmax_mps=6.0)
@@ -231,6 +231,8 @@ def test_tiles_do_not_change_results(gpu):
     rows += [(1, 1, 0, 1, 2), (1, 1, 8, 0, 3), (1, 1, 1, 1, 6)]
     # fused motion + sort (motion variant 2), with one wave per gridbox and with looping waves
     rows += [(1, 1, 0, 2, 0), (2, 2, 16, 2, 0), (1, 1, 3, 2, 6)]
+    # motion in the sort's gather (motion variant 3)
+    rows += [(1, 1, 0, 3, 0), (2, 1, 16, 3, 8)]
     for wpb, cflat, gw, flat, defer in rows:
         d = make_domain(gpu, pop)
         t = abi.Tiles(wpb, wpb, wpb, cflat, flat, gw, gw, gw, min(flat, 1), defer)
@@ -247,7 +249,7 @@ def test_tiles_do_not_change_results(gpu):
         assert st == stats[0]
 
 
-@pytest.mark.parametrize("tiles", [None, (1, 1, 1, 1, 2, 8192, 0, 8192, 1, 6)])
+@pytest.mark.parametrize("tiles", [None, (1, 1, 1, 1, 2, 8192, 0, 8192, 1, 6), (1, 1, 1, 1, 3, 8192, 0, 8192, 1, 8)])
 def test_multiblock_scans_coupled(gpu, oracle, tiles):
     _say(f"case: 4096 gridboxes (multi-block scans), 2 coupled steps, tiles {tiles}")
     pop = P.coupled(SEED, ndims=(16, 16, 16), mean=64)
