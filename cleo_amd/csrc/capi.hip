@@ -353,6 +353,7 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   HIP_TRY(c, dalloc(&c->perm, std::max<uint64_t>(maxn, 1)));
   HIP_TRY(c, dalloc(&c->dstep, std::max<uint64_t>(maxn, 1)));
   HIP_TRY(c, dalloc(&c->order, std::max<uint64_t>(maxn, 1)));
+  HIP_TRY(c, dalloc(&c->qm, std::max<uint64_t>(maxn, 1)));
   c->big_scratch_n = 2 * maxn;
   HIP_TRY(c, dalloc(&c->ctl, 1));
   HIP_TRY(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_ctl), sizeof(DevCtl), hipHostMallocDefault));
@@ -382,7 +383,8 @@ int cleo_destroy(cleo_ctx* c) {
                   static_cast<void*>(c->rs_key[1]), static_cast<void*>(c->rs_idx[0]), static_cast<void*>(c->rs_idx[1]),
                   static_cast<void*>(c->rs_cnt), static_cast<void*>(c->gs), static_cast<void*>(c->dm),
                   static_cast<void*>(c->sp), static_cast<void*>(c->dlist), static_cast<void*>(c->dcount),
-                  static_cast<void*>(c->perm), static_cast<void*>(c->dstep), static_cast<void*>(c->order)})
+                  static_cast<void*>(c->perm), static_cast<void*>(c->dstep), static_cast<void*>(c->order),
+                  static_cast<void*>(c->qm)})
     if (p) hipFree(p);
   if (c->h_ctl) hipHostFree(c->h_ctl);
   for (hipEvent_t& e : c->ev)

@@ -116,6 +116,8 @@ struct cleo_ctx {
   // materialize_order() applies it for callers that need the drops in order.
   uint32_t* order = nullptr;
   bool order_live = false;
+  // per drop: condensed mass * xi before a condensation solve (the qcond recompute after motion)
+  double* qm = nullptr;
   uint64_t sp_slots = 0;  // slots per counter: max(ngbx, waves of a thread-per-drop launch)
   // per-drop Newton iteration record
   bool rec_niters = false;

@@ -30,6 +30,30 @@ __device__ __forceinline__ double tile_sum(double* win, uint32_t t_lo, uint32_t 
   }
   return acc;
 }
+// The same for two value sequences at once (each summed left to right on its own): half the window
+// each, one pass over the tile.
+template <class V1, class V2>
+__device__ __forceinline__ void tile_sum2(double* win, uint32_t t_lo, uint32_t t_hi, uint32_t lo, uint32_t len, V1 v1,
+                                          V2 v2, int lane, double& s1, double& s2) {
+  constexpr uint32_t kHalf = kSumWin / 2;
+  double a1 = 0.0, a2 = 0.0;
+  for (uint32_t w = t_lo; w < t_hi; w += kHalf) {
+    const uint32_t we = min(w + kHalf, t_hi);
+    for (uint32_t j = lane; j < we - w; j += kWave) {
+      win[j] = v1(w + j);
+      win[kHalf + j] = v2(w + j);
+    }
+    wave_sync();
+    const uint32_t b0 = max(lo, w), b1 = min(lo + len, we);
+    for (uint32_t k = b0; k < b1; ++k) {
+      a1 += win[k - w];
+      a2 += win[kHalf + k - w];
+    }
+    wave_sync();
+  }
+  s1 = a1;
+  s2 = a2;
+}
 // condensed water of drop k (condensate_mass * xi, superdrop_attrs.hpp:207-216)
 __device__ __forceinline__ double qcond_term(const SoA& s, const Solute& sol, DevCtl* ctl, uint32_t k) {
   bool bad = false;
@@ -1556,14 +1580,15 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_prep(CondFlatArg
     const uint32_t g0 = tile * kWave, g = g0 + static_cast<uint32_t>(lane);
     const bool valid = g < a.ngbx;
     double tot = 0.0;
-    if (a.fuse_qcond) {
+    const bool qcond_here = a.fuse_qcond && !a.qm;  // else k_cond_state recomputes it from qm
+    if (qcond_here) {
       const uint32_t t_lo = a.off[g0], t_hi = a.off[min(g0 + kWave, a.ngbx)];
       const uint32_t lo = valid ? a.off[g] : t_hi, hi = valid ? a.off[g + 1] : t_hi;
       tot = tile_sum(win, t_lo, t_hi, lo, hi - lo, [&](uint32_t k) { return qcond_term(a.s, a.sol, a.ctl, k); }, lane);
     }
     if (valid) {
       const double temp = a.st.temp[g], press = a.st.press[g], qv = a.st.qv[g];
-      if (a.fuse_qcond) {
+      if (qcond_here) {
         const double rho_cond = tot / a.st.vol[g] * dimless::R0cubed_VOL0;
         a.st.qc[g] = rho_cond / dry_air_density(press, temp, qv);
       }
@@ -1590,12 +1615,15 @@ __device__ __forceinline__ bool cond_drop(const CondFlatArgs& a, const NewtonCfg
   kohler_factors(msol, temp, a.sol, akoh, bkoh);
   SolveOut o{0, false, false, false, false};
   const double newr = solve_condensation(cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
+  // a deferred drop is solved again, from the same radius, and records everything then
   if (o.deferred) return false;
   bool bad = false;
   const double old_m = condensate_mass(r, msol, a.sol, &bad);
+  const double xi = static_cast<double>(a.s.xi[k]);
+  if (a.qm) a.qm[k] = old_m * xi;  // the drop's term of the qcond recompute
   const double rnew = fmax(newr, dryradius(msol, a.sol));
   a.s.r[k] = rnew;
-  a.dm[k] = (condensate_mass(rnew, msol, a.sol, &bad) - old_m) * static_cast<double>(a.s.xi[k]);
+  a.dm[k] = (condensate_mass(rnew, msol, a.sol, &bad) - old_m) * xi;
   if (bad) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, g, k);
   if (o.maxed) set_error(a.ctl, CLEO_ERR_NEWTON_MAXITERS, g, k);
   if (a.niters) a.niters[k] = o.iters;
@@ -1945,7 +1973,9 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_cond_queue(CondFlatAr
   }
 }
 
-// per gridbox: Σ dm in drop order and the thermodynamic feedback (condensation.cpp:126-137)
+// per gridbox: Σ dm in drop order and the thermodynamic feedback (condensation.cpp:126-137).
+// With a.qm, first the qcond recompute after motion from the drops' terms before the solve (the
+// same ordered sum and formula as k_cond_prep's; State is unchanged until here).
 __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_state(CondFlatArgs a) {
   __shared__ double win_all[4][kSumWin];
   double* win = win_all[threadIdx.x >> 6];
@@ -1956,10 +1986,22 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_state(CondFlatAr
     const bool valid = g < a.ngbx;
     const uint32_t t_lo = a.off[g0], t_hi = a.off[min(g0 + kWave, a.ngbx)];
     const uint32_t lo = valid ? a.off[g] : t_hi, n = valid ? a.nact[g] : 0u;
-    const double tot = tile_sum(win, t_lo, t_hi, lo, n, [&](uint32_t k) { return a.dm[k]; }, lane);
+    double qc_new = 0.0, tot = 0.0;
+    if (a.qm) {
+      double totq = 0.0;
+      tile_sum2(win, t_lo, t_hi, lo, n, [&](uint32_t k) { return a.qm[k]; }, [&](uint32_t k) { return a.dm[k]; },
+                lane, totq, tot);
+      if (valid) {
+        const double rho_cond = totq / a.st.vol[g] * dimless::R0cubed_VOL0;
+        qc_new = rho_cond / dry_air_density(a.st.press[g], a.st.temp[g], a.st.qv[g]);
+        if (!a.alter) a.st.qc[g] = qc_new;
+      }
+    } else {
+      tot = tile_sum(win, t_lo, t_hi, lo, n, [&](uint32_t k) { return a.dm[k]; }, lane);
+    }
     if (valid && a.alter) {
       const double temp = a.st.temp[g], press = a.st.press[g];
-      double qv = a.st.qv[g], qc = a.st.qc[g];
+      double qv = a.st.qv[g], qc = a.qm ? qc_new : a.st.qc[g];
       const double totrho = tot / a.st.vol[g] * dimless::R0cubed_VOL0;
       const double dq = totrho / dry_air_density(press, temp, qv);
       qv -= dq;
@@ -2122,6 +2164,9 @@ static void launch_gather_move(cleo_ctx* c, const NsortArgs& n, const MotionArgs
 void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& args) {
   if (args.ngbx == 0) return;
   CondFlatArgs a = args;
+  // the thread-per-drop solve records each drop's qcond term, so the qcond recompute after motion
+  // joins k_cond_state (the task ring keeps it in k_cond_prep)
+  a.qm = (a.fuse_qcond && c->tiles.cond_flat == 1) ? c->qm : nullptr;
   const uint32_t gw = c->tiles.cond_grid_waves;
   const uint64_t ntiles = (a.ngbx + kWave - 1) / kWave;
   hipLaunchKernelGGL(k_cond_prep, dim3(gbx_blocks(ntiles, 4, gw)), dim3(256), 0, c->stream, a);

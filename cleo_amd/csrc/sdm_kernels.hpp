@@ -116,6 +116,9 @@ struct CondFlatArgs {
   uint32_t warm_limit;
   double* gs;  // 2 per gridbox: saturation ratio, diffusion factor
   double* dm;  // per drop: condensed mass * xi
+  // non-null (with fuse_qcond): per drop the condensed mass * xi before the solve; k_cond_state
+  // recomputes the gridbox qcond from it instead of a separate pass in k_cond_prep
+  double* qm;
   StatSlots stat;  // ST_COND_DROPS .. ST_WARM_RESTARTS
   uint32_t* dlist;   // deferred drops, `dcap` per wave of k_cond_drops
   uint32_t* dcount;  // deferred drops per wave (null: no deferral)
