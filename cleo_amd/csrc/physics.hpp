@@ -568,7 +568,8 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
     k1 += 0xBB67AE85u;
   }
 }
-// stream 0: Fisher-Yates index for `iter`; stream 1: (phi_coll, phi_out) for pair `jj`
+// stream 0: Fisher-Yates indices, four swaps per block (item = swap / 4); stream 1: (phi_coll,
+// phi_out) for pair `jj`
 __device__ __forceinline__ void rng_block(uint64_t seed, uint64_t event, uint32_t stream, uint32_t gbx,
                                           uint32_t item, uint32_t out[4]) {
   out[0] = item;

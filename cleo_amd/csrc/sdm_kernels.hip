@@ -192,8 +192,18 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_qcond(QcondArgs a) {
 // CollArgs::order (slot -> position) and only the attributes of enacted pairs are written, in
 // place.  The reference moves every drop to its shuffled place; here the next sort moves each drop
 // once, to its sorted place in that order.
+// word t%4 of a block (selected without indexing the array by a lane-varying value)
+__device__ __forceinline__ uint32_t rng_word(const uint32_t r4[4], uint32_t t) {
+  const uint32_t w = t & 3u;
+  return w == 0u ? r4[0] : (w == 1u ? r4[1] : (w == 2u ? r4[2] : r4[3]));
+}
+// uniform u32 -> index in [0, iter]
+__device__ __forceinline__ uint32_t fy_scale(uint32_t u, uint32_t iter) {
+  return static_cast<uint32_t>((static_cast<uint64_t>(u) * (iter + 1ull)) >> 32);
+}
 // j_t of Fisher-Yates swap t < n-1 of gridbox g: uniform in [0, iter], iter = n-1-t
-// (shuffle.hpp:80-87), from the counter stream or the injected indices
+// (shuffle.hpp:80-87), from the counter stream (word t%4 of the block for item t/4, so one block
+// serves four consecutive swaps) or the injected indices
 __device__ __forceinline__ uint32_t fy_index(const CollArgs& a, uint32_t g, uint32_t base, uint32_t n, uint32_t t,
                                              uint64_t event) {
   const uint32_t iter = n - 1 - t;
@@ -206,8 +216,8 @@ __device__ __forceinline__ uint32_t fy_index(const CollArgs& a, uint32_t g, uint
     }
   } else {
     uint32_t r4[4];
-    rng_block(a.seed, event, 0u, g, iter, r4);
-    j = static_cast<uint32_t>((static_cast<uint64_t>(r4[0]) * (iter + 1ull)) >> 32);
+    rng_block(a.seed, event, 0u, g, t >> 2, r4);
+    j = fy_scale(rng_word(r4, t), iter);
   }
   return j;
 }
@@ -470,9 +480,17 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_coll_tiles(CollArgs a
     if (small) {
       for (uint32_t k = 0; k < n; k += 4) L.perm32[(loff + k) >> 2] = k | ((k + 1) << 8) | ((k + 2) << 16) | ((k + 3) << 24);
       uint8_t* p = pb + loff;
+      uint32_t r4[4] = {0u, 0u, 0u, 0u};
       for (uint32_t t = 0; t + 1 < n; ++t) {
         const uint32_t iter = n - 1 - t;
-        const uint32_t j = fy_index(a, g, base, n, t, event);
+        uint32_t j;
+        if (a.inj_j) {
+          j = fy_index(a, g, base, n, t, event);
+        } else {
+          // t is the same for every lane still shuffling, so the block is drawn once per 4 swaps
+          if ((t & 3u) == 0u) rng_block(a.seed, event, 0u, g, t >> 2, r4);
+          j = fy_scale(rng_word(r4, t), iter);
+        }
         const uint8_t pi = p[iter];
         p[iter] = p[j];
         p[j] = pi;

@@ -892,14 +892,15 @@ static bool coll_gbx(Ctx& c, uint32_t g, const cleo_coll_params& p, uint64_t eve
   /* shuffle_supers / fisher_yates_shuffle (shuffle.cpp:38-53, shuffle.hpp:80-87) */
   if (n >= 2) {
     uint64_t t = 0;
+    uint32_t r[4] = {0u, 0u, 0u, 0u};
     for (uint64_t iter = n - 1; iter > 0; --iter, ++t) {
       uint64_t j;
       if (c.rng_mode == 2) {
         j = c.inj_j[inj_j0 + t];
       } else {
-        uint32_t r[4];
-        rng_block(c.seed, event, 0u, g, static_cast<uint32_t>(iter), r);
-        j = (static_cast<uint64_t>(r[0]) * (iter + 1)) >> 32;
+        /* counter stream: word t%4 of the block for item t/4 */
+        if ((t & 3u) == 0u) rng_block(c.seed, event, 0u, g, static_cast<uint32_t>(t >> 2), r);
+        j = (static_cast<uint64_t>(r[t & 3u]) * (iter + 1)) >> 32;
       }
       std::swap(sub[iter], sub[j]);
     }
