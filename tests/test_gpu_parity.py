@@ -217,6 +217,57 @@ def test_graph_replay_collisions_only(gpu):
     assert_supers_match(a.download(), b.download(), exact_fp=True, what="graph colls")
 
 
+# Collisions leave the drops in place and record their shuffled order; with motion every second
+# microphysics step, condensation and collisions run while such an order is pending (it is applied
+# first), and the next sort applies the order the last collisions left.
+SUBSTEP_TILES = [None, (1, 1, 1, 1, 1, 64, 0, 64, 1, 6), (1, 1, 1, 1, 3, 0, 0, 0, 1, 0)]
+
+
+@pytest.mark.parametrize("tiles", SUBSTEP_TILES)
+def test_microphysics_substeps_between_motion(gpu, oracle, tiles):
+    _say(f"case: motion every 200 steps, condensation and collisions every 100, 4 coupled steps, tiles {tiles}")
+    pop = P.coupled(SEED, ndims=(8, 6, 4), mean=64)
+    g, o = _both(gpu, oracle, pop)
+    if tiles:
+        t = abi.Tiles(*tiles)
+        assert gpu.lib.cleo_set_tiles(g.h, abi.ctypes_byref(t)) == 0
+    s = sdm_params(cond=cond_params(), coll=coll_params(), motion=motion_params(interval=200))
+    for d in (g, o):
+        d.set_sdm(s)
+        d.rng_counter(5)
+    for step in range(4):
+        assert g.run_step(step * 200, (step + 1) * 200) == 0
+        assert o.run_step(step * 200, (step + 1) * 200) == 0
+    assert g.error()[0] == 0 and o.error()[0] == 0, (g.error(), o.error())
+    assert_supers_match(g.download(), o.download(), what="substeps")
+    assert_state_match(g.state(), o.state(), what="substeps")
+    assert np.array_equal(g.refs(), o.refs())
+    sg, so = g.stats(), o.stats()
+    for k in ("cond_drops", "coll_pairs", "coll_events", "nulls", "motion_drops"):
+        assert sg[k] == so[k], k
+
+
+@pytest.mark.parametrize("tiles", SUBSTEP_TILES)
+def test_graph_replay_with_pending_order(gpu, tiles):
+    _say(f"case: hipGraph replay of a step that ends with a pending collision order, tiles {tiles}")
+    pop = P.coupled(SEED, ndims=(8, 6, 4), mean=64)
+    a = make_domain(gpu, pop)
+    b = make_domain(gpu, pop)
+    s = sdm_params(cond=cond_params(), coll=coll_params(), motion=motion_params(interval=200))
+    for d in (a, b):
+        if tiles:
+            t = abi.Tiles(*tiles)
+            assert gpu.lib.cleo_set_tiles(d.h, abi.ctypes_byref(t)) == 0
+        d.set_sdm(s)
+        d.rng_counter(9)
+    for step in range(5):
+        assert a.run_step(step * 200, (step + 1) * 200) == 0
+    assert gpu.lib.cleo_run_steps_graph(b.h, 0, 200, 5) == 0
+    assert a.error()[0] == 0 and b.error()[0] == 0
+    assert_supers_match(a.download(), b.download(), exact_fp=True, what="graph pending order")
+    assert_state_match(a.state(), b.state(), rtol=0.0, what="graph pending order")
+
+
 def test_tiles_do_not_change_results(gpu):
     _say("case: waves-per-block tilings, looping grids and thread-per-drop variants give identical results")
     pop = P.coupled(SEED, ndims=(8, 6, 4), mean=64)
