@@ -240,6 +240,29 @@ int enqueue_motion(cleo_ctx* c, const cleo_motion_params& p, bool qcond) {
   return launch_check(c);
 }
 
+// MoveSupersInDomain::run_step followed at once by DoCondensation on the sorted drops, with the
+// condensation solve in the sort's gather (thread-per-drop motion and condensation, plain guess)
+int enqueue_motion_condensation(cleo_ctx* c, const cleo_motion_params& mp, const cleo_cond_params& cp) {
+  if (mp.interval == 0) return fail(c, CLEO_E_ARG, "motion interval 0");
+  if (cp.interval == 0) return fail(c, CLEO_E_ARG, "condensation interval 0");
+  if (!(cp.minsubdelt_s > 0.0)) return fail(c, CLEO_E_ARG, "MINSUBTSTEP must be positive");
+  MotionArgs a{};
+  a.s = c->buf[c->cur];
+  a.off = c->off;
+  a.grid = c->grid;
+  a.st = c->st;
+  a.delt = step2dimlesstime(mp.interval);
+  a.tv = mp.terminalv;
+  a.wpb = wpb_or1(c->tiles.motion_waves_per_block);
+  a.cnt = c->cnt;
+  a.ctl = c->ctl;
+  const CondArgs ca = cond_args(c, cp, true);
+  CondFlatArgs f{ca.s, ca.off, ca.nact, ca.st, ca.sol, ca.cfg, ca.ngbx, ca.alter, ca.fuse_qcond, ca.ctl, ca.niters,
+                 nullptr, ca.warm_limit, c->gs, c->dm};
+  motion_sort_condense(c, a, f);
+  return launch_check(c);
+}
+
 // SDMMethods::run_step(t_mdl, t_mdl_next) (sdmmethods.hpp:319-332): host loop over SDM sub-steps;
 // every gridbox follows the same schedule, so each process runs as one launch over all gridboxes.
 int enqueue_run_step(cleo_ctx* c, uint32_t t_mdl, uint32_t t_mdl_next) {
@@ -252,14 +275,20 @@ int enqueue_run_step(cleo_ctx* c, uint32_t t_mdl, uint32_t t_mdl_next) {
     const uint32_t next_move = mot ? next_of(t_sdm, p.motion.interval) : UMAX;
     const uint32_t t_sdm_next = std::min(t_mdl_next, next_move);
     bool pending_qcond = false;
+    bool cond_done = false;  // the condensation at t_sdm already ran inside the sort
     if (mot && t_sdm % p.motion.interval == 0) {
-      const int rc = enqueue_motion(c, p.motion, false);
+      // a condensation due at once is solved in the sort's gather (thread-per-drop variants,
+      // plain guess); otherwise motion and sort run alone and qcond is recomputed afterwards
+      const bool fuse = cond && t_sdm % p.cond.interval == 0 && c->tiles.motion_flat == 1 &&
+                        c->tiles.cond_flat == 1 && !p.cond.warm_start;
+      const int rc = fuse ? enqueue_motion_condensation(c, p.motion, p.cond) : enqueue_motion(c, p.motion, false);
       if (rc != CLEO_OK) return rc;
-      pending_qcond = true;
+      pending_qcond = !fuse;
+      cond_done = fuse;
     }
     bool any_coll = false;
     for (uint32_t subt = t_sdm; subt < t_sdm_next;) {
-      const bool on_cond = cond && subt % p.cond.interval == 0;
+      const bool on_cond = cond && subt % p.cond.interval == 0 && !(cond_done && subt == t_sdm);
       const bool on_coll = coll && subt % p.coll.interval == 0;
       if (on_cond) {
         const int rc = enqueue_condensation(c, p.cond, pending_qcond);

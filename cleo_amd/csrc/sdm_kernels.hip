@@ -1624,24 +1624,28 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_prep(CondFlatArg
 struct CondStats {
   uint32_t solved, iters, subs, restarts;
 };
-__device__ __forceinline__ bool cond_drop(const CondFlatArgs& a, const NewtonCfg& cfg, uint32_t g, uint64_t k,
-                                          double warm, CondStats& st) {
+// The solve of one active drop of gridbox g from its radius r and solute mass msol: Koehler
+// factors, Newton, new radius rnew, condensed-mass change * xi (dm) and condensed mass * xi before
+// the solve (qm, the drop's term of the qcond recompute).  Errors and the iteration count are
+// recorded under position k; xi() gives the multiplicity (read after the solve).  Returns false,
+// with nothing recorded, when cfg.budget ran out: the drop is solved again later from the same r.
+template <class Xi>
+__device__ __forceinline__ bool cond_solve(const CondFlatArgs& a, const NewtonCfg& cfg, uint32_t g, uint64_t k, double r,
+                                           double msol, Xi xi_of, double warm, CondStats& st, double& rnew, double& dm,
+                                           double& qm) {
   const double s_ratio = a.gs[2 * g], ffactor = a.gs[2 * g + 1];
   const double temp = a.st.temp[g];
-  const double r = a.s.r[k], msol = a.s.msol[k];
   double akoh, bkoh;
   kohler_factors(msol, temp, a.sol, akoh, bkoh);
   SolveOut o{0, false, false, false, false};
   const double newr = solve_condensation(cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
-  // a deferred drop is solved again, from the same radius, and records everything then
   if (o.deferred) return false;
   bool bad = false;
   const double old_m = condensate_mass(r, msol, a.sol, &bad);
-  const double xi = static_cast<double>(a.s.xi[k]);
-  if (a.qm) a.qm[k] = old_m * xi;  // the drop's term of the qcond recompute
-  const double rnew = fmax(newr, dryradius(msol, a.sol));
-  a.s.r[k] = rnew;
-  a.dm[k] = (condensate_mass(rnew, msol, a.sol, &bad) - old_m) * xi;
+  const double xi = static_cast<double>(xi_of());
+  qm = old_m * xi;
+  rnew = fmax(newr, dryradius(msol, a.sol));
+  dm = (condensate_mass(rnew, msol, a.sol, &bad) - old_m) * xi;
   if (bad) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, g, k);
   if (o.maxed) set_error(a.ctl, CLEO_ERR_NEWTON_MAXITERS, g, k);
   if (a.niters) a.niters[k] = o.iters;
@@ -1649,6 +1653,17 @@ __device__ __forceinline__ bool cond_drop(const CondFlatArgs& a, const NewtonCfg
   st.subs += o.substep ? 1u : 0u;
   st.restarts += o.restarted ? 1u : 0u;
   st.solved += 1;
+  return true;
+}
+// drop k of the store: its solve, the new radius in place, dm[k] and qm[k]
+__device__ __forceinline__ bool cond_drop(const CondFlatArgs& a, const NewtonCfg& cfg, uint32_t g, uint64_t k,
+                                          double warm, CondStats& st) {
+  double rnew, dm, qm;
+  if (!cond_solve(a, cfg, g, k, a.s.r[k], a.s.msol[k], [&] { return a.s.xi[k]; }, warm, st, rnew, dm, qm))
+    return false;
+  a.s.r[k] = rnew;
+  a.dm[k] = dm;
+  if (a.qm) a.qm[k] = qm;
   return true;
 }
 __device__ __forceinline__ void put_cond_stats(const StatSlots& sl, CondStats st) {
@@ -1754,6 +1769,70 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_deferred(CondFla
     const double warm = a.warm ? a.warm[k] : __longlong_as_double(0x7FF8000000000000ll);
     cond_drop(a, nobudget, a.s.gbx[k], k, warm, st);
   }
+  put_cond_stats(a.stat, st);
+}
+
+// Condensation in the sort's gather, for a condensation that follows motion at once: destination
+// q takes the drop at position perm[q] of the unsorted store and, when it stays in the domain,
+// solves it before writing its record (with the new radius) and dm[q], qm[q] to the sorted store.
+// The solve does not depend on the order of the drops; the per-gridbox sums run afterwards on the
+// sorted store (k_cond_state).  Drops that outlive cfg.budget iterations are written unsolved and
+// listed by their sorted position for k_cond_deferred, as in k_cond_drops.  The memory traffic of
+// the gather and the FP64 arithmetic of the solves overlap in one pass.
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_sort_gather_cond(NsortArgs n, CondFlatArgs a) {
+  const uint64_t nsrc = n.off[n.grid.ngbx];
+  const int lane = lane_id();
+  CondStats st{0, 0, 0, 0};
+  uint32_t ndef = 0;
+  uint32_t* mylist = a.dlist + static_cast<uint64_t>(wave_index()) * a.dcap;
+  NewtonCfg nobudget = a.cfg;
+  nobudget.budget = 0;
+  const double nowarm = __longlong_as_double(0x7FF8000000000000ll);
+  // block-uniform chunks keep every wave in the loop together for the ballots below
+  for (uint64_t q0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x; q0 < nsrc;
+       q0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t q = q0 + threadIdx.x;
+    const bool inq = q < nsrc;
+    const uint32_t p = inq ? n.perm[q] : 0u;
+    const bool okp = inq && p < nsrc;
+    if (inq && !okp) set_error(n.ctl, CLEO_ERR_SORT_NONLOCAL, kOOB, q);
+    const uint32_t sp = okp ? p : 0u;
+    const uint32_t g = n.src.gbx[sp];
+    const double c3 = n.src.c3[sp], c1 = n.src.c1[sp], c2 = n.src.c2[sp];
+    const uint64_t xi = n.src.xi[sp], id = n.src.id[sp];
+    const double r = n.src.r[sp], msol = n.src.msol[sp];
+    const bool active = okp && g < a.ngbx;
+    double rnew = r, dm = 0.0, qm = 0.0;
+    bool solved = false;
+    if (active) solved = cond_solve(a, a.cfg, g, q, r, msol, [&] { return xi; }, nowarm, st, rnew, dm, qm);
+    const bool deferred = active && !solved;
+    const uint64_t m = __ballot(deferred);
+    if (m) {
+      const uint32_t idx = ndef + static_cast<uint32_t>(__popcll(m & lanemask_lt()));
+      if (deferred) {
+        if (idx < a.dcap)
+          mylist[idx] = static_cast<uint32_t>(q);
+        else  // the segment is full: solve it here
+          solved = cond_solve(a, nobudget, g, q, r, msol, [&] { return xi; }, nowarm, st, rnew, dm, qm);
+      }
+      ndef = min(a.dcap, ndef + static_cast<uint32_t>(__popcll(m)));
+    }
+    if (okp) {
+      n.dst.gbx[q] = g;
+      n.dst.c3[q] = c3;
+      n.dst.c1[q] = c1;
+      n.dst.c2[q] = c2;
+      n.dst.xi[q] = xi;
+      n.dst.r[q] = solved ? rnew : r;
+      n.dst.msol[q] = msol;
+      n.dst.id[q] = id;
+    }
+    if (solved) {
+      a.dm[q] = dm;
+      if (a.qm) a.qm[q] = qm;
+    }
+  }
+  if (a.dcount && lane == 0) a.dcount[wave_index()] = ndef;
   put_cond_stats(a.stat, st);
 }
 
@@ -2220,6 +2299,49 @@ void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& args) {
     kev_mark(c, 1);
     launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
   }
+  hipLaunchKernelGGL(k_cond_state, dim3(gbx_blocks(ntiles, 4, gw)), dim3(256), 0, c->stream, a);
+}
+
+// Motion, the neighbour sort and a condensation that follows at once, with the condensation solve
+// in the sort's gather (k_sort_gather_cond): motion (thread per drop), the sort tables and ranks,
+// the saturation ratios (State only), the gather with the solves, the commit of the sorted store,
+// the deferred drops, then the ordered sums and the State update (with the qcond recompute).
+// `cond.s` must be the buffer the sort writes (buf[1 - cur]).
+void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatArgs& cond) {
+  const uint32_t ngbx = c->grid.ngbx;
+  if (ngbx == 0) return;
+  MotionArgs m = margs;
+  m.step = c->dstep;
+  launch_motion_flat(c, m);
+  const int src = c->cur, dst = 1 - c->cur;
+  NsortArgs n = nsort_args(c, c->buf[src], c->buf[dst], nullptr);
+  n.step = c->dstep;
+  nsort_tables(c, n, nullptr);
+  hipLaunchKernelGGL(k_nsort_perm, dim3(gbx_blocks(ngbx, n.wpb, c->tiles.motion_grid_waves)), dim3(64 * n.wpb), 0,
+                     c->stream, n);
+  CondFlatArgs a = cond;
+  a.s = c->buf[dst];
+  a.qm = c->qm;  // the qcond recompute joins k_cond_state
+  const uint32_t gw = c->tiles.cond_grid_waves;
+  const uint64_t ntiles = (ngbx + kWave - 1) / kWave;
+  hipLaunchKernelGGL(k_cond_prep, dim3(gbx_blocks(ntiles, 4, gw)), dim3(256), 0, c->stream, a);
+  const unsigned nb = drop_blocks(c->ntot, gw);
+  a.stat = stat_slots(c, 4ull * nb);
+  const uint32_t budget = c->tiles.cond_defer;
+  a.cfg.budget = budget;
+  const uint64_t per_wave = (c->ntot + 4ull * nb - 1) / (4ull * nb);
+  a.dcap = budget ? static_cast<uint32_t>(per_wave / 4 + 16) : 0u;
+  a.dlist = c->dlist;
+  a.dcount = budget ? c->dcount : nullptr;
+  hipLaunchKernelGGL(k_sort_gather_cond, dim3(nb), dim3(256), 0, c->stream, n, a);
+  if (budget) {
+    launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
+    hipLaunchKernelGGL(k_cond_deferred, dim3(nb), dim3(256), 0, c->stream, a);
+  }
+  launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
+  hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, n, c->ntot);
+  hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, static_cast<const uint32_t*>(nullptr));
+  nsort_commit(c, dst);
   hipLaunchKernelGGL(k_cond_state, dim3(gbx_blocks(ntiles, 4, gw)), dim3(256), 0, c->stream, a);
 }
 
