@@ -47,6 +47,9 @@ BYTES = {
     # counting pass of the motion in the sort's gather: read coords, radius, sdgbxindex; write the
     # step | bounds, winds, counts
     "k_motion_keys": (37, 6 * 8 + 6 * 8 + 28 * 4),
+    # the sort's gather with the condensation solves: read the source position and the 60-byte
+    # record; write the record, dm and qm | saturation ratio, diffusion factor, temperature
+    "k_sort_gather_cond": (140, 2 * 8 + 8),
 }
 
 
@@ -224,6 +227,13 @@ def run_gpu(args):
         ms = C.c_float()
         d.check(L.cleo_time_kernel(d.h, stage, 3, C.byref(ms)), "time_kernel")
         kernel_ms[stage_kernel(stage, tiles)] = float(ms.value)
+    if tiles.cond_flat == 1 and tiles.motion_flat == 1:
+        # the step solves condensation inside the sort's gather (k_sort_gather_cond) instead of
+        # in the separate per-drop kernel
+        del kernel_ms[stage_kernel(0, tiles)]
+        ms = C.c_float()
+        d.check(L.cleo_time_kernel(d.h, 3, 3, C.byref(ms)), "time_kernel fused")
+        kernel_ms["k_sort_gather_cond"] = float(ms.value)
     ngbx = int(np.prod(ndims))
     nd = ndomain(d)
     d.check(L.cleo_synchronize(d.h), "final sync")

@@ -259,7 +259,7 @@ int enqueue_motion_condensation(cleo_ctx* c, const cleo_motion_params& mp, const
   const CondArgs ca = cond_args(c, cp, true);
   CondFlatArgs f{ca.s, ca.off, ca.nact, ca.st, ca.sol, ca.cfg, ca.ngbx, ca.alter, ca.fuse_qcond, ca.ctl, ca.niters,
                  nullptr, ca.warm_limit, c->gs, c->dm};
-  motion_sort_condense(c, a, f);
+  motion_sort_condense(c, a, f, c->buf[1 - c->cur], 1 - c->cur);
   return launch_check(c);
 }
 
@@ -912,7 +912,7 @@ static int time_stage(cleo_ctx* c, int kern, const cleo_tiles& t, int reps, bool
   float best = 1e30f, total = 0.f;
   // motion is timed with its sort, into a scratch store (the current one stays untouched)
   SoA sorted{};
-  if (kern == 2 && alloc_soa(sorted, c->ntot) != hipSuccess) {
+  if (kern >= 2 && alloc_soa(sorted, c->ntot) != hipSuccess) {
     free_soa(sorted);
     c->tiles = tiles0;
     return fail(c, CLEO_E_HIP, "stage timing buffer");
@@ -961,6 +961,21 @@ static int time_stage(cleo_ctx* c, int kern, const cleo_tiles& t, int reps, bool
       a.inj_op = c->inj_op;
       a.big = c->big_scratch;
       launch_collisions(c, a);
+    } else if (kern == 3) {  // motion, sort and condensation with the solves in the sort's gather
+      MotionArgs a{};
+      a.s = c->buf[spare];
+      a.off = c->off;
+      a.grid = c->grid;
+      a.st = c->st;
+      a.delt = step2dimlesstime(mp.interval);
+      a.tv = mp.terminalv;
+      a.wpb = t.motion_waves_per_block ? t.motion_waves_per_block : 1;
+      a.cnt = c->cnt;
+      a.ctl = c->ctl;
+      const CondArgs ca = cond_args(c, cp, true);
+      CondFlatArgs f{ca.s, ca.off, ca.nact, ca.st, ca.sol, ca.cfg, ca.ngbx, ca.alter, ca.fuse_qcond, ca.ctl, nullptr,
+                     nullptr, ca.warm_limit, c->gs, c->dm};
+      motion_sort_condense(c, a, f, sorted, -1);
     } else {
       MotionArgs a{};
       a.s = c->buf[spare];
@@ -1076,13 +1091,33 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
       }
     }
   }
+  // thread-per-drop motion and condensation: a condensation that follows motion runs inside the
+  // sort's gather, so its grid and deferral budget are chosen on that fused pass
+  if (best_t.cond_flat == 1 && best_t.motion_flat == 1) {
+    float best = 1e30f;
+    const cleo_tiles base_t = best_t;
+    for (uint32_t gw : {16384u, 32768u, 65536u}) {
+      for (uint32_t budget : {6u, 8u, 12u}) {
+        cleo_tiles t = base_t;
+        t.cond_grid_waves = gw;
+        t.cond_defer = budget;
+        float tmin = 0.f;
+        const int rc = time_stage(c, 3, t, 2, false, &tmin, nullptr);
+        if (rc != CLEO_OK) return rc;
+        if (tmin < best) {
+          best = tmin;
+          best_t = t;
+        }
+      }
+    }
+  }
   c->tiles = best_t;
   if (chosen) *chosen = c->tiles;
   return CLEO_OK;
 }
 
 int cleo_time_kernel(cleo_ctx* c, int32_t stage, uint32_t reps, float* mean_ms) {
-  if (!c || !mean_ms || stage < 0 || stage > 2 || reps == 0) return CLEO_E_ARG;
+  if (!c || !mean_ms || stage < 0 || stage > 3 || reps == 0) return CLEO_E_ARG;
   if (c->ntot == 0) return fail(c, CLEO_E_STATE, "no superdroplets resident");
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return time_stage(c, stage, c->tiles, static_cast<int>(reps), true, nullptr, mean_ms);

@@ -2306,21 +2306,25 @@ void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& args) {
 // in the sort's gather (k_sort_gather_cond): motion (thread per drop), the sort tables and ranks,
 // the saturation ratios (State only), the gather with the solves, the commit of the sorted store,
 // the deferred drops, then the ordered sums and the State update (with the qcond recompute).
-// `cond.s` must be the buffer the sort writes (buf[1 - cur]).
-void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatArgs& cond) {
+// margs.s is the store motion reads, `dst` the sorted store; commit_dst >= 0: that buffer becomes
+// current (else, for timing, the refs stay those of the unsorted store).
+void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatArgs& cond, const SoA& dst,
+                          int commit_dst) {
   const uint32_t ngbx = c->grid.ngbx;
   if (ngbx == 0) return;
   MotionArgs m = margs;
   m.step = c->dstep;
+  const bool on = c->kev_on;
+  c->kev_on = false;  // the main kernel of this sequence is the gather with the solves
   launch_motion_flat(c, m);
-  const int src = c->cur, dst = 1 - c->cur;
-  NsortArgs n = nsort_args(c, c->buf[src], c->buf[dst], nullptr);
+  c->kev_on = on;
+  NsortArgs n = nsort_args(c, m.s, dst, nullptr);
   n.step = c->dstep;
   nsort_tables(c, n, nullptr);
   hipLaunchKernelGGL(k_nsort_perm, dim3(gbx_blocks(ngbx, n.wpb, c->tiles.motion_grid_waves)), dim3(64 * n.wpb), 0,
                      c->stream, n);
   CondFlatArgs a = cond;
-  a.s = c->buf[dst];
+  a.s = dst;
   a.qm = c->qm;  // the qcond recompute joins k_cond_state
   const uint32_t gw = c->tiles.cond_grid_waves;
   const uint64_t ntiles = (ngbx + kWave - 1) / kWave;
@@ -2333,7 +2337,9 @@ void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatAr
   a.dcap = budget ? static_cast<uint32_t>(per_wave / 4 + 16) : 0u;
   a.dlist = c->dlist;
   a.dcount = budget ? c->dcount : nullptr;
+  kev_mark(c, 0);
   hipLaunchKernelGGL(k_sort_gather_cond, dim3(nb), dim3(256), 0, c->stream, n, a);
+  kev_mark(c, 1);
   if (budget) {
     launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
     hipLaunchKernelGGL(k_cond_deferred, dim3(nb), dim3(256), 0, c->stream, a);
@@ -2341,7 +2347,7 @@ void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatAr
   launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
   hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, n, c->ntot);
   hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, static_cast<const uint32_t*>(nullptr));
-  nsort_commit(c, dst);
+  if (commit_dst >= 0) nsort_commit(c, commit_dst);
   hipLaunchKernelGGL(k_cond_state, dim3(gbx_blocks(ntiles, 4, gw)), dim3(256), 0, c->stream, a);
 }
 

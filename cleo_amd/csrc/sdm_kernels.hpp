@@ -155,8 +155,10 @@ void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool fr
 // the neighbour sort of `src` (after motion) into `dst`; commit_dst >= 0: that buffer becomes current
 void neighbour_sort_to(cleo_ctx* c, const SoA& src, const SoA& dst, int commit_dst, const uint8_t* step = nullptr);
 // motion + neighbour sort + a condensation that follows at once, the solve in the sort's gather
-// (thread-per-drop motion and condensation, plain guess); cond.s is replaced by the sorted store
-void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatArgs& cond);
+// (thread-per-drop motion and condensation, plain guess): margs.s is read, the moved, sorted and
+// condensed drops go to dst; commit_dst >= 0: that buffer becomes current
+void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatArgs& cond, const SoA& dst,
+                          int commit_dst);
 // motion + neighbour sort with the motion arithmetic in the sort's gather (motion_flat 3): the
 // store args.s is left unchanged, the moved and sorted drops go to dst
 void motion_sort_gather(cleo_ctx* c, const MotionArgs& args, const SoA& dst, int commit_dst);
