@@ -241,7 +241,7 @@ int enqueue_motion(cleo_ctx* c, const cleo_motion_params& p, bool qcond) {
 }
 
 // MoveSupersInDomain::run_step followed at once by DoCondensation on the sorted drops, with the
-// condensation solve in the sort's gather (thread-per-drop motion and condensation, plain guess)
+// condensation solve in the sort's gather (thread-per-drop motion and condensation)
 int enqueue_motion_condensation(cleo_ctx* c, const cleo_motion_params& mp, const cleo_cond_params& cp) {
   if (mp.interval == 0) return fail(c, CLEO_E_ARG, "motion interval 0");
   if (cp.interval == 0) return fail(c, CLEO_E_ARG, "condensation interval 0");
@@ -259,6 +259,13 @@ int enqueue_motion_condensation(cleo_ctx* c, const cleo_motion_params& mp, const
   const CondArgs ca = cond_args(c, cp, true);
   CondFlatArgs f{ca.s, ca.off, ca.nact, ca.st, ca.sol, ca.cfg, ca.ngbx, ca.alter, ca.fuse_qcond, ca.ctl, ca.niters,
                  nullptr, ca.warm_limit, c->gs, c->dm};
+  if (cp.warm_start) {  // learned guesses, evaluated on the moved drops before the gather
+    const int rc = mlp_check_ready(c);
+    if (rc != CLEO_OK) return rc;
+    f.mlp_inline = 1;
+    f.W = mlp_weights(c);
+    f.warm_buf = mlp_warm_buffer(c);
+  }
   motion_sort_condense(c, a, f, c->buf[1 - c->cur], 1 - c->cur);
   return launch_check(c);
 }
@@ -277,10 +284,11 @@ int enqueue_run_step(cleo_ctx* c, uint32_t t_mdl, uint32_t t_mdl_next) {
     bool pending_qcond = false;
     bool cond_done = false;  // the condensation at t_sdm already ran inside the sort
     if (mot && t_sdm % p.motion.interval == 0) {
-      // a condensation due at once is solved in the sort's gather (thread-per-drop variants,
-      // plain guess); otherwise motion and sort run alone and qcond is recomputed afterwards
+      // a condensation due at once is solved in the sort's gather (thread-per-drop variants; not
+      // while guesses are recorded, whose positions are those of the sorted store); otherwise
+      // motion and sort run alone and qcond is recomputed afterwards
       const bool fuse = cond && t_sdm % p.cond.interval == 0 && c->tiles.motion_flat == 1 &&
-                        c->tiles.cond_flat == 1 && !p.cond.warm_start;
+                        c->tiles.cond_flat == 1 && !(p.cond.warm_start && c->rec_guess);
       const int rc = fuse ? enqueue_motion_condensation(c, p.motion, p.cond) : enqueue_motion(c, p.motion, false);
       if (rc != CLEO_OK) return rc;
       pending_qcond = !fuse;

@@ -1699,7 +1699,7 @@ __global__ void __launch_bounds__(256) k_cond_guess(CondFlatArgs a) {
     double r = 0.0;
     if (k < ndom) {
       g = a.s.gbx[k];
-      valid = g < a.ngbx && k < a.off[g] + a.nact[g];
+      valid = g < a.ngbx && (a.unsorted || k < a.off[g] + a.nact[g]);
       if (valid) {
         r = a.s.r[k];
         drop_features(r, a.s.msol[k], a.gs[2 * g], a.st.temp[g], a.st.press[g], a.cfg.delt, f);
@@ -1802,9 +1802,10 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_sort_gather_cond(Nsor
     const uint64_t xi = n.src.xi[sp], id = n.src.id[sp];
     const double r = n.src.r[sp], msol = n.src.msol[sp];
     const bool active = okp && g < a.ngbx;
+    const double warm = a.warm ? a.warm[sp] : nowarm;  // learned guesses by source position
     double rnew = r, dm = 0.0, qm = 0.0;
     bool solved = false;
-    if (active) solved = cond_solve(a, a.cfg, g, q, r, msol, [&] { return xi; }, nowarm, st, rnew, dm, qm);
+    if (active) solved = cond_solve(a, a.cfg, g, q, r, msol, [&] { return xi; }, warm, st, rnew, dm, qm);
     const bool deferred = active && !solved;
     const uint64_t m = __ballot(deferred);
     if (m) {
@@ -1813,7 +1814,7 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_sort_gather_cond(Nsor
         if (idx < a.dcap)
           mylist[idx] = static_cast<uint32_t>(q);
         else  // the segment is full: solve it here
-          solved = cond_solve(a, nobudget, g, q, r, msol, [&] { return xi; }, nowarm, st, rnew, dm, qm);
+          solved = cond_solve(a, nobudget, g, q, r, msol, [&] { return xi; }, warm, st, rnew, dm, qm);
       }
       ndef = min(a.dcap, ndef + static_cast<uint32_t>(__popcll(m)));
     }
@@ -2329,9 +2330,19 @@ void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatAr
   const uint32_t gw = c->tiles.cond_grid_waves;
   const uint64_t ntiles = (ngbx + kWave - 1) / kWave;
   hipLaunchKernelGGL(k_cond_prep, dim3(gbx_blocks(ntiles, 4, gw)), dim3(256), 0, c->stream, a);
+  if (a.mlp_inline) {  // learned guesses of the drops where motion left them
+    CondFlatArgs ga = a;
+    ga.s = m.s;
+    ga.unsorted = 1;
+    ga.guess_out = nullptr;
+    hipLaunchKernelGGL(k_cond_guess, dim3(drop_blocks(c->ntot, gw ? gw : 32768)), dim3(256), 0, c->stream, ga);
+    a.warm = a.warm_buf;
+  }
   const unsigned nb = drop_blocks(c->ntot, gw);
   a.stat = stat_slots(c, 4ull * nb);
-  const uint32_t budget = c->tiles.cond_defer;
+  // a deferred drop is solved again by its sorted position, which the guess buffer is not
+  // indexed by: with learned guesses every drop is solved in the gather
+  const uint32_t budget = a.warm ? 0u : c->tiles.cond_defer;
   a.cfg.budget = budget;
   const uint64_t per_wave = (c->ntot + 4ull * nb - 1) / (4ull * nb);
   a.dcap = budget ? static_cast<uint32_t>(per_wave / 4 + 16) : 0u;

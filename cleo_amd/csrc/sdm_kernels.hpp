@@ -129,6 +129,9 @@ struct CondFlatArgs {
   MLPWeights W;
   double* warm_buf;
   double* guess_out;  // non-null: the guesses are recorded here too (array position order)
+  // k_cond_guess on the store as motion left it (before the sort): a drop is active when its new
+  // gridbox is in the domain (the refs still describe the unsorted store)
+  int32_t unsorted;
 };
 
 // launchers (sdm_kernels.hip); all enqueue on ctx->stream
@@ -155,7 +158,7 @@ void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool fr
 // the neighbour sort of `src` (after motion) into `dst`; commit_dst >= 0: that buffer becomes current
 void neighbour_sort_to(cleo_ctx* c, const SoA& src, const SoA& dst, int commit_dst, const uint8_t* step = nullptr);
 // motion + neighbour sort + a condensation that follows at once, the solve in the sort's gather
-// (thread-per-drop motion and condensation, plain guess): margs.s is read, the moved, sorted and
+// (thread-per-drop motion and condensation): margs.s is read, the moved, sorted and
 // condensed drops go to dst; commit_dst >= 0: that buffer becomes current
 void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatArgs& cond, const SoA& dst,
                           int commit_dst);

@@ -284,7 +284,7 @@ int cleo_autotune(cleo_ctx* ctx, cleo_tiles* chosen);
    `reps` launches with the current tiling, on a copy of the resident drops (drops and State stay
    unchanged): stage 0 condensation (the per-drop kernel), 1 collisions, 2 motion, 3 motion + sort
    + condensation with the solves in the sort's gather (the gather kernel; the path cleo_run_step
-   takes when condensation follows motion with thread-per-drop variants and the plain guess). */
+   takes when condensation follows motion with thread-per-drop variants). */
 int cleo_time_kernel(cleo_ctx* ctx, int32_t stage, uint32_t reps, float* mean_ms);
 
 /* ---- learned warm start ------------------------------------------------------------- */

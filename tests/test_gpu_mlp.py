@@ -134,3 +134,34 @@ def test_training_and_warm_start(gpu, oracle, variant, defer):
     it_p = sp["newton_iters"] / sp["cond_drops"]
     _say(f"  Newton iterations per drop: plain {it_p:.3f}, learned {it_w:.3f}, restarts {sw['warm_restarts']}")
     assert it_w < it_p
+
+
+def test_fused_warm_start_matches_unfused(gpu):
+    _say("case: learned guesses in the sort's gather (motion + condensation fused) vs the separate passes")
+    pop = P.coupled(SEED, ndims=(8, 6, 4), mean=64)
+    rng = np.random.default_rng(5)
+    blob = _blob(rng)
+    outs, stats = [], []
+    # motion variant 1 fuses the condensation into the sort's gather; variant 0 keeps them apart
+    for motion_flat, defer in ((1, 8), (0, 8), (0, 0)):
+        d = make_domain(gpu, pop)
+        t = abi.Tiles(1, 1, 1, 1, motion_flat, 64, 0, 64, 1, defer)
+        assert gpu.lib.cleo_set_tiles(d.h, abi.ctypes_byref(t)) == 0
+        assert gpu.lib.cleo_mlp_set_weights(d.h, _fp(blob)) == 0
+        s = abi.SDMParams()
+        s.enable_cond, s.enable_colls, s.enable_motion = 1, 1, 1
+        s.cond = cond_params(warm=1, warm_iters=10)
+        s.coll = abi.CollParams(100, abi.PROB_LONG, 1.0, abi.ENACT_COALBURE, 5.0, 0.85)
+        s.motion = abi.MotionParams(100, abi.TV_SIMMEL)
+        d.set_sdm(s)
+        d.rng_counter(13)
+        for step in range(3):
+            assert d.run_step(step * 100, (step + 1) * 100) == 0
+        assert d.error()[0] == 0, d.error()
+        outs.append(d.download())
+        stats.append(d.stats())
+    for o in outs[1:]:
+        assert_supers_match(outs[0], o, exact_fp=True, what="fused warm start")
+    for st in stats[1:]:
+        assert st["newton_iters"] == stats[0]["newton_iters"] and st["warm_restarts"] == stats[0]["warm_restarts"]
+    _say(f"  Newton iterations {stats[0]['newton_iters']}, warm restarts {stats[0]['warm_restarts']}")
