@@ -87,12 +87,40 @@ __device__ __forceinline__ void record_error(DevCtl* ctl, int code, uint32_t gbx
   }
 }
 
+// ---- reciprocals from the hardware estimates -------------------------------------------------
+// 1/x for finite nonzero normal x: v_rcp_f64 and one Newton-Raphson step, within about an ulp.
+__device__ __forceinline__ double rcp_nr(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+// 1/sqrt(x): v_rsq_f64 and two Newton-Raphson steps y <- y + y/2 (1 - x y^2), within an ulp or
+// two of 1.0 / sqrt(x) (itself two roundings) for positive finite x; 0 and +inf give the
+// estimate's own inf and 0, negative or NaN x gives NaN, as 1.0 / sqrt(x) does.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  const double y0 = __builtin_amdgcn_rsq(x);
+  double y = y0;
+  for (int i = 0; i < 2; ++i) {
+    const double e = fma(-(x * y), y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return (x > 0.0 && x < __builtin_huge_val()) ? y : y0;
+}
+
 // ---- superdrop attributes: superdrop_attrs.{hpp,cpp} ---------------------------------------
 __device__ __forceinline__ double rcubed(double r) { return r * r * r; }
 __device__ __forceinline__ double dryradius(double msol, const Solute& s) {
   constexpr double vconst = 3.0 / (4.0 * kPi);
   const double dryrcubed = vconst * msol / s.rho_sol;
   return cbrt(dryrcubed);  // pow(x, 1.0 / 3.0): the exponent is 1/3 rounded; cbrt is within 1 ulp of it
+}
+// fmax(r, dryradius(msol)) of change_radius (superdrop_attrs.cpp:35-45).  The cube root is taken
+// only when r^3 is not clearly above the dry radius cubed (1e-9 relative, far beyond the rounding
+// of either side): for every other drop the maximum is r itself.
+__device__ __forceinline__ double at_least_dry(double r, double msol, const Solute& s) {
+  constexpr double vconst = 3.0 / (4.0 * kPi);
+  const double dry3 = vconst * msol * rcp_nr(s.rho_sol);
+  if (r * r * r > dry3 * (1.0 + 1e-9)) return r;
+  return fmax(r, dryradius(msol, s));
 }
 __device__ __forceinline__ double drop_mass(double r, double msol, const Solute& s) {
   constexpr double massconst(4.0 / 3.0 * kPi * dimless::Rho_l);
@@ -141,16 +169,29 @@ __device__ __forceinline__ double diffusion_factor(double press, double temp, do
   const double fdl = TEMP / (DIFFUSE_V * PSAT) / dimless::F0;
   return dimless::Rho_l * (fkl + fdl);
 }
-// x^A and x^B from one logarithm: exp(A log x) is within ~|A log x| ulp of pow(x, A) (a few 1e-15
-// relative here), and the factor enters as 1/(a + b) next to 1.
-__device__ __forceinline__ double ventilation_factor(double radius) {
+// ffactor / ventilation_factor(radius) (impliciteuler.cpp:38, thermodynamic_equations.cpp:132-145)
+// and its reciprocal with two divisions instead of five.  The reference's factor is
+// min(1 + 1/(a + b), 20) with a = 1/(C1 x^A), b = 1/(C2 x^B), x the radius in metres.  With
+// u = C1 x^A and v = C2 x^B, 1 + 1/(1/u + 1/v) = 1 + uv/(u + v), so below the cap
+// ffactor / vent = ffactor (u + v) / (u + v + uv).  x^A and x^B come from one logarithm:
+// exp(A log x) is within ~|A log x| ulp of pow(x, A) (a few 1e-15 relative here), and both
+// results are within a few ulp of the reference's quotients.
+__device__ __forceinline__ void ffactor_over_vent(double ffactor, double radius, double& ffv, double& inv_ffv) {
   constexpr double vent_factor_max = 20;
   constexpr double C1 = 6.954e+7, A = 1.963, C2 = 1.069e+3, B = 0.702;
   const double lx = log(radius * dimless::R0);
-  const double a = 1.0 / (C1 * exp(A * lx));
-  const double b = 1.0 / (C2 * exp(B * lx));
-  const double vent = 1.0 + 1.0 / (a + b);
-  return fmin(vent, vent_factor_max);
+  const double u = C1 * exp(A * lx), v = C2 * exp(B * lx);
+  const double sum = u + v, prod = u * v;
+  double num = ffactor * sum, den = sum + prod;
+  if (prod > (vent_factor_max - 1.0) * sum) {
+    num = ffactor;
+    den = vent_factor_max;
+  } else if (!(sum > 0.0)) {  // radius 0: 1/u and 1/v infinite, factor 1
+    num = ffactor;
+    den = 1.0;
+  }
+  ffv = num / den;
+  inv_ffv = den / num;
 }
 __device__ __forceinline__ void kohler_factors(double msol, double temp, const Solute& s, double& akoh,
                                                double& bkoh) {
@@ -185,12 +226,11 @@ struct NewtonCfg {
 };
 // g(z) (impliciteuler.cpp:270-281) and dg/dz (impliciteuler.cpp:291-301) at the same z.  The
 // reference's quotients akoh/r, bkoh/r^3, subdelt/(z ffactor) and rprev/r are products with one
-// reciprocal of r (and 1/ffactor per drop): one FP64 division per evaluation instead of six, each
-// term within a few ulp of the reference's.
+// reciprocal square root of z (and 1/ffactor per drop): no FP64 division or square root per
+// evaluation instead of six divisions and a root, each term within a few ulp of the reference's.
 __device__ __forceinline__ void ode_g_dg(const ODEConsts& c, double subdelt, double rprev, double rsqrd,
                                          double& g, double& dg) {
-  const double radius = sqrt(rsqrd);
-  const double inv_r = 1.0 / radius;
+  const double inv_r = rsqrt_nr(rsqrd);
   const double inv_r2 = inv_r * inv_r;
   const double a_r = c.akoh * inv_r;
   const double b_r3 = c.bkoh * (inv_r2 * inv_r);
@@ -279,6 +319,51 @@ __device__ __forceinline__ double critical_timestep(const ODEConsts& c) {
   return 2.5 * c.ffactor_fv / c.akoh * cuberoot;
 }
 
+// Three-valued comparisons: +1 true, -1 false, 0 too close to call from approximate operands.
+// and = min, or = max.
+// x < t where t is known to 1e-13 relative
+__device__ __forceinline__ int tri_less(double x, double t) {
+  return x < t * (1.0 - 1e-12) ? 1 : (x > t * (1.0 + 1e-12) ? -1 : 0);
+}
+// sign of d - sqrt(y) for y >= 0 known to 1e-13 relative, decided when it also fixes the sign of
+// s - (1 + sqrt(y)) rounded (d = s - 1): the margins cover the 1e-15 relative error of sqrt(y)
+// and the half-ulp of 1 in the sum.
+__device__ __forceinline__ int tri_sqrt_sign(double d, double y) {
+  if (y != y) return 0;
+  if (d <= 0.0) return -1;
+  const double d2 = d * d;
+  if (d2 > fma(y, 1.0 + 1e-8, 1e-20)) return 1;
+  if (d2 < fma(y, 1.0 - 1e-8, -1e-20)) return -1;
+  return 0;
+}
+// initialguess, first_unique_criteria and the critical-timestep test of solve_condensation
+// (impliciteuler.cpp:35-54, 80-126).  Each comparison is decided from products with approximate
+// reciprocals when its sides differ by more than 1e-8 relative (the reference's quotients and
+// roots are exact to a few ulp, so the outcome is the same); a drop with any comparison inside
+// that band evaluates the reference's expressions instead.
+__device__ __forceinline__ void ode_setup(const ODEConsts& c, double rprev, double delt, double& ziter, bool& ucrit) {
+  const double ia = rcp_nr(c.akoh), ib = rcp_nr(c.bkoh);
+  const double y = (c.akoh * c.akoh * c.akoh) * ib * (4.0 / 27.0);  // (s_act - 1)^2
+  const int sact = tri_sqrt_sign(c.s_ratio - 1.0, y);                // s_ratio > s_act
+  constexpr double bigr(1e-3 / dimless::R0);
+  const double maxradius = fmax(bigr, rprev);
+  ziter = sact > 0 ? maxradius * maxradius : rprev * rprev;
+  const double rcrit = 3.0 * c.bkoh * ia;  // rcritsqrd
+  const int unact = min(tri_less(rprev * rprev, rcrit), tri_less(ziter, rcrit));
+  const int ucrit1 = min(unact, -sact);
+  // delt <= critdelt = 2.5 ffv/akoh x^1.5 with x = 5 bkoh/akoh, compared squared
+  const double x = 5.0 * c.bkoh * ia, k = 2.5 * c.ffactor_fv * ia;
+  const double crit2 = (k * k) * (x * x * x);
+  const double d2 = delt * delt;
+  const int ucrit2 = delt > 0.0 ? (d2 < crit2 * (1.0 - 1e-8) ? 1 : (d2 > crit2 * (1.0 + 1e-8) ? -1 : 0)) : 0;
+  const int uc = max(ucrit1, ucrit2);
+  ucrit = uc > 0;
+  if (sact == 0 || uc == 0) {
+    ziter = initialguess(c, rprev);
+    ucrit = first_unique_criteria(c, rprev, ziter) || (delt <= critical_timestep(c));
+  }
+}
+
 struct SolveOut {
   uint32_t iters;
   bool substep, restarted, maxed;
@@ -291,13 +376,13 @@ struct SolveOut {
 __device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, double s_ratio, double akoh,
                                                      double bkoh, double ffactor, double rprev, double warm,
                                                      uint32_t warm_limit, SolveOut& o) {
-  const double ffactor_fv = ffactor / ventilation_factor(rprev);
-  const ODEConsts c{s_ratio, akoh, bkoh, ffactor_fv, 1.0 / ffactor_fv};
-  double ziter = initialguess(c, rprev);
-  const bool ucrit1 = first_unique_criteria(c, rprev, ziter);
-  const bool ucrit2 = (cfg.delt <= critical_timestep(c));
+  ODEConsts c{s_ratio, akoh, bkoh, 0.0, 0.0};
+  ffactor_over_vent(ffactor, rprev, c.ffactor_fv, c.inv_ffv);
+  double ziter;
+  bool ucrit;
+  ode_setup(c, rprev, cfg.delt, ziter, ucrit);
   double rsqrd;
-  if (ucrit1 || ucrit2) {
+  if (ucrit) {
     if (warm == warm) {
       NewtonState s;
       s.z = warm;

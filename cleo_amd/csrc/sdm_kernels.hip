@@ -114,7 +114,7 @@ __device__ __forceinline__ void cond_gbx(const CondArgs& a, uint32_t g, int lane
       const double warm = a.warm ? a.warm[k] : __longlong_as_double(0x7FF8000000000000ll);
       SolveOut o{0, false, false, false, false};
       const double newr = solve_condensation(a.cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
-      const double rnew = fmax(newr, dryradius(msol, a.sol));  // change_radius (superdrop_attrs.cpp:35-45)
+      const double rnew = at_least_dry(newr, msol, a.sol);
       a.s.r[k] = rnew;
       dm = (condensate_mass(rnew, msol, a.sol, &bad_cm) - old_m) * static_cast<double>(xi);
       if (o.maxed) set_error(a.ctl, CLEO_ERR_NEWTON_MAXITERS, g, k);
@@ -1644,7 +1644,7 @@ __device__ __forceinline__ bool cond_solve(const CondFlatArgs& a, const NewtonCf
   const double old_m = condensate_mass(r, msol, a.sol, &bad);
   const double xi = static_cast<double>(xi_of());
   qm = old_m * xi;
-  rnew = fmax(newr, dryradius(msol, a.sol));
+  rnew = at_least_dry(newr, msol, a.sol);
   dm = (condensate_mass(rnew, msol, a.sol, &bad) - old_m) * xi;
   if (bad) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, g, k);
   if (o.maxed) set_error(a.ctl, CLEO_ERR_NEWTON_MAXITERS, g, k);
@@ -1895,11 +1895,12 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_cond_queue(CondFlatAr
         if (bad) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, g, kk);
         double akoh, bkoh;
         kohler_factors(ms, temp, a.sol, akoh, bkoh);
-        const double ffv = ffactor / ventilation_factor(r);
-        const ODEConsts cc{s_ratio, akoh, bkoh, ffv, 1.0 / ffv};
-        const double zg = initialguess(cc, r);
-        const double critdelt = critical_timestep(cc);
-        const bool uniq = first_unique_criteria(cc, r, zg) || (cfg.delt <= critdelt);
+        ODEConsts cc{s_ratio, akoh, bkoh, 0.0, 0.0};
+        ffactor_over_vent(ffactor, r, cc.ffactor_fv, cc.inv_ffv);
+        double zg;
+        bool uniq;
+        ode_setup(cc, r, cfg.delt, zg, uniq);
+        const double critdelt = uniq ? 0.0 : critical_timestep(cc);
         R.s_ratio[slot] = s_ratio;
         R.akoh[slot] = akoh;
         R.bkoh[slot] = bkoh;
