@@ -426,10 +426,12 @@ __device__ __forceinline__ double tv_simmel(double radius) {
   constexpr double a3 = 1732 / VELCONST;
   constexpr double a4 = 917 / VELCONST;
   if (radius >= r3) return a4;
-  constexpr double massconst(4.0 / 3.0 * kPi * dimless::Rho_l);
-  const double MASS = (massconst * radius * radius * radius) * dimless::MASS0grams;
-  // MASS^(1/6), ^(1/3), ^(2/3) from one cube root (pow in the reference; within 2 ulp of it)
-  const double cr = cbrt(MASS);
+  // MASS = massconst r^3 MASS0grams with massconst = 4/3 pi Rho_l, so MASS^(1/3) = k r for
+  // k = (massconst MASS0grams)^(1/3), the correctly rounded cube root of the product of the two
+  // double constants (0.000161102579187357...).  The reference's pow(MASS, 1/3), ^(2/3), ^(1/6)
+  // become k r, (k r)^2 and sqrt(k r), each within a few ulp of it.
+  constexpr double kMassCbrt = 0.00016110257918735754;
+  const double cr = kMassCbrt * radius;
   if (radius >= r2) return a3 * sqrt(cr);
   if (radius >= r1) return a2 * cr;
   return a1 * (cr * cr);
