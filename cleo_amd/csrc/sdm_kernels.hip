@@ -603,16 +603,21 @@ __device__ __forceinline__ uint32_t source_of(const Grid& gr, uint32_t d, int s)
 __device__ __forceinline__ uint32_t neighbour_of(const Grid& gr, uint32_t g, int s) { return source_of(gr, g, 26 - s); }
 
 
-__device__ __forceinline__ double interpolation(double blo, double bhi, double vlo, double vhi, double sd) {
-  const double coord = fmin(bhi, fmax(blo, sd));  // predcorr.cpp:31-42
-  const double alpha = (coord - blo) / (bhi - blo);
+// wind at coordinate sd between the faces blo, bhi (predcorr.cpp:31-42).  inv_span = 1/(bhi - blo)
+// of the gridbox, so the reference's alpha = (coord - blo) / (bhi - blo) is a product, within an
+// ulp or two of the quotient.
+__device__ __forceinline__ double interpolation(double blo, double bhi, double inv_span, double vlo, double vhi,
+                                                double sd) {
+  const double coord = fmin(bhi, fmax(blo, sd));
+  const double alpha = (coord - blo) * inv_span;
   return alpha * vhi + (1 - alpha) * vlo;
 }
 
 // One axis of CartesianMaps::get_local_bounding_gridbox_index (cartesianmaps.cpp:68-286,
 // domainboundaries.hpp:34-109): step to the backward/forward neighbour when the coordinate left
-// [lower, upper) and wrap the coordinate at a periodic domain edge.  Returns the step taken.
-__device__ __forceinline__ int axis_step(uint32_t& idx, double& coord, uint32_t i, uint32_t nd, uint32_t inc,
+// [lower, upper) and wrap the coordinate at a periodic domain edge.  i is the drop's index along
+// the axis, updated to the neighbour's.  Returns the step taken.
+__device__ __forceinline__ int axis_step(uint32_t& idx, double& coord, uint32_t& i, uint32_t nd, uint32_t inc,
                                          bool periodic, const double* h) {
   const double lo = h[i], hi = h[i + 1];
   if (coord < lo) {
@@ -621,11 +626,13 @@ __device__ __forceinline__ int axis_step(uint32_t& idx, double& coord, uint32_t 
         const uint32_t nb = idx + (nd - 1) * inc;
         coord = coord + h[nd] - lo;  // lim1 = upper bound of the backward neighbour
         idx = nb;
+        i = nd - 1;
       } else {
         idx = kOOB;
       }
     } else {
       idx = idx - inc;
+      i -= 1;
     }
     return -1;
   }
@@ -635,21 +642,24 @@ __device__ __forceinline__ int axis_step(uint32_t& idx, double& coord, uint32_t 
         const uint32_t nb = idx - (nd - 1) * inc;
         coord = coord + h[0] - hi;  // lim1 = lower bound of the forward neighbour
         idx = nb;
+        i = 0;
       } else {
         idx = kOOB;
       }
     } else {
       idx = idx + inc;
+      i += 1;
     }
     return 1;
   }
   return 0;
 }
 
-// bounds and face winds of gridbox g
+// bounds, reciprocal widths and face winds of gridbox g
 struct GbxGeom {
   uint32_t iz, ix, iy;
   double zlo, zhi, xlo, xhi, ylo, yhi, wlo, whi, ulo, uhi, vlo, vhi;
+  double inv_z, inv_x, inv_y;
 };
 __device__ __forceinline__ GbxGeom gbx_geom(const MotionArgs& a, uint32_t g) {
   const uint32_t nz = a.grid.nz, nx = a.grid.nx;
@@ -669,6 +679,9 @@ __device__ __forceinline__ GbxGeom gbx_geom(const MotionArgs& a, uint32_t g) {
   G.uhi = a.st.uhi[g];
   G.vlo = a.st.vlo[g];
   G.vhi = a.st.vhi[g];
+  G.inv_z = rcp_nr(G.zhi - G.zlo);
+  G.inv_x = rcp_nr(G.xhi - G.xlo);
+  G.inv_y = rcp_nr(G.yhi - G.ylo);
   return G;
 }
 // drop k of gridbox g after one predictor-corrector step: new coordinates, new gridbox (kOOB
@@ -695,21 +708,21 @@ __device__ __forceinline__ Moved move_drop(const MotionArgs& a, uint32_t g, cons
   const double c3_0 = P.c3, c1_0 = P.c1, c2_0 = P.c2;
   double coord3 = c3_0;
   const double terminal = terminal_velocity(a.tv, P.r);
-  double vel3 = interpolation(G.zlo, G.zhi, G.wlo, G.whi, coord3);
+  double vel3 = interpolation(G.zlo, G.zhi, G.inv_z, G.wlo, G.whi, coord3);
   vel3 -= terminal;
   coord3 += vel3 * delt;
-  double corrvel3 = interpolation(G.zlo, G.zhi, G.wlo, G.whi, coord3);
+  double corrvel3 = interpolation(G.zlo, G.zhi, G.inv_z, G.wlo, G.whi, coord3);
   corrvel3 -= terminal;
   const double delta3 = (vel3 + corrvel3) * (delt / 2);
   double coord1 = c1_0;
-  const double vel1 = interpolation(G.xlo, G.xhi, G.ulo, G.uhi, coord1);
+  const double vel1 = interpolation(G.xlo, G.xhi, G.inv_x, G.ulo, G.uhi, coord1);
   coord1 += vel1 * delt;
-  const double corrvel1 = interpolation(G.xlo, G.xhi, G.ulo, G.uhi, coord1);
+  const double corrvel1 = interpolation(G.xlo, G.xhi, G.inv_x, G.ulo, G.uhi, coord1);
   const double delta1 = (vel1 + corrvel1) * (delt / 2);
   double coord2 = c2_0;
-  const double vel2 = interpolation(G.ylo, G.yhi, G.vlo, G.vhi, coord2);
+  const double vel2 = interpolation(G.ylo, G.yhi, G.inv_y, G.vlo, G.vhi, coord2);
   coord2 += vel2 * delt;
-  const double corrvel2 = interpolation(G.ylo, G.yhi, G.vlo, G.vhi, coord2);
+  const double corrvel2 = interpolation(G.ylo, G.yhi, G.inv_y, G.vlo, G.vhi, coord2);
   const double delta2 = (vel2 + corrvel2) * (delt / 2);
   bool cfl = (fabs(delta3) <= fabs(G.zhi - G.zlo));
   cfl = (fabs(delta1) <= fabs(G.xhi - G.xlo)) && cfl;
@@ -719,15 +732,14 @@ __device__ __forceinline__ Moved move_drop(const MotionArgs& a, uint32_t g, cons
   M.c3 = c3_0 + delta3;
   M.c1 = c1_0 + delta1;
   M.c2 = c2_0 + delta2;
-  uint32_t idx = g;
-  const int dz = axis_step(idx, M.c3, G.iz, nz, 1u, a.grid.per[0] != 0, a.grid.zh);
+  uint32_t idx = g, jz = G.iz, jx = G.ix, jy = G.iy;
+  const int dz = axis_step(idx, M.c3, jz, nz, 1u, a.grid.per[0] != 0, a.grid.zh);
   int dx = 0, dy = 0;
-  if (idx != kOOB) dx = axis_step(idx, M.c1, G.ix, nx, nz, a.grid.per[1] != 0, a.grid.xh);
-  if (idx != kOOB) dy = axis_step(idx, M.c2, G.iy, ny, nz * nx, a.grid.per[2] != 0, a.grid.yh);
+  if (idx != kOOB) dx = axis_step(idx, M.c1, jx, nx, nz, a.grid.per[1] != 0, a.grid.xh);
+  if (idx != kOOB) dy = axis_step(idx, M.c2, jy, ny, nz * nx, a.grid.per[2] != 0, a.grid.yh);
   M.idx = idx;
   M.dir = 27;
   if (idx != kOOB) {
-    const uint32_t jz = idx % nz, jx = (idx / nz) % nx, jy = idx / (nz * nx);
     const bool bad = (M.c3 < a.grid.zh[jz]) | (M.c3 >= a.grid.zh[jz + 1]) | (M.c1 < a.grid.xh[jx]) |
                      (M.c1 >= a.grid.xh[jx + 1]) | (M.c2 < a.grid.yh[jy]) | (M.c2 >= a.grid.yh[jy + 1]);
     if (report && bad) set_error(a.ctl, CLEO_ERR_BOUNDS, g, k);
