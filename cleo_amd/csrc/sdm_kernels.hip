@@ -849,25 +849,17 @@ __global__ void k_nsort_tot(NsortArgs a) {
 // recorded as perm[dest] = p; k_nsort_gather then moves the records, so that each cache line of the
 // sorted store is written whole by one wave (runs scattered from many source gridboxes into one
 // line cost HBM read-modify-writes).
-// what the ranking of gridbox g needs before it touches its drops: its range of positions and, in
-// lane s < 27, the next place of its drops that take step s (lane 27: of those leaving)
-struct RankGbx {
-  uint32_t g, sbase, nfull, trun;
-  int gz, gx, gy;
-};
-__device__ __forceinline__ RankGbx nsort_rank_head(const NsortArgs& a, uint32_t g, int lane) {
-  RankGbx R;
-  R.g = g;
-  R.sbase = a.off[g];
-  R.nfull = a.off[g + 1] - R.sbase;
+__device__ __forceinline__ void nsort_rank_gbx(const NsortArgs& a, uint32_t g, int lane) {
+  const uint32_t sbase = a.off[g];
+  const uint32_t nfull = a.off[g + 1] - sbase;
   const uint32_t ngbx = a.grid.ngbx;
   const int nz = static_cast<int>(a.grid.nz), nx = static_cast<int>(a.grid.nx), ny = static_cast<int>(a.grid.ny);
-  R.gz = static_cast<int>(g % a.grid.nz);
-  R.gx = static_cast<int>((g / a.grid.nz) % a.grid.nx);
-  R.gy = static_cast<int>(g / (a.grid.nz * a.grid.nx));
+  const int gz = static_cast<int>(g % a.grid.nz), gx = static_cast<int>((g / a.grid.nz) % a.grid.nx),
+            gy = static_cast<int>(g / (a.grid.nz * a.grid.nx));
+  // lane s < 27: next place of this gridbox's drops that take step s; lane 27: of those leaving.
   // The neighbour reached by step s = (dz+1) + 3(dx+1) + 9(dy+1), wrapped on periodic axes
   // (the same gridbox as neighbour_of, without a division per lane).
-  R.trun = 0;
+  uint32_t trun = 0;
   if (lane < 27) {
     auto axis = [](int i, int d, int n, bool per) {
       int j = i + d;
@@ -875,32 +867,16 @@ __device__ __forceinline__ RankGbx nsort_rank_head(const NsortArgs& a, uint32_t 
       if (j >= n) j = per ? j - n : -1;
       return j;
     };
-    const int jz = axis(R.gz, lane % 3 - 1, nz, a.grid.per[0] != 0);
-    const int jx = axis(R.gx, (lane / 3) % 3 - 1, nx, a.grid.per[1] != 0);
-    const int jy = axis(R.gy, lane / 9 - 1, ny, a.grid.per[2] != 0);
+    const int jz = axis(gz, lane % 3 - 1, nz, a.grid.per[0] != 0);
+    const int jx = axis(gx, (lane / 3) % 3 - 1, nx, a.grid.per[1] != 0);
+    const int jy = axis(gy, lane / 9 - 1, ny, a.grid.per[2] != 0);
     if (jz >= 0 && jx >= 0 && jy >= 0) {
       const uint32_t d = static_cast<uint32_t>(jz + nz * (jx + nx * jy));
-      R.trun = a.base[d] + a.cnt[28ull * d + lane];
+      trun = a.base[d] + a.cnt[28ull * d + lane];
     }
   } else if (lane == 27) {
-    R.trun = a.base[ngbx] + a.oobpre[g];
+    trun = a.base[ngbx] + a.oobpre[g];
   }
-  return R;
-}
-// position held by slot sbase + i of the logical order (the order collisions left, or the
-// store's own order); 0 past the gridbox
-__device__ __forceinline__ uint32_t nsort_slot_pos(const NsortArgs& a, const RankGbx& R, uint32_t i) {
-  if (i >= R.nfull) return 0u;
-  const uint32_t q = R.sbase + i;
-  return a.order ? a.order[q] : q;
-}
-// ranks the drops of gridbox R.g and records perm[dest] = p; p0 is the position of the first
-// chunk's slot of this lane (nsort_slot_pos)
-__device__ __forceinline__ void nsort_rank_gbx(const NsortArgs& a, RankGbx R, uint32_t p0, int lane) {
-  const uint32_t g = R.g, sbase = R.sbase, nfull = R.nfull, ngbx = a.grid.ngbx;
-  const int nz = static_cast<int>(a.grid.nz), nx = static_cast<int>(a.grid.nx), ny = static_cast<int>(a.grid.ny);
-  const int gz = R.gz, gx = R.gx, gy = R.gy;
-  uint32_t trun = R.trun;
   auto near = [](int q, int r, int nd, bool per) {
     const int dd = q - r;
     if (dd >= -1 && dd <= 1) return true;
@@ -916,7 +892,7 @@ __device__ __forceinline__ void nsort_rank_gbx(const NsortArgs& a, RankGbx R, ui
     // left, or the store's own order)
     const uint32_t q = sbase + c + lane;
     const bool act = c + lane < nfull;
-    const uint32_t p = c == 0 ? p0 : nsort_slot_pos(a, R, c + lane);
+    const uint32_t p = (act && a.order) ? a.order[q] : q;
     // the step g -> key (27: out of the domain): recorded by motion, or found from the key; a key
     // that is no neighbour of g, or a position outside the domain, is an error
     int sv = 27;
@@ -965,33 +941,10 @@ __device__ __forceinline__ void nsort_rank_gbx(const NsortArgs& a, RankGbx R, ui
 // first pass of the neighbour sort: the destination of every drop of the domain, as perm[dest] = p.
 // Gridboxes go to waves through xcd_wave_index(), so each XCD sweeps one contiguous range and the
 // count-table rows that neighbouring gridboxes share are fetched into one L2.
-// The gridboxes of a wave go through three stages, one gridbox each per iteration: the head loads
-// (range and run starts) two gridboxes ahead, the slot positions one ahead, the step bytes and the
-// ranking of the current one, so each iteration waits for one round trip to memory, not three.
 __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(8) k_nsort_perm(NsortArgs a) {
   if (guard_off(a.flag)) return;
   const int lane = lane_id();
-  const uint64_t ngbx = a.grid.ngbx, stride = wave_count();
-  uint64_t g = xcd_wave_index();
-  if (g >= ngbx) return;
-  RankGbx r0 = nsort_rank_head(a, static_cast<uint32_t>(g), lane);
-  bool h1 = g + stride < ngbx;
-  RankGbx r1{};
-  if (h1) r1 = nsort_rank_head(a, static_cast<uint32_t>(g + stride), lane);
-  uint32_t p0 = nsort_slot_pos(a, r0, static_cast<uint32_t>(lane));
-  for (;;) {
-    const bool h2 = h1 && g + 2 * stride < ngbx;
-    RankGbx r2{};
-    if (h2) r2 = nsort_rank_head(a, static_cast<uint32_t>(g + 2 * stride), lane);
-    const uint32_t p1 = h1 ? nsort_slot_pos(a, r1, static_cast<uint32_t>(lane)) : 0u;
-    nsort_rank_gbx(a, r0, p0, lane);
-    if (!h1) break;
-    r0 = r1;
-    p0 = p1;
-    r1 = r2;
-    h1 = h2;
-    g += stride;
-  }
+  for (uint32_t g = xcd_wave_index(); g < a.grid.ngbx; g += wave_count()) nsort_rank_gbx(a, g, lane);
 }
 
 // Second pass: destination position q takes the record at source position perm[q].  Consecutive
