@@ -384,9 +384,12 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   // a gridbox launch has at most ngbx + 3 waves, a drop launch 4 per 256 drops
   c->sp_slots = std::max<uint64_t>(ngbx + 4, 4 * ((maxn + 255) / 256) + 4);
   HIP_TRY(c, dalloc(&c->sp, kStatSlotsMax * c->sp_slots));
-  // deferred list: waves of a drop launch times (a quarter of their drops + 16)
-  HIP_TRY(c, dalloc(&c->dcount, c->sp_slots));
-  HIP_TRY(c, dalloc(&c->dlist, maxn / 4 + 20 * c->sp_slots + 64));
+  // deferred drops of the condensation: per-wave segments (waves of a drop launch times a quarter
+  // of their drops + 16), the shared list they move to (as large) and its counter
+  HIP_TRY(c, dalloc(&c->dcount, 1));
+  c->dlist_cap = static_cast<uint32_t>(std::min<uint64_t>(maxn / 4 + 20 * c->sp_slots + 64, 0xFFFFFFFFull));
+  HIP_TRY(c, dalloc(&c->dlist, c->dlist_cap));
+  HIP_TRY(c, dalloc(&c->dseg, c->dlist_cap));
   HIP_TRY(c, dalloc(&c->perm, std::max<uint64_t>(maxn, 1)));
   HIP_TRY(c, dalloc(&c->dstep, std::max<uint64_t>(maxn, 1)));
   HIP_TRY(c, dalloc(&c->order, std::max<uint64_t>(maxn, 1)));
@@ -420,7 +423,7 @@ int cleo_destroy(cleo_ctx* c) {
                   static_cast<void*>(c->rs_key[1]), static_cast<void*>(c->rs_idx[0]), static_cast<void*>(c->rs_idx[1]),
                   static_cast<void*>(c->rs_cnt), static_cast<void*>(c->gs), static_cast<void*>(c->dm),
                   static_cast<void*>(c->sp), static_cast<void*>(c->dlist), static_cast<void*>(c->dcount),
-                  static_cast<void*>(c->perm), static_cast<void*>(c->dstep), static_cast<void*>(c->order),
+                  static_cast<void*>(c->dseg), static_cast<void*>(c->perm), static_cast<void*>(c->dstep), static_cast<void*>(c->order),
                   static_cast<void*>(c->qm)})
     if (p) hipFree(p);
   if (c->h_ctl) hipHostFree(c->h_ctl);

@@ -106,6 +106,8 @@ struct cleo_ctx {
   // deferred-drop list of the thread-per-drop condensation (segments per wave) and their counts
   uint32_t* dlist = nullptr;
   uint32_t* dcount = nullptr;
+  uint32_t dlist_cap = 0;
+  uint32_t* dseg = nullptr;
   // neighbour sort: source position of every destination position (maxn), and the step each drop
   // took in the last thread-per-drop motion (maxn bytes, 27 = left the domain)
   uint32_t* perm = nullptr;

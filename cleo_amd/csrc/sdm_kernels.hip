@@ -1619,6 +1619,7 @@ namespace cleo {
 // per gridbox: optional qcond recompute (ordered, before any radius changes), then the
 // saturation ratio and diffusion factor condensation uses (condensation.cpp:44-56)
 __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_prep(CondFlatArgs a) {
+  if (a.dcount && blockIdx.x == 0 && threadIdx.x == 0) *a.dcount = 0u;  // the deferred list starts empty
   __shared__ double win_all[4][kSumWin];
   double* win = win_all[threadIdx.x >> 6];
   const int lane = lane_id();
@@ -1745,15 +1746,44 @@ __global__ void __launch_bounds__(256) k_cond_guess(CondFlatArgs a) {
   }
 }
 
+// Deferred drops of a wave are first listed in the wave's own segment of a.dseg (`dsegcap` each,
+// no contention); defer_drop appends those of the wave's deferring lanes (position q) and
+// returns false for a deferring lane whose drop found the segment full (the caller solves it at
+// once).  Every lane of the wave calls it.
+__device__ __forceinline__ bool defer_drop(const CondFlatArgs& a, uint32_t* seg, uint32_t& ndef, bool deferred,
+                                           uint32_t q) {
+  const uint64_t m = __ballot(deferred);
+  if (!m) return true;
+  const uint32_t idx = ndef + static_cast<uint32_t>(__popcll(m & lanemask_lt()));
+  ndef = min(a.dsegcap, ndef + static_cast<uint32_t>(__popcll(m)));
+  if (!deferred) return true;
+  if (idx >= a.dsegcap) return false;
+  seg[idx] = q;
+  return true;
+}
+// At the end of the kernel the wave moves its segment to the shared list with one atomic, so that
+// k_cond_deferred finds the deferred drops of all waves packed 64 to a wave.  The shared list has
+// room for every segment, so nothing is lost.
+__device__ __forceinline__ void defer_flush(const CondFlatArgs& a, const uint32_t* seg, uint32_t ndef, int lane) {
+  if (ndef == 0) return;
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(a.dcount, ndef);
+  base = readlane_u32(base, 0);
+  for (uint32_t i = static_cast<uint32_t>(lane); i < ndef; i += kWave) {
+    const uint64_t j = static_cast<uint64_t>(base) + i;
+    if (j < a.dcap) a.dlist[j] = seg[i];
+  }
+}
+
 // per active drop, one thread each.  Drops whose solve outlives cfg.budget iterations (a few per
-// cent; their lanes would hold the other 63 of the wave) go to this wave's segment of the
-// deferred list, up to `dcap` of them, and are solved by k_cond_deferred.
+// cent; their lanes would hold the other 63 of the wave) go to the deferred list, up to `dcap`
+// of them, and are solved by k_cond_deferred, 64 to a wave.
 __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_drops(CondFlatArgs a) {
   const uint32_t ndom = a.off[a.ngbx];
   const int lane = lane_id();
   CondStats st{0, 0, 0, 0};
   uint32_t ndef = 0;
-  uint32_t* mylist = a.dlist + static_cast<uint64_t>(wave_index()) * a.dcap;
+  uint32_t* seg = a.dseg + static_cast<uint64_t>(wave_index()) * a.dsegcap;
   NewtonCfg nobudget = a.cfg;
   nobudget.budget = 0;
   for (uint64_t k0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x; k0 < ndom;
@@ -1769,32 +1799,22 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_drops(CondFlatAr
     if (active && a.warm) warm = a.warm[k];
     bool deferred = false;
     if (active) deferred = !cond_drop(a, a.cfg, g, k, warm, st);
-    const uint64_t m = __ballot(deferred);
-    if (m) {
-      const uint32_t idx = ndef + static_cast<uint32_t>(__popcll(m & lanemask_lt()));
-      if (deferred) {
-        if (idx < a.dcap)
-          mylist[idx] = static_cast<uint32_t>(k);
-        else
-          cond_drop(a, nobudget, g, k, warm, st);  // the segment is full: solve it here
-      }
-      ndef = min(a.dcap, ndef + static_cast<uint32_t>(__popcll(m)));
-    }
+    if (!defer_drop(a, seg, ndef, deferred, static_cast<uint32_t>(k))) cond_drop(a, nobudget, g, k, warm, st);
   }
-  if (a.dcount && lane == 0) a.dcount[wave_index()] = ndef;
+  if (a.dcount) defer_flush(a, seg, ndef, lane);
   put_cond_stats(a.stat, st);
 }
 
-// the deferred drops of wave w's segment, solved without a budget by wave w of this launch
+// the deferred drops, thread i taking entry i of the list (grid-stride), solved without a budget;
+// every wave writes its statistics slot
 __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_deferred(CondFlatArgs a) {
-  const int lane = lane_id();
   CondStats st{0, 0, 0, 0};
   NewtonCfg nobudget = a.cfg;
   nobudget.budget = 0;
-  const uint32_t n = a.dcount[wave_index()];
-  const uint32_t* mylist = a.dlist + static_cast<uint64_t>(wave_index()) * a.dcap;
-  for (uint32_t i = lane; i < n; i += kWave) {
-    const uint32_t k = mylist[i];
+  const uint64_t n = min(static_cast<uint64_t>(*a.dcount), static_cast<uint64_t>(a.dcap));
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t k = a.dlist[i];
     const double warm = a.warm ? a.warm[k] : __longlong_as_double(0x7FF8000000000000ll);
     cond_drop(a, nobudget, a.s.gbx[k], k, warm, st);
   }
@@ -1813,7 +1833,7 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_sort_gather_cond(Nsor
   const int lane = lane_id();
   CondStats st{0, 0, 0, 0};
   uint32_t ndef = 0;
-  uint32_t* mylist = a.dlist + static_cast<uint64_t>(wave_index()) * a.dcap;
+  uint32_t* seg = a.dseg + static_cast<uint64_t>(wave_index()) * a.dsegcap;
   NewtonCfg nobudget = a.cfg;
   nobudget.budget = 0;
   const double nowarm = __longlong_as_double(0x7FF8000000000000ll);
@@ -1836,17 +1856,8 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_sort_gather_cond(Nsor
     bool solved = false;
     if (active) solved = cond_solve(a, a.cfg, g, q, r, msol, [&] { return xi; }, warm, st, rnew, dm, qm);
     const bool deferred = active && !solved;
-    const uint64_t m = __ballot(deferred);
-    if (m) {
-      const uint32_t idx = ndef + static_cast<uint32_t>(__popcll(m & lanemask_lt()));
-      if (deferred) {
-        if (idx < a.dcap)
-          mylist[idx] = static_cast<uint32_t>(q);
-        else  // the segment is full: solve it here
-          solved = cond_solve(a, nobudget, g, q, r, msol, [&] { return xi; }, warm, st, rnew, dm, qm);
-      }
-      ndef = min(a.dcap, ndef + static_cast<uint32_t>(__popcll(m)));
-    }
+    if (!defer_drop(a, seg, ndef, deferred, static_cast<uint32_t>(q)))
+      solved = cond_solve(a, nobudget, g, q, r, msol, [&] { return xi; }, warm, st, rnew, dm, qm);
     if (okp) {
       n.dst.gbx[q] = g;
       n.dst.c3[q] = c3;
@@ -1862,7 +1873,7 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_sort_gather_cond(Nsor
       if (a.qm) a.qm[q] = qm;
     }
   }
-  if (a.dcount && lane == 0) a.dcount[wave_index()] = ndef;
+  if (a.dcount) defer_flush(a, seg, ndef, lane);
   put_cond_stats(a.stat, st);
 }
 
@@ -2289,6 +2300,14 @@ static void launch_gather_move(cleo_ctx* c, const NsortArgs& n, const MotionArgs
     hipLaunchKernelGGL(k_nsort_gather_move<false>, dim3(nb), dim3(256), 0, c->stream, n, m);
 }
 
+// per-wave segments of the deferred list for a drop launch of nb blocks: a quarter of a wave's
+// drops, at least 16
+static void set_defer_segments(const cleo_ctx* c, CondFlatArgs& a, unsigned nb) {
+  const uint64_t per_wave = (c->ntot + 4ull * nb - 1) / (4ull * nb);
+  a.dseg = c->dseg;
+  a.dsegcap = a.dcount ? static_cast<uint32_t>(per_wave / 4 + 16) : 0u;
+}
+
 void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& args) {
   if (args.ngbx == 0) return;
   CondFlatArgs a = args;
@@ -2297,6 +2316,11 @@ void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& args) {
   a.qm = (a.fuse_qcond && c->tiles.cond_flat == 1) ? c->qm : nullptr;
   const uint32_t gw = c->tiles.cond_grid_waves;
   const uint64_t ntiles = (a.ngbx + kWave - 1) / kWave;
+  const uint32_t budget = c->tiles.cond_flat == 2 ? 0u : c->tiles.cond_defer;
+  a.cfg.budget = budget;
+  a.dcap = budget ? c->dlist_cap : 0u;
+  a.dlist = c->dlist;
+  a.dcount = budget ? c->dcount : nullptr;  // zeroed by k_cond_prep
   hipLaunchKernelGGL(k_cond_prep, dim3(gbx_blocks(ntiles, 4, gw)), dim3(256), 0, c->stream, a);
   if (c->tiles.cond_flat == 2) {
     // task-ring kernel: waves of 4 per block; by default as many waves as fit on the device at once
@@ -2310,13 +2334,7 @@ void launch_condensation_flat(cleo_ctx* c, const CondFlatArgs& args) {
   } else {
     const unsigned nb = drop_blocks(c->ntot, gw);
     a.stat = stat_slots(c, 4ull * nb);  // every wave of k_cond_drops writes its slot
-    const uint32_t budget = c->tiles.cond_defer;
-    a.cfg.budget = budget;
-    // per-wave segments of the deferred list: a quarter of a wave's drops, at least 16
-    const uint64_t per_wave = (c->ntot + 4ull * nb - 1) / (4ull * nb);
-    a.dcap = budget ? static_cast<uint32_t>(per_wave / 4 + 16) : 0u;
-    a.dlist = c->dlist;
-    a.dcount = budget ? c->dcount : nullptr;
+    set_defer_segments(c, a, nb);
     if (a.mlp_inline) {  // the learned guesses, from the saturation ratios k_cond_prep left
       hipLaunchKernelGGL(k_cond_guess, dim3(drop_blocks(c->ntot, gw ? gw : 32768)), dim3(256), 0, c->stream, a);
       a.warm = a.warm_buf;
@@ -2359,6 +2377,13 @@ void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatAr
   a.qm = c->qm;  // the qcond recompute joins k_cond_state
   const uint32_t gw = c->tiles.cond_grid_waves;
   const uint64_t ntiles = (ngbx + kWave - 1) / kWave;
+  // a deferred drop is solved again by its sorted position, which the guess buffer is not
+  // indexed by: with learned guesses every drop is solved in the gather
+  const uint32_t budget = (a.mlp_inline || a.warm) ? 0u : c->tiles.cond_defer;
+  a.cfg.budget = budget;
+  a.dcap = budget ? c->dlist_cap : 0u;
+  a.dlist = c->dlist;
+  a.dcount = budget ? c->dcount : nullptr;  // zeroed by k_cond_prep
   hipLaunchKernelGGL(k_cond_prep, dim3(gbx_blocks(ntiles, 4, gw)), dim3(256), 0, c->stream, a);
   if (a.mlp_inline) {  // learned guesses of the drops where motion left them
     CondFlatArgs ga = a;
@@ -2370,14 +2395,7 @@ void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatAr
   }
   const unsigned nb = drop_blocks(c->ntot, gw);
   a.stat = stat_slots(c, 4ull * nb);
-  // a deferred drop is solved again by its sorted position, which the guess buffer is not
-  // indexed by: with learned guesses every drop is solved in the gather
-  const uint32_t budget = a.warm ? 0u : c->tiles.cond_defer;
-  a.cfg.budget = budget;
-  const uint64_t per_wave = (c->ntot + 4ull * nb - 1) / (4ull * nb);
-  a.dcap = budget ? static_cast<uint32_t>(per_wave / 4 + 16) : 0u;
-  a.dlist = c->dlist;
-  a.dcount = budget ? c->dcount : nullptr;
+  set_defer_segments(c, a, nb);
   kev_mark(c, 0);
   hipLaunchKernelGGL(k_sort_gather_cond, dim3(nb), dim3(256), 0, c->stream, n, a);
   kev_mark(c, 1);

@@ -120,9 +120,11 @@ struct CondFlatArgs {
   // recomputes the gridbox qcond from it instead of a separate pass in k_cond_prep
   double* qm;
   StatSlots stat;  // ST_COND_DROPS .. ST_WARM_RESTARTS
-  uint32_t* dlist;   // deferred drops, `dcap` per wave of k_cond_drops
-  uint32_t* dcount;  // deferred drops per wave (null: no deferral)
+  uint32_t* dlist;   // deferred drops (positions) of all waves, room for `dcap`
+  uint32_t* dcount;  // one counter: entries claimed in dlist, zeroed by k_cond_prep (null: no deferral)
   uint32_t dcap;
+  uint32_t* dseg;    // per-wave segments where a wave lists its deferred drops first, `dsegcap` each
+  uint32_t dsegcap;
   // mlp_inline: the learned guesses are evaluated by k_cond_guess into warm_buf inside the launch
   // sequence (after the saturation ratios exist) and read from there
   int32_t mlp_inline;
