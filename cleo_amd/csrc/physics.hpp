@@ -93,6 +93,18 @@ __device__ __forceinline__ double rcp_nr(double x) {
   const double r = __builtin_amdgcn_rcp(x);
   return fma(r, fma(-x, r, 1.0), r);
 }
+// n / d for the Newton update: n times the refined reciprocal of d and one correction step
+// (q + r (n - d q)), within an ulp of the quotient; a d or n outside [1e-290, 1e290] in magnitude
+// (or zero, inf, NaN) takes the division itself, so the special cases are those of n / d.
+__device__ __forceinline__ double quotient(double n, double d) {
+  const double ad = fabs(d), an = fabs(n);
+  if (ad > 1e-290 && ad < 1e290 && an < 1e290) {
+    const double r = rcp_nr(d);
+    const double q = n * r;
+    return fma(fma(-d, q, n), r, q);
+  }
+  return n / d;
+}
 // 1/sqrt(x): v_rsq_f64 and two Newton-Raphson steps y <- y + y/2 (1 - x y^2), within an ulp or
 // two of 1.0 / sqrt(x) (itself two roundings) for positive finite x; 0 and +inf give the
 // estimate's own inf and 0, negative or NaN x gives NaN, as 1.0 / sqrt(x) does.
@@ -230,6 +242,8 @@ struct NewtonCfg {
 // evaluation instead of six divisions and a root, each term within a few ulp of the reference's.
 __device__ __forceinline__ void ode_g_dg(const ODEConsts& c, double subdelt, double rprev, double rsqrd,
                                          double& g, double& dg) {
+  // products feeding a sum may fuse into FMAs here (one rounding fewer each)
+#pragma clang fp contract(fast)
   const double inv_r = rsqrt_nr(rsqrd);
   const double inv_r2 = inv_r * inv_r;
   const double a_r = c.akoh * inv_r;
@@ -253,7 +267,7 @@ struct NewtonState {
 __device__ __forceinline__ bool newton_iterate(const NewtonCfg& cfg, const ODEConsts& c, double subdelt,
                                                double rprev, NewtonState& s, uint32_t& iters) {
   ++iters;
-  double z = s.z * (1 - s.g / s.dg);
+  double z = s.z * (1 - quotient(s.g, s.dg));
   z = fmax(z, 1e-8);
   double g1, dg1;
   ode_g_dg(c, subdelt, rprev, z, g1, dg1);
