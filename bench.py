@@ -227,9 +227,9 @@ def run_gpu(args):
         ms = C.c_float()
         d.check(L.cleo_time_kernel(d.h, stage, 3, C.byref(ms)), "time_kernel")
         kernel_ms[stage_kernel(stage, tiles)] = float(ms.value)
-    if tiles.cond_flat == 1 and tiles.motion_flat == 1:
-        # the step solves condensation inside the sort's gather (k_sort_gather_cond) instead of
-        # in the separate per-drop kernel
+    if tiles.cond_flat == 1 and tiles.motion_flat in (1, 3):
+        # the step solves condensation inside the sort's gather (k_sort_gather_cond; with motion
+        # variant 3 the drops also move in it) instead of in the separate per-drop kernel
         del kernel_ms[stage_kernel(0, tiles)]
         ms = C.c_float()
         d.check(L.cleo_time_kernel(d.h, 3, 3, C.byref(ms)), "time_kernel fused")
@@ -324,6 +324,8 @@ def main():
     km = g["kernel_ms"]
     dom = max(km, key=km.get)
     bpd, bpg = BYTES[dom]
+    if dom == "k_sort_gather_cond" and g["tiles"].motion_flat == 3:
+        bpg += 6 * 8  # the drops move in the gather: the face winds of each gridbox
     alg_bytes = bpd * nd + bpg * g["ngbx"]
     achieved = alg_bytes / (km[dom] / 1e3) / 1e9
     traffic = measured_traffic(dom, g["ndims"])

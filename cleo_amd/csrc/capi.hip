@@ -287,8 +287,9 @@ int enqueue_run_step(cleo_ctx* c, uint32_t t_mdl, uint32_t t_mdl_next) {
       // a condensation due at once is solved in the sort's gather (thread-per-drop variants; not
       // while guesses are recorded, whose positions are those of the sorted store); otherwise
       // motion and sort run alone and qcond is recomputed afterwards
-      const bool fuse = cond && t_sdm % p.cond.interval == 0 && c->tiles.motion_flat == 1 &&
-                        c->tiles.cond_flat == 1 && !(p.cond.warm_start && c->rec_guess);
+      const bool fuse = cond && t_sdm % p.cond.interval == 0 &&
+                        (c->tiles.motion_flat == 1 || c->tiles.motion_flat == 3) && c->tiles.cond_flat == 1 &&
+                        !(p.cond.warm_start && c->rec_guess);
       const int rc = fuse ? enqueue_motion_condensation(c, p.motion, p.cond) : enqueue_motion(c, p.motion, false);
       if (rc != CLEO_OK) return rc;
       pending_qcond = !fuse;
@@ -1103,21 +1104,25 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
     }
   }
   // thread-per-drop motion and condensation: a condensation that follows motion runs inside the
-  // sort's gather, so its grid and deferral budget are chosen on that fused pass
-  if (best_t.cond_flat == 1 && best_t.motion_flat == 1) {
+  // sort's gather, so the motion variant (drops moved before the sort, or inside the gather), the
+  // gather's grid and the deferral budget are chosen on that fused pass
+  if (best_t.cond_flat == 1 && (best_t.motion_flat == 1 || best_t.motion_flat == 3)) {
     float best = 1e30f;
     const cleo_tiles base_t = best_t;
-    for (uint32_t gw : {16384u, 32768u, 65536u}) {
-      for (uint32_t budget : {6u, 8u, 12u}) {
-        cleo_tiles t = base_t;
-        t.cond_grid_waves = gw;
-        t.cond_defer = budget;
-        float tmin = 0.f;
-        const int rc = time_stage(c, 3, t, 2, false, &tmin, nullptr);
-        if (rc != CLEO_OK) return rc;
-        if (tmin < best) {
-          best = tmin;
-          best_t = t;
+    for (uint32_t mv : {1u, 3u}) {
+      for (uint32_t gw : {16384u, 32768u, 65536u}) {
+        for (uint32_t budget : {6u, 8u, 12u}) {
+          cleo_tiles t = base_t;
+          t.motion_flat = mv;
+          t.cond_grid_waves = gw;
+          t.cond_defer = budget;
+          float tmin = 0.f;
+          const int rc = time_stage(c, 3, t, 2, false, &tmin, nullptr);
+          if (rc != CLEO_OK) return rc;
+          if (tmin < best) {
+            best = tmin;
+            best_t = t;
+          }
         }
       }
     }

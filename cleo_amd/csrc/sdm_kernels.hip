@@ -1828,7 +1828,23 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_deferred(CondFla
 // sorted store (k_cond_state).  Drops that outlive cfg.budget iterations are written unsolved and
 // listed by their sorted position for k_cond_deferred, as in k_cond_drops.  The memory traffic of
 // the gather and the FP64 arithmetic of the solves overlap in one pass.
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_sort_gather_cond(NsortArgs n, CondFlatArgs a) {
+// kMove: the unsorted store holds the drops before motion (the counting pass k_motion_drops<false>
+// only recorded their steps); each drop moves here, with the same arithmetic, before its solve in
+// the gridbox it reaches, so motion writes nothing of its own.  kLds: the coordinate bounds are read
+// from an LDS copy (dynamic shared memory of nz + nx + ny + 3 doubles).
+template <bool kMove, bool kLds>
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_sort_gather_cond(NsortArgs n, CondFlatArgs a, MotionArgs m) {
+  extern __shared__ double bounds_lds[];
+  MotionArgs b = m;
+  if constexpr (kMove && kLds) {
+    const uint32_t nzh = m.grid.nz + 1, nxh = m.grid.nx + 1, nyh = m.grid.ny + 1;
+    for (uint32_t i = threadIdx.x; i < nzh + nxh + nyh; i += blockDim.x)
+      bounds_lds[i] = i < nzh ? m.grid.zh[i] : (i < nzh + nxh ? m.grid.xh[i - nzh] : m.grid.yh[i - nzh - nxh]);
+    __syncthreads();
+    b.grid.zh = bounds_lds;
+    b.grid.xh = bounds_lds + nzh;
+    b.grid.yh = bounds_lds + nzh + nxh;
+  }
   const uint64_t nsrc = n.off[n.grid.ngbx];
   const int lane = lane_id();
   CondStats st{0, 0, 0, 0};
@@ -1846,10 +1862,20 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_sort_gather_cond(Nsor
     const bool okp = inq && p < nsrc;
     if (inq && !okp) set_error(n.ctl, CLEO_ERR_SORT_NONLOCAL, kOOB, q);
     const uint32_t sp = okp ? p : 0u;
-    const uint32_t g = n.src.gbx[sp];
-    const double c3 = n.src.c3[sp], c1 = n.src.c1[sp], c2 = n.src.c2[sp];
+    uint32_t g = n.src.gbx[sp];
+    double c3 = n.src.c3[sp], c1 = n.src.c1[sp], c2 = n.src.c2[sp];
     const uint64_t xi = n.src.xi[sp], id = n.src.id[sp];
     const double r = n.src.r[sp], msol = n.src.msol[sp];
+    if constexpr (kMove) {
+      // an in-domain drop's gridbox is below ngbx; the guard keeps a corrupted one from indexing
+      const uint32_t gg = g < n.grid.ngbx ? g : 0u;
+      const GbxGeom G = gbx_geom(b, gg);
+      const Moved M = move_drop(b, gg, G, DropPos{c3, c1, c2, r}, sp, false);
+      c3 = M.c3;
+      c1 = M.c1;
+      c2 = M.c2;
+      g = M.idx;
+    }
     const bool active = okp && g < a.ngbx;
     const double warm = a.warm ? a.warm[sp] : nowarm;  // learned guesses by source position
     double rnew = r, dm = 0.0, qm = 0.0;
@@ -2300,6 +2326,21 @@ static void launch_gather_move(cleo_ctx* c, const NsortArgs& n, const MotionArgs
     hipLaunchKernelGGL(k_nsort_gather_move<false>, dim3(nb), dim3(256), 0, c->stream, n, m);
 }
 
+// the gather with the solves; with m, the drops also move in it (k_sort_gather_cond<true, ...>)
+static void launch_gather_cond(cleo_ctx* c, const NsortArgs& n, const CondFlatArgs& a, const MotionArgs* m,
+                               unsigned nb) {
+  if (!m) {
+    hipLaunchKernelGGL((k_sort_gather_cond<false, false>), dim3(nb), dim3(256), 0, c->stream, n, a, MotionArgs{});
+    return;
+  }
+  const uint32_t nb_ = m->grid.nz + m->grid.nx + m->grid.ny + 3;
+  if (nb_ <= kMotionLdsBounds)
+    hipLaunchKernelGGL((k_sort_gather_cond<true, true>), dim3(nb), dim3(256), nb_ * sizeof(double), c->stream, n, a,
+                       *m);
+  else
+    hipLaunchKernelGGL((k_sort_gather_cond<true, false>), dim3(nb), dim3(256), 0, c->stream, n, a, *m);
+}
+
 // per-wave segments of the deferred list for a drop launch of nb blocks: a quarter of a wave's
 // drops, at least 16
 static void set_defer_segments(const cleo_ctx* c, CondFlatArgs& a, unsigned nb) {
@@ -2363,9 +2404,21 @@ void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatAr
   if (ngbx == 0) return;
   MotionArgs m = margs;
   m.step = c->dstep;
+  // motion variant 3: the drops move inside the gather (the counting pass writes only the steps);
+  // not with learned guesses, which are evaluated on the moved store before the gather
+  const bool move_in_gather = c->tiles.motion_flat == 3 && !cond.mlp_inline && !cond.warm;
   const bool on = c->kev_on;
   c->kev_on = false;  // the main kernel of this sequence is the gather with the solves
-  launch_motion_flat(c, m);
+  if (move_in_gather) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(stride_blocks(28ull * ngbx)), dim3(256), 0, c->stream, m.cnt, 28ull * ngbx,
+                       0u);
+    const unsigned mb = motion_blocks(c->ntot, c->tiles.motion_grid_waves);
+    m.stat = stat_slots(c, 4ull * mb);
+    launch_motion_drops<false>(c, m, mb);
+    launch_stats_reduce(c, 4u * mb, ST_MOTION_DROPS, 1);
+  } else {
+    launch_motion_flat(c, m);
+  }
   c->kev_on = on;
   NsortArgs n = nsort_args(c, m.s, dst, nullptr);
   n.step = c->dstep;
@@ -2397,7 +2450,7 @@ void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatAr
   a.stat = stat_slots(c, 4ull * nb);
   set_defer_segments(c, a, nb);
   kev_mark(c, 0);
-  hipLaunchKernelGGL(k_sort_gather_cond, dim3(nb), dim3(256), 0, c->stream, n, a);
+  launch_gather_cond(c, n, a, move_in_gather ? &m : nullptr, nb);
   kev_mark(c, 1);
   if (budget) {
     launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);

@@ -268,7 +268,8 @@ typedef struct {
                                          atomics; 2: motion and sort fused (a counting pass, then
                                          each drop moved again straight to its sorted place);
                                       3: a counting pass, then the sort's gather moves each drop
-                                         as it copies it to its sorted place */
+                                         as it copies it to its sorted place (with cond_flat 1 and
+                                         a condensation due, the same gather also solves it) */
   uint32_t cond_grid_waves;        /* waves per launch; each loops over gridboxes (or drops) */
   uint32_t coll_grid_waves;        /*   0: one wave per gridbox (one thread per drop) */
   uint32_t motion_grid_waves;
