@@ -849,34 +849,100 @@ __global__ void k_nsort_tot(NsortArgs a) {
 // recorded as perm[dest] = p; k_nsort_gather then moves the records, so that each cache line of the
 // sorted store is written whole by one wave (runs scattered from many source gridboxes into one
 // line cost HBM read-modify-writes).
-__device__ __forceinline__ void nsort_rank_gbx(const NsortArgs& a, uint32_t g, int lane) {
-  const uint32_t sbase = a.off[g];
-  const uint32_t nfull = a.off[g + 1] - sbase;
+// The ranking of a gridbox is split in four stages so that a wave works on four gridboxes at once,
+// each stage reading what the previous one loaded an iteration earlier: the head (range and run
+// starts), the positions of the first two chunks of slots, the keys of those drops, then the
+// ranking itself.  Every load of a stage is issued by all lanes, with indices clamped into the
+// arrays, so the number of loads in flight is the same on every path and the waits of the ranking
+// leave the younger stages' loads outstanding.
+//
+// head of gridbox g: its range of positions and, in lane s < 27, the next place of its drops that
+// take step s (lane 27: of those leaving).  The neighbour reached by step s = (dz+1) + 3(dx+1) +
+// 9(dy+1) is g + noff away from the edges, else wrapped on periodic axes (the same gridbox as
+// neighbour_of, without a division per lane).  A g past the last gridbox gives an empty head.
+struct RankHead {
+  uint32_t g, sbase, nfull, trun;
+  int gz, gx, gy;
+};
+__device__ __forceinline__ RankHead rank_head(const NsortArgs& a, uint64_t g, int gz, int gx, int gy, int32_t noff,
+                                              int lane) {
   const uint32_t ngbx = a.grid.ngbx;
   const int nz = static_cast<int>(a.grid.nz), nx = static_cast<int>(a.grid.nx), ny = static_cast<int>(a.grid.ny);
-  const int gz = static_cast<int>(g % a.grid.nz), gx = static_cast<int>((g / a.grid.nz) % a.grid.nx),
-            gy = static_cast<int>(g / (a.grid.nz * a.grid.nx));
-  // lane s < 27: next place of this gridbox's drops that take step s; lane 27: of those leaving.
-  // The neighbour reached by step s = (dz+1) + 3(dx+1) + 9(dy+1), wrapped on periodic axes
-  // (the same gridbox as neighbour_of, without a division per lane).
-  uint32_t trun = 0;
-  if (lane < 27) {
-    auto axis = [](int i, int d, int n, bool per) {
-      int j = i + d;
-      if (j < 0) j = per ? j + n : -1;
-      if (j >= n) j = per ? j - n : -1;
-      return j;
-    };
-    const int jz = axis(gz, lane % 3 - 1, nz, a.grid.per[0] != 0);
-    const int jx = axis(gx, (lane / 3) % 3 - 1, nx, a.grid.per[1] != 0);
-    const int jy = axis(gy, lane / 9 - 1, ny, a.grid.per[2] != 0);
-    if (jz >= 0 && jx >= 0 && jy >= 0) {
-      const uint32_t d = static_cast<uint32_t>(jz + nz * (jx + nx * jy));
-      trun = a.base[d] + a.cnt[28ull * d + lane];
+  const bool valid = g < ngbx;
+  const uint32_t gc = valid ? static_cast<uint32_t>(g) : ngbx - 1u;
+  RankHead h{gc, 0u, 0u, 0u, gz, gx, gy};
+  const uint32_t lo = a.off[gc], hi = a.off[gc + 1];
+  h.sbase = lo;
+  h.nfull = valid ? hi - lo : 0u;
+  uint32_t d = ngbx;  // lanes 27..63: the out-of-domain run (base[ngbx] + oobpre[g])
+  bool ok = valid && lane == 27;
+  if (lane < 27 && !valid) {
+    d = 0u;  // no gridbox: a harmless load, discarded
+  } else if (lane < 27) {
+    const bool interior = gz > 0 && gz + 1 < nz && gx > 0 && gx + 1 < nx && gy > 0 && gy + 1 < ny;
+    d = gc + static_cast<uint32_t>(noff);
+    ok = true;
+    if (!interior) {
+      auto axis = [](int i, int dd, int n, bool per) {
+        int j = i + dd;
+        if (j < 0) j = per ? j + n : -1;
+        if (j >= n) j = per ? j - n : -1;
+        return j;
+      };
+      const int jz = axis(gz, lane % 3 - 1, nz, a.grid.per[0] != 0);
+      const int jx = axis(gx, (lane / 3) % 3 - 1, nx, a.grid.per[1] != 0);
+      const int jy = axis(gy, lane / 9 - 1, ny, a.grid.per[2] != 0);
+      ok = jz >= 0 && jx >= 0 && jy >= 0;
+      d = ok ? static_cast<uint32_t>(jz + nz * (jx + nx * jy)) : 0u;
     }
-  } else if (lane == 27) {
-    trun = a.base[ngbx] + a.oobpre[g];
   }
+  const uint32_t b = a.base[d];
+  const uint32_t extra = lane < 27 ? a.cnt[28ull * (d < ngbx ? d : 0u) + static_cast<uint32_t>(lane)] : a.oobpre[gc];
+  h.trun = ok ? b + extra : 0u;
+  return h;
+}
+// positions held by slots lane and lane + 64 of the gridbox's logical order (the order collisions
+// left, or the store's own order); kOOB past the gridbox
+struct RankSlots {
+  uint32_t p0, p1;
+};
+template <bool kOrder>
+__device__ __forceinline__ RankSlots rank_slots(const NsortArgs& a, const RankHead& h, uint32_t nsrc, int lane) {
+  const uint32_t i0 = static_cast<uint32_t>(lane), i1 = i0 + kWave;
+  const uint32_t q0 = min(h.sbase + i0, nsrc - 1u), q1 = min(h.sbase + i1, nsrc - 1u);
+  uint32_t p0 = q0, p1 = q1;
+  if constexpr (kOrder) {
+    p0 = a.order[q0];
+    p1 = a.order[q1];
+  }
+  return RankSlots{i0 < h.nfull ? p0 : kOOB, i1 < h.nfull ? p1 : kOOB};
+}
+// the step recorded for each of those drops (kStep), or its gridbox key; 0 for no drop
+struct RankKeys {
+  uint32_t k0, k1;
+};
+template <bool kStep>
+__device__ __forceinline__ RankKeys rank_keys(const NsortArgs& a, const RankSlots& s, uint32_t nsrc) {
+  const uint32_t c0 = s.p0 < nsrc ? s.p0 : 0u, c1 = s.p1 < nsrc ? s.p1 : 0u;
+  uint32_t k0, k1;
+  if constexpr (kStep) {
+    k0 = a.step[c0];
+    k1 = a.step[c1];
+  } else {
+    k0 = a.src.gbx[c0];
+    k1 = a.src.gbx[c1];
+  }
+  return RankKeys{k0, k1};
+}
+// ranks the drops of the head's gridbox and records perm[dest] = p (stable scatter above); the
+// first two chunks use the preloaded positions and keys, later chunks load their own
+template <bool kOrder, bool kStep>
+__device__ __forceinline__ void rank_gbx(const NsortArgs& a, const RankHead& h, const RankSlots& s, const RankKeys& k,
+                                         uint32_t nsrc, int lane) {
+  const uint32_t g = h.g, sbase = h.sbase, nfull = h.nfull, ngbx = a.grid.ngbx;
+  const int nz = static_cast<int>(a.grid.nz), nx = static_cast<int>(a.grid.nx), ny = static_cast<int>(a.grid.ny);
+  const int gz = h.gz, gx = h.gx, gy = h.gy;
+  uint32_t trun = h.trun;
   auto near = [](int q, int r, int nd, bool per) {
     const int dd = q - r;
     if (dd >= -1 && dd <= 1) return true;
@@ -886,13 +952,10 @@ __device__ __forceinline__ void nsort_rank_gbx(const NsortArgs& a, uint32_t g, i
     const int dd = q - r;
     return dd > 1 ? -1 : (dd < -1 ? 1 : dd);
   };
-  const uint32_t nsrc = a.off[ngbx];
-  for (uint32_t c = 0; c < nfull; c += kWave) {
-    // slot q of the gridbox's logical order holds the drop at position p (the order collisions
-    // left, or the store's own order)
+  // chunk c of 64 slots: the drop of slot sbase + c + lane is at position p with key kv
+  auto chunk = [&](uint32_t c, uint32_t p, uint32_t kv) {
     const uint32_t q = sbase + c + lane;
     const bool act = c + lane < nfull;
-    const uint32_t p = (act && a.order) ? a.order[q] : q;
     // the step g -> key (27: out of the domain): recorded by motion, or found from the key; a key
     // that is no neighbour of g, or a position outside the domain, is an error
     int sv = 27;
@@ -901,18 +964,17 @@ __device__ __forceinline__ void nsort_rank_gbx(const NsortArgs& a, uint32_t g, i
       set_error(a.ctl, CLEO_ERR_SORT_NONLOCAL, g, q);
       ok = false;
     }
-    const uint32_t key = (ok && !a.step) ? a.src.gbx[p] : 0u;
-    if (ok && a.step) {
-      sv = a.step[p];
+    if (ok && kStep) {
+      sv = static_cast<int>(kv);
       if (sv > 27) {
         set_error(a.ctl, CLEO_ERR_SORT_NONLOCAL, g, p);
         ok = false;
       }
-    } else if (ok && key == g) {
+    } else if (ok && kv == g) {
       sv = 13;
-    } else if (ok && key < ngbx) {
-      const int vz = static_cast<int>(key % a.grid.nz), vx = static_cast<int>((key / a.grid.nz) % a.grid.nx),
-                vy = static_cast<int>(key / (a.grid.nz * a.grid.nx));
+    } else if (ok && kv < ngbx) {
+      const int vz = static_cast<int>(kv % a.grid.nz), vx = static_cast<int>((kv / a.grid.nz) % a.grid.nx),
+                vy = static_cast<int>(kv / (a.grid.nz * a.grid.nx));
       if (near(vz, gz, nz, a.grid.per[0]) && near(vx, gx, nx, a.grid.per[1]) && near(vy, gy, ny, a.grid.per[2])) {
         sv = (step(vz, gz) + 1) + 3 * (step(vx, gx) + 1) + 9 * (step(vy, gy) + 1);
       } else {
@@ -936,15 +998,74 @@ __device__ __forceinline__ void nsort_rank_gbx(const NsortArgs& a, uint32_t g, i
     } else if (ok) {
       a.perm[dest] = p;
     }
+  };
+  // the first two chunks from the preloaded positions and keys, straight-line; larger gridboxes
+  // (rare at these sizes) load the rest as they go
+  if (nfull > 0) chunk(0u, s.p0, k.k0);
+  if (nfull > kWave) chunk(kWave, s.p1, k.k1);
+  for (uint32_t c = 2 * kWave; c < nfull; c += kWave) {
+    const bool act = c + lane < nfull;
+    const uint32_t q = sbase + c + lane;
+    const uint32_t p = act ? (kOrder ? a.order[q] : q) : kOOB;
+    const uint32_t kv = (act && p < nsrc) ? (kStep ? a.step[p] : a.src.gbx[p]) : 0u;
+    chunk(c, p, kv);
   }
 }
 // first pass of the neighbour sort: the destination of every drop of the domain, as perm[dest] = p.
 // Gridboxes go to waves through xcd_wave_index(), so each XCD sweeps one contiguous range and the
-// count-table rows that neighbouring gridboxes share are fetched into one L2.
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(8) k_nsort_perm(NsortArgs a) {
+// count-table rows that neighbouring gridboxes share are fetched into one L2.  The gridbox's
+// (z, x, y) advance by the stride's with carries (no division per gridbox).
+template <bool kOrder, bool kStep>
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(kStep ? 6 : 4) k_nsort_perm(NsortArgs a) {
   if (guard_off(a.flag)) return;
   const int lane = lane_id();
-  for (uint32_t g = xcd_wave_index(); g < a.grid.ngbx; g += wave_count()) nsort_rank_gbx(a, g, lane);
+  const uint32_t nz = a.grid.nz, nx = a.grid.nx;
+  const uint64_t ngbx = a.grid.ngbx;
+  const uint32_t g0 = xcd_wave_index(), stride = wave_count();
+  const uint32_t nsrc = a.off[ngbx];
+  if (g0 >= ngbx || nsrc == 0) return;
+  uint32_t gz = g0 % nz, gx = (g0 / nz) % nx, gy = g0 / (nz * nx);
+  const uint32_t sz = stride % nz, sx = (stride / nz) % nx, sy = stride / (nz * nx);
+  const int32_t noff = (lane % 3 - 1) + static_cast<int32_t>(nz) * ((lane / 3) % 3 - 1) +
+                       static_cast<int32_t>(nz * nx) * (lane / 9 - 1);
+  uint64_t gh = g0;  // next gridbox whose head is loaded
+  auto next_head = [&]() {
+    const RankHead h = rank_head(a, gh, static_cast<int>(gz), static_cast<int>(gx), static_cast<int>(gy), noff, lane);
+    gh += stride;
+    gz += sz;
+    uint32_t carry = gz >= nz ? 1u : 0u;
+    gz -= carry * nz;
+    gx += sx + carry;
+    carry = gx >= nx ? 1u : 0u;
+    gx -= carry * nx;
+    gy += sy + carry;
+    return h;
+  };
+  // rings of four per stage, rotated by name over a four-fold unrolled loop: no register moves
+  // between iterations, which would wait for the loads in flight
+  RankHead H0 = next_head(), H1 = next_head(), H2, H3;
+  RankSlots S0 = rank_slots<kOrder>(a, H0, nsrc, lane), S1, S2, S3;
+  H2 = next_head();
+  S1 = rank_slots<kOrder>(a, H1, nsrc, lane);
+  RankKeys K0 = rank_keys<kStep>(a, S0, nsrc), K1, K2, K3;
+  // gridbox i: loads the head of i + 3, the slots of i + 2, the keys of i + 1, then ranks i
+  auto stage = [&](const RankHead& hu, RankHead& hl, const RankHead& hs, const RankSlots& su, RankSlots& sl,
+                   const RankSlots& sk, const RankKeys& ku, RankKeys& kl) {
+    hl = next_head();
+    sl = rank_slots<kOrder>(a, hs, nsrc, lane);
+    kl = rank_keys<kStep>(a, sk, nsrc);
+    rank_gbx<kOrder, kStep>(a, hu, su, ku, nsrc, lane);
+  };
+  for (uint64_t g = g0;;) {
+    stage(H0, H3, H2, S0, S2, S1, K0, K1);
+    if ((g += stride) >= ngbx) break;
+    stage(H1, H0, H3, S1, S3, S2, K1, K2);
+    if ((g += stride) >= ngbx) break;
+    stage(H2, H1, H0, S2, S0, S3, K2, K3);
+    if ((g += stride) >= ngbx) break;
+    stage(H3, H2, H1, S3, S1, S0, K3, K0);
+    if ((g += stride) >= ngbx) break;
+  }
 }
 
 // Second pass: destination position q takes the record at source position perm[q].  Consecutive
@@ -1314,6 +1435,18 @@ static inline unsigned drop_blocks(uint64_t n, uint32_t grid_waves) {
   const unsigned b = nblocks(n, 256);
   return grid_waves ? std::min<unsigned>(b, std::max<unsigned>(1u, grid_waves / 4)) : b;
 }
+// launches k_nsort_perm for the order and key source of `a`
+static void launch_nsort_perm(cleo_ctx* c, const NsortArgs& a, uint32_t ngbx) {
+  const dim3 grid(gbx_blocks(ngbx, a.wpb, c->tiles.motion_grid_waves)), block(64 * a.wpb);
+  if (a.order && a.step)
+    hipLaunchKernelGGL((k_nsort_perm<true, true>), grid, block, 0, c->stream, a);
+  else if (a.order)
+    hipLaunchKernelGGL((k_nsort_perm<true, false>), grid, block, 0, c->stream, a);
+  else if (a.step)
+    hipLaunchKernelGGL((k_nsort_perm<false, true>), grid, block, 0, c->stream, a);
+  else
+    hipLaunchKernelGGL((k_nsort_perm<false, false>), grid, block, 0, c->stream, a);
+}
 static inline unsigned stride_blocks(uint64_t n) {
   const uint64_t b = (n + 255) / 256;
   return static_cast<unsigned>(b == 0 ? 1 : (b > 65536 ? 65536 : b));
@@ -1444,8 +1577,7 @@ static NsortArgs nsort_args(cleo_ctx* c, const SoA& src, const SoA& dst, const u
 // the drops of the domain to their sorted places: destinations, then the records
 static void nsort_move(cleo_ctx* c, const NsortArgs& a) {
   const uint32_t ngbx = c->grid.ngbx;
-  hipLaunchKernelGGL(k_nsort_perm, dim3(gbx_blocks(ngbx, a.wpb, c->tiles.motion_grid_waves)), dim3(64 * a.wpb), 0,
-                     c->stream, a);
+  launch_nsort_perm(c, a, ngbx);
   // streaming copy: one pair of positions per thread (measured faster than a capped grid)
   hipLaunchKernelGGL(k_nsort_gather, dim3(motion_blocks(c->ntot, 0)), dim3(256), 0, c->stream, a);
 }
@@ -1568,8 +1700,7 @@ void motion_sort_gather(cleo_ctx* c, const MotionArgs& args, const SoA& dst, int
   NsortArgs n = nsort_args(c, a.s, dst, nullptr);
   n.step = c->dstep;
   nsort_tables(c, n, nullptr);
-  hipLaunchKernelGGL(k_nsort_perm, dim3(gbx_blocks(ngbx, n.wpb, c->tiles.motion_grid_waves)), dim3(64 * n.wpb), 0,
-                     c->stream, n);
+  launch_nsort_perm(c, n, ngbx);
   launch_gather_move(c, n, a);
   hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, n, c->ntot);
   hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, static_cast<const uint32_t*>(nullptr));
@@ -2423,8 +2554,7 @@ void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatAr
   NsortArgs n = nsort_args(c, m.s, dst, nullptr);
   n.step = c->dstep;
   nsort_tables(c, n, nullptr);
-  hipLaunchKernelGGL(k_nsort_perm, dim3(gbx_blocks(ngbx, n.wpb, c->tiles.motion_grid_waves)), dim3(64 * n.wpb), 0,
-                     c->stream, n);
+  launch_nsort_perm(c, n, ngbx);
   CondFlatArgs a = cond;
   a.s = dst;
   a.qm = c->qm;  // the qcond recompute joins k_cond_state
