@@ -236,11 +236,15 @@ def run_gpu(args):
         kernel_ms["k_sort_gather_cond"] = float(ms.value)
     ngbx = int(np.prod(ndims))
     nd = ndomain(d)
+    # the mass-moment observer's reduction on the state the steps left (not part of the step)
+    ms = C.c_float()
+    d.check(L.cleo_time_kernel(d.h, 4, 3, C.byref(ms)), "time_kernel mass moments")
+    observer_ms = float(ms.value)
     d.check(L.cleo_synchronize(d.h), "final sync")
     err = d.error()
     d.close()
     return dict(n=n, ngbx=ngbx, ndims=ndims, plain=plain, stage_ms=stage_ms, kernel_ms=kernel_ms, nd=nd, tiles=tiles,
-                gen_s=t_gen, learned=learned, err=err)
+                gen_s=t_gen, learned=learned, err=err, observer_ms=observer_ms)
 
 
 def learned_leg(args, b, d, sdm, t, timed_steps, torch):
@@ -358,6 +362,9 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": alg_bytes, "kernel_ms": km[dom]},
         "kernel_ms": km,
+        # MassMomentsObserver's reduction: read xi, r, msol | write mom0, mom1, mom2 per gridbox
+        "observer": {"kernel": "k_mass_moments", "ms": g["observer_ms"],
+                     "GB/s": (24 * nd + 16 * g["ngbx"]) / (g["observer_ms"] / 1e3) / 1e9},
         "cpu_baseline": cpu,
     }
     if g["learned"]:

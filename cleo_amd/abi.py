@@ -20,8 +20,9 @@ CLEO_E_STATE = -4
 ERR_NAMES = {
     0: "NONE", 1: "XI_ZERO", 2: "RADIUS_BELOW_DRY", 3: "NEG_CONDENSATE", 4: "NEWTON_MAXITERS",
     5: "TEMP_NONPOSITIVE", 6: "COAL_UNDEFINED", 7: "BREAKUP_NFRAGS", 8: "CFL", 9: "BOUNDS",
-    10: "SORT_NONLOCAL", 11: "SUBSTEP", 12: "INJECTED_INDEX",
+    10: "SORT_NONLOCAL", 11: "SUBSTEP", 12: "INJECTED_INDEX", 13: "XI_OVERFLOW",
 }
+ERR_XI_OVERFLOW = 13
 
 PROB_GOLOVIN, PROB_LONG, PROB_CONST = 0, 1, 2
 ENACT_COAL, ENACT_BREAKUP, ENACT_COALBURE = 0, 1, 2
@@ -33,6 +34,7 @@ MLP_NPARAMS = 64 * 6 + 64 + 64 * 64 + 64 + 64 + 1
 MLP_NBLOB = MLP_NPARAMS + 6 + 6 + 2
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _u32p = C.POINTER(C.c_uint32)
 _u64p = C.POINTER(C.c_uint64)
 
@@ -163,6 +165,7 @@ _SIGS = {
     "download_niters": (C.c_int, [C.c_void_p, _u32p, C.c_uint64]),
     "set_tiles": (C.c_int, [C.c_void_p, C.c_void_p]),
     "get_tiles": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mass_moments": (C.c_int, [C.c_void_p, C.c_int32, _u64p, _fp, _fp]),
 }
 
 
@@ -239,6 +242,15 @@ class Domain:
         r = np.zeros(self.ngbx + 1, dtype=np.uint32)
         self.check(self.b.download_refs(self.h, _ptr(r, C.c_uint32)), "download_refs")
         return r
+
+    def mass_moments(self, rain=False):
+        """(mom0, mom1, mom2) per gridbox: sum of xi, and float sums of xi m and xi m^2."""
+        m0 = np.zeros(self.ngbx, dtype=np.uint64)
+        m1 = np.zeros(self.ngbx, dtype=np.float32)
+        m2 = np.zeros(self.ngbx, dtype=np.float32)
+        self.check(self.b.mass_moments(self.h, int(bool(rain)), _ptr(m0, C.c_uint64), _ptr(m1, C.c_float),
+                                       _ptr(m2, C.c_float)), "mass_moments")
+        return m0, m1, m2
 
     def set_state(self, st):
         arrs = {k: np.ascontiguousarray(st[k], dtype=np.float64) for k in STATE_FIELDS}

@@ -424,7 +424,7 @@ int cleo_destroy(cleo_ctx* c) {
                   static_cast<void*>(c->rs_key[1]), static_cast<void*>(c->rs_idx[0]), static_cast<void*>(c->rs_idx[1]),
                   static_cast<void*>(c->rs_cnt), static_cast<void*>(c->gs), static_cast<void*>(c->dm),
                   static_cast<void*>(c->sp), static_cast<void*>(c->dlist), static_cast<void*>(c->dcount),
-                  static_cast<void*>(c->dseg), static_cast<void*>(c->perm), static_cast<void*>(c->dstep), static_cast<void*>(c->order),
+                  static_cast<void*>(c->dseg), c->mom, static_cast<void*>(c->perm), static_cast<void*>(c->dstep), static_cast<void*>(c->order),
                   static_cast<void*>(c->qm)})
     if (p) hipFree(p);
   if (c->h_ctl) hipHostFree(c->h_ctl);
@@ -567,6 +567,27 @@ int cleo_download_refs(cleo_ctx* c, uint32_t* refs) {
   const int rc = sync_check(c);
   HIP_TRY(c, ctx_copy(c, refs, c->off, (c->grid.ngbx + 1ull) * 4, hipMemcpyDeviceToHost));
   return rc;
+}
+
+// MassMomentsObserver's reductions (observers/massmoments_observer.cpp:23-73) on the device,
+// copied to the host arrays
+int cleo_mass_moments(cleo_ctx* c, int32_t rain, uint64_t* mom0, float* mom1, float* mom2) {
+  if (!c || !mom0 || !mom1 || !mom2) return CLEO_E_ARG;
+  const uint64_t ngbx = c->grid.ngbx;
+  if (!c->mom) {
+    uint8_t* p = nullptr;
+    HIP_TRY(c, dalloc(&p, 16 * ngbx));
+    c->mom = p;
+  }
+  uint64_t* d0 = static_cast<uint64_t*>(c->mom);
+  float* d1 = reinterpret_cast<float*>(d0 + ngbx);
+  float* d2 = d1 + ngbx;
+  launch_mass_moments(c, rain != 0, d0, d1, d2);
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, ctx_copy(c, mom0, d0, ngbx * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, mom1, d1, ngbx * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(c, ctx_copy(c, mom2, d2, ngbx * 4, hipMemcpyDeviceToHost));
+  return sync_check(c);
 }
 
 static double* const* state_fields(cleo_ctx* c, double* (&out)[11]) {
@@ -1133,9 +1154,31 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
 }
 
 int cleo_time_kernel(cleo_ctx* c, int32_t stage, uint32_t reps, float* mean_ms) {
-  if (!c || !mean_ms || stage < 0 || stage > 3 || reps == 0) return CLEO_E_ARG;
+  if (!c || !mean_ms || stage < 0 || stage > 4 || reps == 0) return CLEO_E_ARG;
   if (c->ntot == 0) return fail(c, CLEO_E_STATE, "no superdroplets resident");
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (stage == 4) {  // the mass-moment reduction (reads only: the drops stay as they are)
+    const uint64_t ngbx = c->grid.ngbx;
+    if (!c->mom) {
+      uint8_t* p = nullptr;
+      HIP_TRY(c, dalloc(&p, 16 * ngbx));
+      c->mom = p;
+    }
+    uint64_t* d0 = static_cast<uint64_t*>(c->mom);
+    float* d1 = reinterpret_cast<float*>(d0 + ngbx);
+    materialize_order(c);
+    for (int s = 0; s < 2; ++s)
+      if (!c->kev[s]) HIP_TRY(c, hipEventCreate(&c->kev[s]));
+    launch_mass_moments(c, false, d0, d1, d1 + ngbx);  // warm-up
+    HIP_TRY(c, hipEventRecord(c->kev[0], c->stream));
+    for (uint32_t r = 0; r < reps; ++r) launch_mass_moments(c, false, d0, d1, d1 + ngbx);
+    HIP_TRY(c, hipEventRecord(c->kev[1], c->stream));
+    HIP_TRY(c, hipEventSynchronize(c->kev[1]));
+    float ms = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->kev[0], c->kev[1]));
+    *mean_ms = ms / static_cast<float>(reps);
+    return sync_check(c);
+  }
   return time_stage(c, stage, c->tiles, static_cast<int>(reps), true, nullptr, mean_ms);
 }
 

@@ -120,6 +120,8 @@ struct cleo_ctx {
   bool order_live = false;
   // per drop: condensed mass * xi before a condensation solve (the qcond recompute after motion)
   double* qm = nullptr;
+  // mass-moment observer results (ngbx uint64 + 2 * ngbx float, allocated at first use)
+  void* mom = nullptr;
   uint64_t sp_slots = 0;  // slots per counter: max(ngbx, waves of a thread-per-drop launch)
   // per-drop Newton iteration record
   bool rec_niters = false;

@@ -2309,6 +2309,76 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_state(CondFlatAr
   }
 }
 
+// ============================ observers =======================================================
+// Mass moments per gridbox (observers/massmoments_observer.cpp:23-73): one lane per gridbox of a
+// 64-gridbox tile.  The tile's drops pass through an LDS window with coalesced loads; the loading
+// lane turns each drop into its three terms (xi; (float)(xi m); (float)(xi m m), m the drop mass,
+// zero for a drop below the raindrop radius when kRain), and each lane then adds its gridbox's
+// terms left to right, so the float sums are the reference's serial ones.
+constexpr uint32_t kMomWin = 512;
+template <bool kRain>
+__global__ void __launch_bounds__(256) k_mass_moments(SoA s, const uint32_t* off, uint32_t ngbx, Solute sol,
+                                                      DevCtl* ctl, uint64_t* mom0, float* mom1, float* mom2) {
+  __shared__ uint64_t w0_all[4][kMomWin];
+  __shared__ float w1_all[4][kMomWin], w2_all[4][kMomWin];
+  const uint32_t wv = threadIdx.x >> 6;
+  uint64_t* w0 = w0_all[wv];
+  float* w1 = w1_all[wv];
+  float* w2 = w2_all[wv];
+  const int lane = lane_id();
+  constexpr double rlim(40e-6 / dimless::R0);  // calculate_rainmassmoments: raindrop radius
+  const uint32_t ntiles = (ngbx + kWave - 1) / kWave;
+  for (uint32_t tile = wave_index(); tile < ntiles; tile += wave_count()) {
+    const uint32_t g0 = tile * kWave, g = g0 + static_cast<uint32_t>(lane);
+    const bool valid = g < ngbx;
+    const uint32_t t_lo = off[g0], t_hi = off[min(g0 + kWave, ngbx)];
+    const uint32_t lo = valid ? off[g] : t_hi, hi = valid ? off[g + 1] : t_hi;
+    uint64_t m0 = 0;
+    float m1 = 0.f, m2 = 0.f;
+    for (uint32_t w = t_lo; w < t_hi; w += kMomWin) {
+      const uint32_t we = min(w + kMomWin, t_hi);
+      for (uint32_t j = static_cast<uint32_t>(lane); j < we - w; j += kWave) {
+        const uint32_t k = w + j;
+        const uint64_t xi = s.xi[k];
+        const double r = s.r[k];
+        if (xi == ~0ull) set_error(ctl, CLEO_ERR_XI_OVERFLOW, s.gbx[k], k);
+        const bool in = !kRain || r >= rlim;
+        const double m = drop_mass(r, s.msol[k], sol);
+        const double xm = static_cast<double>(xi) * m;
+        w0[j] = in ? xi : 0ull;
+        w1[j] = in ? static_cast<float>(xm) : 0.f;
+        w2[j] = in ? static_cast<float>(xm * m) : 0.f;
+      }
+      wave_sync();
+      const uint32_t b0 = max(lo, w), b1 = min(hi, we);
+      for (uint32_t k = b0; k < b1; ++k) {
+        m0 += w0[k - w];
+        m1 += w1[k - w];
+        m2 += w2[k - w];
+      }
+      wave_sync();
+    }
+    if (valid) {
+      mom0[g] = m0;
+      mom1[g] = m1;
+      mom2[g] = m2;
+    }
+  }
+}
+void launch_mass_moments(cleo_ctx* c, bool rain, uint64_t* mom0, float* mom1, float* mom2) {
+  const uint32_t ngbx = c->grid.ngbx;
+  if (ngbx == 0) return;
+  materialize_order(c);  // the sums follow the drops' logical order
+  const uint64_t ntiles = (ngbx + kWave - 1) / kWave;
+  const unsigned nb = gbx_blocks(ntiles, 4, 0);
+  if (rain)
+    hipLaunchKernelGGL(k_mass_moments<true>, dim3(nb), dim3(256), 0, c->stream, c->buf[c->cur], c->off, ngbx,
+                       c->solute, c->ctl, mom0, mom1, mom2);
+  else
+    hipLaunchKernelGGL(k_mass_moments<false>, dim3(nb), dim3(256), 0, c->stream, c->buf[c->cur], c->off, ngbx,
+                       c->solute, c->ctl, mom0, mom1, mom2);
+}
+
 // p[i] = i for i < n, skipped unless *flag when flag is non-null
 __global__ void k_iota_u32(uint32_t* p, uint64_t n, const uint32_t* flag) {
   if (flag && *flag == 0u) return;

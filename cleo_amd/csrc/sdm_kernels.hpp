@@ -174,6 +174,9 @@ void generic_sort(cleo_ctx* c);
 void launch_copy_oob_region(cleo_ctx* c, int src, int dst);
 // applies a pending collision order to the store (buf[cur] then holds the drops in order)
 void materialize_order(cleo_ctx* c);
+// 0th, 1st and 2nd mass moments of every gridbox into device arrays of ngbx entries (raindrops
+// only when rain)
+void launch_mass_moments(cleo_ctx* c, bool rain, uint64_t* mom0, float* mom1, float* mom2);
 // the identity as the pending order (the store unchanged)
 void pending_identity_order(cleo_ctx* c);
 

@@ -225,6 +225,16 @@ class GpuDomain {
     check(ctx_, cleo_get_stats(ctx_, &s), "stats");
     return s;
   }
+  // MassMomentsObserver's per-gridbox moments (massmoments_observer.cpp:23-73); rain: raindrops only
+  struct MassMoments {
+    std::vector<uint64_t> mom0;
+    std::vector<float> mom1, mom2;
+  };
+  MassMoments mass_moments(bool rain = false) {
+    MassMoments m{std::vector<uint64_t>(ngbx_), std::vector<float>(ngbx_), std::vector<float>(ngbx_)};
+    check(ctx_, cleo_mass_moments(ctx_, rain ? 1 : 0, m.mom0.data(), m.mom1.data(), m.mom2.data()), "mass moments");
+    return m;
+  }
 
  private:
   cleo_ctx* ctx_ = nullptr;

@@ -55,7 +55,8 @@ enum {
   CLEO_ERR_BOUNDS = 9,           /* cartesian_motion.hpp:44-52 */
   CLEO_ERR_SORT_NONLOCAL = 10,   /* a key left the 27-neighbourhood: neighbour sort not applicable */
   CLEO_ERR_SUBSTEP = 11,         /* impliciteuler.hpp:336-339 delt < minsubdelt */
-  CLEO_ERR_INJECTED_INDEX = 12   /* an injected Fisher-Yates index j > iter */
+  CLEO_ERR_INJECTED_INDEX = 12,  /* an injected Fisher-Yates index j > iter */
+  CLEO_ERR_XI_OVERFLOW = 13      /* massmoments_observer.cpp:31-35 xi >= uint64 max */
 };
 
 typedef struct cleo_ctx cleo_ctx;
@@ -256,6 +257,15 @@ int cleo_download_niters(cleo_ctx* ctx, uint32_t* niters, uint64_t n);
 int cleo_record_guess(cleo_ctx* ctx, int32_t enable);
 int cleo_download_guess(cleo_ctx* ctx, double* z, uint64_t n);
 
+/* ---- observers ----------------------------------------------------------------------- */
+/* 0th, 1st and 2nd moments of the droplet mass distribution of every gridbox
+   (MassMomentsObserver: observers/massmoments_observer.cpp:23-73): mom0[g] = sum of xi,
+   mom1[g] = sum of (float)(xi * m), mom2[g] = sum of (float)(xi * m * m), summed in float in the
+   order of the gridbox's drops, with m the drop mass (superdrop_attrs.cpp:55-62).  rain != 0:
+   raindrops only, radius >= 40 micrometres (calculate_rainmassmoments).  mom0, mom1 and mom2 are
+   host arrays of ngbx entries.  A multiplicity of 2^64 - 1 records CLEO_ERR_XI_OVERFLOW. */
+int cleo_mass_moments(cleo_ctx* ctx, int32_t rain, uint64_t* mom0, float* mom1, float* mom2);
+
 /* Launch tiling, chosen by the autotuner; values 0 = default. */
 typedef struct {
   uint32_t cond_waves_per_block;   /* gridboxes (waves) per workgroup, 1..4 */
@@ -285,7 +295,8 @@ int cleo_autotune(cleo_ctx* ctx, cleo_tiles* chosen);
    `reps` launches with the current tiling, on a copy of the resident drops (drops and State stay
    unchanged): stage 0 condensation (the per-drop kernel), 1 collisions, 2 motion, 3 motion + sort
    + condensation with the solves in the sort's gather (the gather kernel; the path cleo_run_step
-   takes when condensation follows motion with thread-per-drop variants). */
+   takes when condensation follows motion with thread-per-drop variants), 4 the mass-moment
+   reduction of cleo_mass_moments (all drops, device arrays only). */
 int cleo_time_kernel(cleo_ctx* ctx, int32_t stage, uint32_t reps, float* mean_ms);
 
 /* ---- learned warm start ------------------------------------------------------------- */

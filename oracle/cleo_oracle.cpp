@@ -1116,6 +1116,32 @@ int orc_download_refs(orc_ctx* h, uint32_t* refs) {
   refs[c.ngbx] = static_cast<uint32_t>(c.ndomain);
   return CLEO_OK;
 }
+/* calculate_massmoments / calculate_rainmassmoments (observers/massmoments_observer.cpp:23-73) in
+   Serial order: per gridbox, over its drops in order, mom0 += xi, mom1 += (float)(xi * mass),
+   mom2 += (float)(xi * mass * mass); rain: only drops with radius >= 40 micrometres */
+int orc_mass_moments(orc_ctx* h, int32_t rain, uint64_t* mom0, float* mom1, float* mom2) {
+  Ctx& c = *reinterpret_cast<Ctx*>(h);
+  if (!mom0 || !mom1 || !mom2) return CLEO_E_ARG;
+  constexpr double rlim(40e-6 / dlc::R0);
+  for (uint32_t g = 0; g < c.ngbx; ++g) {
+    uint64_t m0 = 0;
+    float m1 = 0.f, m2 = 0.f;
+    for (uint64_t k = c.ref0[g]; k < c.ref1[g]; ++k) {
+      const Superdrop& d = c.sup[k];
+      if (d.attrs.xi >= UINT64_MAX) ErrSink{&c.err, g, k}.set(CLEO_ERR_XI_OVERFLOW);
+      const bool binary = rain ? d.attrs.radius >= rlim : true;
+      m0 += static_cast<uint64_t>(d.attrs.xi * binary);
+      const double m = mass(d, c.sol);
+      const double xi = static_cast<double>(d.attrs.xi);
+      m1 += static_cast<float>(xi * m * binary);
+      m2 += static_cast<float>(xi * m * m * binary);
+    }
+    mom0[g] = m0;
+    mom1[g] = m1;
+    mom2[g] = m2;
+  }
+  return c.err.code ? CLEO_E_DEVICE : CLEO_OK;
+}
 int orc_upload_gbx_state(orc_ctx* h, const cleo_gbx_state* s) {
   Ctx& c = *reinterpret_cast<Ctx*>(h);
   for (uint32_t g = 0; g < c.ngbx; ++g) {
