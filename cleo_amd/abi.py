@@ -16,6 +16,8 @@ CLEO_E_ARG = -1
 CLEO_E_HIP = -2
 CLEO_E_DEVICE = -3
 CLEO_E_STATE = -4
+CLEO_E_NOMEM = -5
+CLEO_E_FILE = -6
 
 ERR_NAMES = {
     0: "NONE", 1: "XI_ZERO", 2: "RADIUS_BELOW_DRY", 3: "NEG_CONDENSATE", 4: "NEWTON_MAXITERS",

@@ -38,7 +38,8 @@ enum {
   CLEO_E_HIP = -2,     /* HIP runtime call failed */
   CLEO_E_DEVICE = -3,  /* a kernel recorded an error in the device error word */
   CLEO_E_STATE = -4,   /* call out of order (e.g. collisions with an unconsumed/absent RNG) */
-  CLEO_E_NOMEM = -5
+  CLEO_E_NOMEM = -5,
+  CLEO_E_FILE = -6     /* a file could not be read or does not follow the expected layout */
 };
 
 /* device error codes: one per Kokkos::abort site of the reference on this path */
@@ -256,6 +257,27 @@ int cleo_download_niters(cleo_ctx* ctx, uint32_t* niters, uint64_t n);
    condensation calls. */
 int cleo_record_guess(cleo_ctx* ctx, int32_t enable);
 int cleo_download_guess(cleo_ctx* ctx, double* z, uint64_t n);
+
+/* ---- CLEO binary initialisation files (host only: no context, no GPU) ---------------------- */
+/* The layout of initialise/readbinary.cpp:22-99 (written by cleopy/writebinary.py): 4 uint32
+   (first data byte, bytes of the global metadata string, number of variables, bytes of metadata
+   per variable), the string, then per variable 3 uint32 (first byte, bytes per datapoint,
+   datapoints), 2 chars (type, units) and a double (scale factor), little-endian and unpadded.
+
+   Superdroplet initial conditions (init_all_supers_from_binary.cpp:57-100): variables 0..6 are
+   sdgbxindex (uint32), xi (uint64), radius, msol, coord3, coord1, coord2 (double, dimensionless);
+   a coordinate variable with no datapoints (fewer space dimensions) reads as 0.  sdIds are 0..n-1
+   in file order (Superdrop::IDType::Gen::next).  With out == NULL only *n is set; otherwise the
+   host arrays of `out` (except sdid, filled too) must hold `cap` >= n entries.  The drops go to
+   the GPU with cleo_upload_supers. */
+int cleo_read_initsupers(const char* path, cleo_supers_soa* out, uint64_t cap, uint64_t* n);
+/* Gridbox boundaries (gbx_bounds_from_binary.cpp:29-58): variable 0 ndims (3 uint64: nz, nx, ny),
+   1 gridbox indices (uint32), 2 six bounds per gridbox [zlow, zup, xlow, xup, ylow, yup]
+   (double, dimensionless).  Writes ndims and, when the arrays are non-null, the face coordinates
+   zhalf[nz+1], xhalf[nx+1], yhalf[ny+1] of the Cartesian grid that cleo_domain_desc takes
+   (gridbox index iz + nz (ix + nx iy)); CLEO_E_FILE when a gridbox's bounds are not those of that
+   grid. */
+int cleo_read_gridfile(const char* path, uint32_t ndims[3], double* zhalf, double* xhalf, double* yhalf);
 
 /* ---- observers ----------------------------------------------------------------------- */
 /* 0th, 1st and 2nd moments of the droplet mass distribution of every gridbox
