@@ -240,11 +240,37 @@ def run_gpu(args):
     ms = C.c_float()
     d.check(L.cleo_time_kernel(d.h, 4, 3, C.byref(ms)), "time_kernel mass moments")
     observer_ms = float(ms.value)
+    # one coupling step of prescribed dynamics from device arrays (the State is set from them; the
+    # drops are not touched and nothing runs after this)
+    coupling_ms = time_receive_dynamics(d, L, ndims, torch, rec, elapsed)
     d.check(L.cleo_synchronize(d.h), "final sync")
     err = d.error()
     d.close()
     return dict(n=n, ngbx=ngbx, ndims=ndims, plain=plain, stage_ms=stage_ms, kernel_ms=kernel_ms, nd=nd, tiles=tiles,
-                gen_s=t_gen, learned=learned, err=err, observer_ms=observer_ms)
+                gen_s=t_gen, learned=learned, err=err, observer_ms=observer_ms, coupling_ms=coupling_ms)
+
+
+def time_receive_dynamics(d, L, ndims, torch, rec, elapsed, reps=5):
+    """Mean time of cleo_receive_dynamics over `reps` calls, HIP events on the library's stream:
+    thermodynamics per gridbox and winds on the faces of the C grid, synthetic, in HBM."""
+    from cleo_amd import abi
+    nz, nx, ny = ndims
+    ngbx = nz * nx * ny
+    dev = dict(press=torch.full((ngbx,), 1.0), temp=torch.full((ngbx,), 1.05), qvap=torch.full((ngbx,), 0.01),
+               qcond=torch.zeros(ngbx), wvel_zfaces=torch.zeros((nz + 1) * nx * ny),
+               uvel_xfaces=torch.zeros(nz * (nx + 1) * ny), vvel_yfaces=torch.zeros(nz * nx * (ny + 1)))
+    dev = {k: v.to(dtype=torch.float64, device="cuda:0") for k, v in dev.items()}
+    torch.cuda.synchronize()
+    step = abi.DynamicsStep()
+    for k, v in dev.items():
+        setattr(step, k, v.data_ptr())
+    L.cleo_receive_dynamics.argtypes = [C.c_void_p, C.POINTER(abi.DynamicsStep)]
+    d.check(L.cleo_receive_dynamics(d.h, C.byref(step)), "receive_dynamics")  # warm-up
+    rec(d.h, 6)
+    for _ in range(reps):
+        d.check(L.cleo_receive_dynamics(d.h, C.byref(step)), "receive_dynamics")
+    rec(d.h, 7)
+    return elapsed(d, 6, 7) / reps
 
 
 def learned_leg(args, b, d, sdm, t, timed_steps, torch):
@@ -365,6 +391,10 @@ def main():
         # MassMomentsObserver's reduction: read xi, r, msol | write mom0, mom1, mom2 per gridbox
         "observer": {"kernel": "k_mass_moments", "ms": g["observer_ms"],
                      "GB/s": (24 * nd + 16 * g["ngbx"]) / (g["observer_ms"] / 1e3) / 1e9},
+        # FromFileComms::receive_dynamics from device arrays: read 4 thermodynamic values and 6
+        # face winds, write 10 State values per gridbox
+        "coupling": {"kernel": "k_receive_dynamics", "ms": g["coupling_ms"],
+                     "GB/s": 20 * 8 * g["ngbx"] / (g["coupling_ms"] / 1e3) / 1e9},
         "cpu_baseline": cpu,
     }
     if g["learned"]:

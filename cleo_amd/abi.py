@@ -141,6 +141,11 @@ def empty_state(ngbx):
     return {name: np.zeros(ngbx, dtype=np.float64) for name in STATE_FIELDS}
 
 
+class DynamicsStep(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("press", "temp", "qvap", "qcond", "wvel_zfaces", "uvel_xfaces",
+                                           "vvel_yfaces")]
+
+
 _SIGS = {
     "create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint64, C.POINTER(DomainDesc)]),
     "destroy": (C.c_int, [C.c_void_p]),
@@ -168,6 +173,7 @@ _SIGS = {
     "set_tiles": (C.c_int, [C.c_void_p, C.c_void_p]),
     "get_tiles": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mass_moments": (C.c_int, [C.c_void_p, C.c_int32, _u64p, _fp, _fp]),
+    "receive_dynamics": (C.c_int, [C.c_void_p, C.POINTER(DynamicsStep)]),
 }
 
 

@@ -1,6 +1,7 @@
-// binfiles.hip -- CLEO's binary initialisation files: the metadata layout of
-// initialise/readbinary.cpp:22-99, superdroplet initial conditions
-// (init_all_supers_from_binary.cpp:57-100) and gridbox boundaries (gbx_bounds_from_binary.cpp:29-58).
+// binfiles.hip -- CLEO's binary input files: the metadata layout of initialise/readbinary.cpp:22-99,
+// superdroplet initial conditions (init_all_supers_from_binary.cpp:57-100), gridbox boundaries
+// (gbx_bounds_from_binary.cpp:29-58) and from-file thermodynamics (fromfile_cartesian_dynamics.cpp:
+// 22-31).
 // Host code only: these calls need neither a context nor a GPU.
 #include <cstdint>
 #include <cstring>
@@ -133,4 +134,17 @@ int cleo_read_gridfile(const char* path, uint32_t ndims[3], double* zhalf, doubl
       return CLEO_E_FILE;
   }
   return CLEO_OK;
+}
+
+int cleo_read_thermofile(const char* path, double* out, uint64_t cap, uint64_t* n) {
+  if (!path || !n) return CLEO_E_ARG;
+  std::ifstream f;
+  uint64_t fsize = 0;
+  if (!open_file(path, f, fsize)) return CLEO_E_FILE;
+  std::vector<VarMeta> meta;
+  if (!read_metadata(f, fsize, meta) || meta.empty()) return CLEO_E_FILE;
+  *n = meta[0].nvar;
+  if (!out) return meta[0].bsize == sizeof(double) ? CLEO_OK : CLEO_E_FILE;
+  if (*n > cap) return CLEO_E_ARG;
+  return read_var(f, meta[0], out) ? CLEO_OK : CLEO_E_FILE;
 }

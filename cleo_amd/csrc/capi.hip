@@ -569,6 +569,13 @@ int cleo_download_refs(cleo_ctx* c, uint32_t* refs) {
   return rc;
 }
 
+// FromFileComms::receive_dynamics from device arrays: no host round trip of the State
+int cleo_receive_dynamics(cleo_ctx* c, const cleo_dynamics_step* dyn) {
+  if (!c || !dyn || !dyn->press || !dyn->temp || !dyn->qvap || !dyn->qcond) return CLEO_E_ARG;
+  launch_receive_dynamics(c, *dyn);
+  return launch_check(c);
+}
+
 // MassMomentsObserver's reductions (observers/massmoments_observer.cpp:23-73) on the device,
 // copied to the host arrays
 int cleo_mass_moments(cleo_ctx* c, int32_t rain, uint64_t* mom0, float* mom1, float* mom2) {

@@ -2309,6 +2309,46 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_state(CondFlatAr
   }
 }
 
+// ============================ coupling ========================================================
+// FromFileComms::update_gridbox_state for every gridbox (fromfilecomms.cpp:51-62): thermodynamics
+// at the gridbox, winds at its lower and upper faces (fromfile_cartesian_dynamics.cpp:150-215)
+__global__ void k_receive_dynamics(GState st, Grid gr, cleo_dynamics_step dyn) {
+  const uint32_t nz = gr.nz, nx = gr.nx, ngbx = gr.ngbx;
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ngbx; g += gridDim.x * blockDim.x) {
+    const uint32_t k = g % nz, i = (g / nz) % nx, j = g / (nz * nx);
+    st.press[g] = dyn.press[g];
+    st.temp[g] = dyn.temp[g];
+    st.qv[g] = dyn.qvap[g];
+    st.qc[g] = dyn.qcond[g];
+    double wl = 0.0, wh = 0.0, ul = 0.0, uh = 0.0, vl = 0.0, vh = 0.0;
+    if (dyn.wvel_zfaces) {
+      const uint64_t l = static_cast<uint64_t>(nx) * (nz + 1) * j + static_cast<uint64_t>(nz + 1) * i + k;
+      wl = dyn.wvel_zfaces[l];
+      wh = dyn.wvel_zfaces[l + 1];
+    }
+    if (dyn.uvel_xfaces) {
+      const uint64_t l = static_cast<uint64_t>(nx + 1) * nz * j + static_cast<uint64_t>(nz) * i + k;
+      ul = dyn.uvel_xfaces[l];
+      uh = dyn.uvel_xfaces[l + nz];
+    }
+    if (dyn.vvel_yfaces) {
+      vl = dyn.vvel_yfaces[g];
+      vh = dyn.vvel_yfaces[static_cast<uint64_t>(g) + static_cast<uint64_t>(nx) * nz];
+    }
+    st.wlo[g] = wl;
+    st.whi[g] = wh;
+    st.ulo[g] = ul;
+    st.uhi[g] = uh;
+    st.vlo[g] = vl;
+    st.vhi[g] = vh;
+  }
+}
+void launch_receive_dynamics(cleo_ctx* c, const cleo_dynamics_step& dyn) {
+  if (c->grid.ngbx == 0) return;
+  hipLaunchKernelGGL(k_receive_dynamics, dim3(stride_blocks(c->grid.ngbx)), dim3(256), 0, c->stream, c->st, c->grid,
+                     dyn);
+}
+
 // ============================ observers =======================================================
 // Mass moments per gridbox (observers/massmoments_observer.cpp:23-73): one lane per gridbox of a
 // 64-gridbox tile.  The tile's drops pass through an LDS window with coalesced loads; the loading

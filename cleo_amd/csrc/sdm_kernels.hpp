@@ -177,6 +177,8 @@ void materialize_order(cleo_ctx* c);
 // 0th, 1st and 2nd mass moments of every gridbox into device arrays of ngbx entries (raindrops
 // only when rain)
 void launch_mass_moments(cleo_ctx* c, bool rain, uint64_t* mom0, float* mom1, float* mom2);
+// one coupling step of prescribed dynamics (device arrays) into every gridbox's State
+void launch_receive_dynamics(cleo_ctx* c, const cleo_dynamics_step& dyn);
 // the identity as the pending order (the store unchanged)
 void pending_identity_order(cleo_ctx* c);
 

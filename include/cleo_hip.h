@@ -279,6 +279,30 @@ int cleo_read_initsupers(const char* path, cleo_supers_soa* out, uint64_t cap, u
    grid. */
 int cleo_read_gridfile(const char* path, uint32_t ndims[3], double* zhalf, double* xhalf, double* yhalf);
 
+/* ---- coupling ------------------------------------------------------------------------- */
+/* One coupling step of prescribed dynamics into every gridbox's State
+   (FromFileComms::receive_dynamics / update_gridbox_state, coupldyn_fromfile/fromfilecomms.cpp:
+   36-62; face indexing of fromfile_cartesian_dynamics.cpp:150-215).  Arrays of this step, in
+   device memory for cleo_receive_dynamics: press, temp, qvap, qcond per gridbox (ngbx); the wind
+   components on the faces of the C grid, wvel on z faces ((nz+1) nx ny values, index
+   k + (nz+1)(i + nx j)), uvel on x faces (nz (nx+1) ny, k + nz (i + (nx+1) j)) and vvel on y
+   faces (nz nx (ny+1), k + nz (i + nx j)), for gridbox k + nz (i + nx j).  A null wind array
+   gives that component 0 on both faces (a model with fewer space dimensions).  Volumes stay. */
+typedef struct {
+  const double* press;
+  const double* temp;
+  const double* qvap;
+  const double* qcond;
+  const double* wvel_zfaces;
+  const double* uvel_xfaces;
+  const double* vvel_yfaces;
+} cleo_dynamics_step;
+int cleo_receive_dynamics(cleo_ctx* ctx, const cleo_dynamics_step* dev);
+/* A thermodynamics file of CLEO's from-file dynamics (thermodynamicvar_from_binary,
+   fromfile_cartesian_dynamics.cpp:22-31): the doubles of the file's first variable, all
+   timesteps one after another.  With out == NULL only *n is set.  Host only. */
+int cleo_read_thermofile(const char* path, double* out, uint64_t cap, uint64_t* n);
+
 /* ---- observers ----------------------------------------------------------------------- */
 /* 0th, 1st and 2nd moments of the droplet mass distribution of every gridbox
    (MassMomentsObserver: observers/massmoments_observer.cpp:23-73): mom0[g] = sum of xi,

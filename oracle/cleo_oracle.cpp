@@ -1116,6 +1116,32 @@ int orc_download_refs(orc_ctx* h, uint32_t* refs) {
   refs[c.ngbx] = static_cast<uint32_t>(c.ndomain);
   return CLEO_OK;
 }
+/* FromFileComms::update_gridbox_state for every gridbox (coupldyn_fromfile/fromfilecomms.cpp:51-62)
+   with the face positions of fromfile_cartesian_dynamics.cpp:150-215 (wvel: lower z face
+   nx (nz+1) j + (nz+1) i + k, upper + 1; uvel: lower x face (nx+1) nz j + nz i + k, upper + nz;
+   vvel: lower y face at the gridbox index, upper + nx nz); a null wind array is nullwinds (0, 0) */
+int orc_receive_dynamics(orc_ctx* h, const cleo_dynamics_step* d) {
+  Ctx& c = *reinterpret_cast<Ctx*>(h);
+  if (!d || !d->press || !d->temp || !d->qvap || !d->qcond) return CLEO_E_ARG;
+  const uint64_t nz = c.nz, nx = c.nx;
+  for (uint32_t g = 0; g < c.ngbx; ++g) {
+    const uint64_t k = c.iz(g), i = c.ix(g), j = c.iy(g);
+    State& t = c.st[g];
+    t.press = d->press[g];
+    t.temp = d->temp[g];
+    t.qvap = d->qvap[g];
+    t.qcond = d->qcond[g];
+    const uint64_t wl = nx * (nz + 1) * j + (nz + 1) * i + k;
+    const uint64_t ul = (nx + 1) * nz * j + nz * i + k;
+    t.wlo = d->wvel_zfaces ? d->wvel_zfaces[wl] : 0.0;
+    t.whi = d->wvel_zfaces ? d->wvel_zfaces[wl + 1] : 0.0;
+    t.ulo = d->uvel_xfaces ? d->uvel_xfaces[ul] : 0.0;
+    t.uhi = d->uvel_xfaces ? d->uvel_xfaces[ul + nz] : 0.0;
+    t.vlo = d->vvel_yfaces ? d->vvel_yfaces[g] : 0.0;
+    t.vhi = d->vvel_yfaces ? d->vvel_yfaces[g + nx * nz] : 0.0;
+  }
+  return CLEO_OK;
+}
 /* calculate_massmoments / calculate_rainmassmoments (observers/massmoments_observer.cpp:23-73) in
    Serial order: per gridbox, over its drops in order, mom0 += xi, mom1 += (float)(xi * mass),
    mom2 += (float)(xi * mass * mass); rain: only drops with radius >= 40 micrometres */
