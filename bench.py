@@ -338,8 +338,8 @@ def run_cpu(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is design-only "
-              "in this build.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the library has no multi-GPU "
+              "transport (the spatial sharding it would use is described with the design).", file=sys.stderr)
         return 2
     if int(os.environ.get("RANK", "0")) != 0:
         return 0
