@@ -1897,6 +1897,7 @@ __device__ __forceinline__ bool defer_drop(const CondFlatArgs& a, uint32_t* seg,
 // room for every segment, so nothing is lost.
 __device__ __forceinline__ void defer_flush(const CondFlatArgs& a, const uint32_t* seg, uint32_t ndef, int lane) {
   if (ndef == 0) return;
+  wave_sync();  // the segment's entries were written by other lanes of this wave
   uint32_t base = 0;
   if (lane == 0) base = atomicAdd(a.dcount, ndef);
   base = readlane_u32(base, 0);
