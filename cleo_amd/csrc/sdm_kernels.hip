@@ -1947,7 +1947,8 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_deferred(CondFla
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     const uint32_t k = a.dlist[i];
-    const double warm = a.warm ? a.warm[k] : __longlong_as_double(0x7FF8000000000000ll);
+    const double warm =
+        a.warm ? a.warm[a.warm_perm ? a.warm_perm[k] : k] : __longlong_as_double(0x7FF8000000000000ll);
     cond_drop(a, nobudget, a.s.gbx[k], k, warm, st);
   }
   put_cond_stats(a.stat, st);
@@ -2671,9 +2672,10 @@ void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatAr
   a.qm = c->qm;  // the qcond recompute joins k_cond_state
   const uint32_t gw = c->tiles.cond_grid_waves;
   const uint64_t ntiles = (ngbx + kWave - 1) / kWave;
-  // a deferred drop is solved again by its sorted position, which the guess buffer is not
-  // indexed by: with learned guesses every drop is solved in the gather
-  const uint32_t budget = (a.mlp_inline || a.warm) ? 0u : c->tiles.cond_defer;
+  // a deferred drop is solved again by its sorted position k; its learned guess is that of source
+  // position perm[k]
+  const uint32_t budget = c->tiles.cond_defer;
+  if (a.mlp_inline) a.warm_perm = n.perm;
   a.cfg.budget = budget;
   a.dcap = budget ? c->dlist_cap : 0u;
   a.dlist = c->dlist;

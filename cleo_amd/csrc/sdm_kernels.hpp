@@ -130,6 +130,9 @@ struct CondFlatArgs {
   int32_t mlp_inline;
   MLPWeights W;
   double* warm_buf;
+  // warm (learned guesses) indexed by the drop's source position perm[k] rather than by k: the
+  // guesses of the sort's gather with the solves, read again by k_cond_deferred
+  const uint32_t* warm_perm;
   double* guess_out;  // non-null: the guesses are recorded here too (array position order)
   // k_cond_guess on the store as motion left it (before the sort): a drop is active when its new
   // gridbox is in the domain (the refs still describe the unsorted store)
