@@ -375,6 +375,7 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   HIP_TRY(c, dalloc(&c->cnt, 28 * ngbx));
   HIP_TRY(c, dalloc(&c->tot, ngbx + 1));
   HIP_TRY(c, dalloc(&c->oobpre, ngbx + 1));
+  HIP_TRY(c, dalloc(&c->oobcnt, ngbx));
   HIP_TRY(c, dalloc(&c->inj_oj, ngbx + 1));
   HIP_TRY(c, dalloc(&c->inj_op, ngbx + 1));
   c->scan_tmp_n = scan_tmp_words(28 * ngbx + 1);
@@ -417,7 +418,7 @@ int cleo_destroy(cleo_ctx* c) {
   free_soa(c->buf[1]);
   for (void* p : {static_cast<void*>(c->d_halves), static_cast<void*>(c->off), static_cast<void*>(c->off_alt),
                   static_cast<void*>(c->nact), static_cast<void*>(c->d_state), static_cast<void*>(c->cnt),
-                  static_cast<void*>(c->tot), static_cast<void*>(c->oobpre), static_cast<void*>(c->inj_oj),
+                  static_cast<void*>(c->tot), static_cast<void*>(c->oobpre), static_cast<void*>(c->oobcnt), static_cast<void*>(c->inj_oj),
                   static_cast<void*>(c->inj_op), static_cast<void*>(c->scan_tmp), static_cast<void*>(c->big_scratch),
                   static_cast<void*>(c->ctl), static_cast<void*>(c->inj_j), static_cast<void*>(c->inj_phi),
                   static_cast<void*>(c->niters), static_cast<void*>(c->guess), static_cast<void*>(c->rs_key[0]),

@@ -74,6 +74,7 @@ struct cleo_ctx {
   uint32_t* cnt = nullptr;      // ngbx*28
   uint32_t* tot = nullptr;      // ngbx+1
   uint32_t* oobpre = nullptr;   // ngbx+1
+  uint32_t* oobcnt = nullptr;   // ngbx: out-of-domain count per gridbox (compact copy for its scan)
   uint32_t* scan_tmp = nullptr;
   size_t scan_tmp_n = 0;
   // generic radix sort scratch (lazily allocated)

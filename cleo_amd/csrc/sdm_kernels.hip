@@ -805,6 +805,7 @@ __global__ void k_nsort_tot(NsortArgs a) {
   const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= a.grid.ngbx) return;
   uint32_t* row = a.cnt + 28ull * d;
+  a.oobcnt[d] = row[27];  // the out-of-domain count, compact for its scan
   uint32_t cnt[27];
   uint32_t t = 0;
 #pragma unroll
@@ -1565,6 +1566,7 @@ static NsortArgs nsort_args(cleo_ctx* c, const SoA& src, const SoA& dst, const u
   a.cnt = c->cnt;
   a.oobpre = c->oobpre;
   a.tot = c->tot;
+  a.oobcnt = c->oobcnt;
   a.grid = c->grid;
   a.wpb = c->tiles.motion_waves_per_block ? c->tiles.motion_waves_per_block : 1;
   a.ctl = c->ctl;
@@ -1586,7 +1588,7 @@ static void nsort_tables(cleo_ctx* c, const NsortArgs& a, const uint32_t* flag) 
   const uint32_t ngbx = c->grid.ngbx;
   hipLaunchKernelGGL(k_nsort_tot, dim3(nblocks(ngbx, 256)), dim3(256), 0, c->stream, a);
   scan_u32(c, c->tot, ngbx, 1u, 0u, c->off_alt, flag);
-  scan_u32(c, c->cnt, ngbx, 28u, 27u, c->oobpre, flag);
+  scan_u32(c, c->oobcnt, ngbx, 1u, 0u, c->oobpre, flag);
 }
 // the sorted store in buffer `dst` becomes current: refs and active counts follow
 static void nsort_commit(cleo_ctx* c, int dst) {

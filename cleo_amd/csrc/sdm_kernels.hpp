@@ -89,6 +89,7 @@ struct NsortArgs {
   uint32_t* cnt;           // ngbx*28: arrivals per (destination, step), then start offsets
   const uint32_t* oobpre;  // ngbx+1
   uint32_t* tot;
+  uint32_t* oobcnt;        // ngbx: drops of each gridbox that leave the domain (row d's last count)
   Grid grid;
   uint32_t wpb;
   DevCtl* ctl;
