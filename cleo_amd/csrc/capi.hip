@@ -1139,7 +1139,7 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
     float best = 1e30f;
     const cleo_tiles base_t = best_t;
     for (uint32_t mv : {1u, 3u}) {
-      for (uint32_t gw : {16384u, 32768u, 65536u}) {
+      for (uint32_t gw : {32768u, 65536u, 131072u}) {
         for (uint32_t budget : {6u, 8u, 12u}) {
           cleo_tiles t = base_t;
           t.motion_flat = mv;
