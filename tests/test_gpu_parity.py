@@ -166,6 +166,28 @@ def test_motion(gpu, oracle, variant):
         assert_state_match(g.state(), o.state(), what=f"motion qcond {step}")
 
 
+@pytest.mark.parametrize("motion_flat", [1, 3])
+def test_coupled_large_gridboxes(gpu, oracle, motion_flat):
+    # gridboxes of 150-300 drops: the sort ranks chunks beyond the two it preloads, and the gather
+    # with the condensation solves (motion variant 3: also the motion) spans several chunks each
+    _say(f"case: coupled steps with 150-300-drop gridboxes, motion variant {motion_flat}, fused gather")
+    pop = P.coupled(SEED, ndims=(4, 3, 3), mean=220)
+    g, o = _both(gpu, oracle, pop)
+    t = abi.Tiles(1, 1, 1, 1, motion_flat, 8192, 0, 8192, 1, 6)
+    assert gpu.lib.cleo_set_tiles(g.h, abi.ctypes_byref(t)) == 0
+    s = sdm_params(cond=cond_params(), coll=coll_params(), motion=motion_params())
+    for d in (g, o):
+        d.set_sdm(s)
+        d.rng_counter(99)
+    for step in range(3):
+        assert g.run_step(step * 100, (step + 1) * 100) == 0
+        assert o.run_step(step * 100, (step + 1) * 100) == 0
+    assert g.error()[0] == 0 and o.error()[0] == 0
+    assert_supers_match(g.download(), o.download(), what="large gridboxes")
+    assert np.array_equal(g.refs(), o.refs())
+    assert_state_match(g.state(), o.state(), what="large gridboxes")
+
+
 def test_coupled_run_step(gpu, oracle):
     _say("case: full coupled run_step x4 (motion, sort, condensation, collisions)")
     pop = P.coupled(SEED, ndims=(8, 6, 4), mean=64)
