@@ -35,15 +35,32 @@ __device__ __forceinline__ double tile_sum(double* win, uint32_t t_lo, uint32_t 
 template <class V1, class V2>
 __device__ __forceinline__ void tile_sum2(double* win, uint32_t t_lo, uint32_t t_hi, uint32_t lo, uint32_t len, V1 v1,
                                           V2 v2, int lane, double& s1, double& s2) {
-  constexpr uint32_t kHalf = kSumWin / 2;
+  constexpr uint32_t kHalf = kSumWin / 2, kPer = kHalf / kWave;
   double a1 = 0.0, a2 = 0.0;
+  if (t_lo >= t_hi) {
+    s1 = s2 = 0.0;
+    return;
+  }
+  // the next window's values are loaded into registers while the lanes add the current one's
+  // (every lane loads, with indices clamped into the tile, so the loads stay in flight)
+  double r1[kPer], r2[kPer];
+  auto load = [&](uint32_t w) {
+    for (uint32_t i = 0; i < kPer; ++i) {
+      const uint32_t k = min(w + static_cast<uint32_t>(lane) + i * kWave, t_hi - 1u);
+      r1[i] = v1(k);
+      r2[i] = v2(k);
+    }
+  };
+  load(t_lo);
   for (uint32_t w = t_lo; w < t_hi; w += kHalf) {
     const uint32_t we = min(w + kHalf, t_hi);
-    for (uint32_t j = lane; j < we - w; j += kWave) {
-      win[j] = v1(w + j);
-      win[kHalf + j] = v2(w + j);
+    for (uint32_t i = 0; i < kPer; ++i) {
+      const uint32_t j = static_cast<uint32_t>(lane) + i * kWave;
+      win[j] = r1[i];
+      win[kHalf + j] = r2[i];
     }
     wave_sync();
+    if (we < t_hi) load(we);
     const uint32_t b0 = max(lo, w), b1 = min(lo + len, we);
     for (uint32_t k = b0; k < b1; ++k) {
       a1 += win[k - w];
