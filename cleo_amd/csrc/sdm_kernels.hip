@@ -14,16 +14,27 @@ namespace cleo {
 // ============================ per-gridbox sums ================================================
 // Drop-order sums of v(k) over [lo, lo + len) for the gridboxes of a tile of 64 (lane i passes its
 // own gridbox's range and gets its sum).  The tile's drops [t_lo, t_hi) are staged through an LDS
-// window of kSumWin values with coalesced loads; each lane then adds its gridbox's values left to
-// right, exactly as a serial loop does (the reference's sums run in drop order under Serial).
+// window of kSumWin values with coalesced loads (the next window's loads in flight while the current
+// one is added); each lane then adds its gridbox's values left to right, exactly as a serial loop
+// does (the reference's sums run in drop order under Serial).
 template <class V>
 __device__ __forceinline__ double tile_sum(double* win, uint32_t t_lo, uint32_t t_hi, uint32_t lo, uint32_t len, V v,
                                            int lane) {
+  constexpr uint32_t kPer = kSumWin / kWave;
   double acc = 0.0;
+  if (t_lo >= t_hi) return acc;
+  // the next window's values are loaded into registers while the lanes add the current one's
+  // (every lane loads, with indices clamped into the tile, so the loads stay in flight)
+  double r[kPer];
+  auto load = [&](uint32_t w) {
+    for (uint32_t i = 0; i < kPer; ++i) r[i] = v(min(w + static_cast<uint32_t>(lane) + i * kWave, t_hi - 1u));
+  };
+  load(t_lo);
   for (uint32_t w = t_lo; w < t_hi; w += kSumWin) {
     const uint32_t we = min(w + kSumWin, t_hi);
-    for (uint32_t j = lane; j < we - w; j += kWave) win[j] = v(w + j);
+    for (uint32_t i = 0; i < kPer; ++i) win[static_cast<uint32_t>(lane) + i * kWave] = r[i];
     wave_sync();
+    if (we < t_hi) load(we);
     const uint32_t a0 = max(lo, w), a1 = min(lo + len, we);
     for (uint32_t k = a0; k < a1; ++k) acc += win[k - w];
     wave_sync();
@@ -2378,8 +2389,9 @@ void launch_receive_dynamics(cleo_ctx* c, const cleo_dynamics_step& dyn) {
 
 // ============================ observers =======================================================
 // Mass moments per gridbox (observers/massmoments_observer.cpp:23-73): one lane per gridbox of a
-// 64-gridbox tile.  The tile's drops pass through an LDS window with coalesced loads; the loading
-// lane turns each drop into its three terms (xi; (float)(xi m); (float)(xi m m), m the drop mass,
+// 64-gridbox tile.  The tile's drops pass through an LDS window with coalesced loads (the next
+// window's loads in flight while the current one is added); the loading lane turns each drop into
+// its three terms (xi; (float)(xi m); (float)(xi m m), m the drop mass,
 // zero for a drop below the raindrop radius when kRain), and each lane then adds its gridbox's
 // terms left to right, so the float sums are the reference's serial ones.
 constexpr uint32_t kMomWin = 512;
@@ -2402,21 +2414,44 @@ __global__ void __launch_bounds__(256) k_mass_moments(SoA s, const uint32_t* off
     const uint32_t lo = valid ? off[g] : t_hi, hi = valid ? off[g + 1] : t_hi;
     uint64_t m0 = 0;
     float m1 = 0.f, m2 = 0.f;
+    if (t_lo >= t_hi) {
+      if (valid) {
+        mom0[g] = 0;
+        mom1[g] = 0.f;
+        mom2[g] = 0.f;
+      }
+      continue;
+    }
+    // the next window's drops are loaded into registers while the lanes add the current window
+    // (every lane loads, with indices clamped into the tile, so the loads stay in flight)
+    constexpr uint32_t kPer = kMomWin / kWave;
+    uint64_t rx[kPer];
+    double rr[kPer], rm[kPer];
+    auto load = [&](uint32_t w) {
+      for (uint32_t i = 0; i < kPer; ++i) {
+        const uint32_t k = min(w + static_cast<uint32_t>(lane) + i * kWave, t_hi - 1u);
+        rx[i] = s.xi[k];
+        rr[i] = s.r[k];
+        rm[i] = s.msol[k];
+      }
+    };
+    load(t_lo);
     for (uint32_t w = t_lo; w < t_hi; w += kMomWin) {
       const uint32_t we = min(w + kMomWin, t_hi);
-      for (uint32_t j = static_cast<uint32_t>(lane); j < we - w; j += kWave) {
-        const uint32_t k = w + j;
-        const uint64_t xi = s.xi[k];
-        const double r = s.r[k];
-        if (xi == ~0ull) set_error(ctl, CLEO_ERR_XI_OVERFLOW, s.gbx[k], k);
+      for (uint32_t i = 0; i < kPer; ++i) {
+        const uint32_t j = static_cast<uint32_t>(lane) + i * kWave;
+        const uint64_t xi = rx[i];
+        const double r = rr[i];
+        if (j < we - w && xi == ~0ull) set_error(ctl, CLEO_ERR_XI_OVERFLOW, s.gbx[w + j], w + j);
         const bool in = !kRain || r >= rlim;
-        const double m = drop_mass(r, s.msol[k], sol);
+        const double m = drop_mass(r, rm[i], sol);
         const double xm = static_cast<double>(xi) * m;
         w0[j] = in ? xi : 0ull;
         w1[j] = in ? static_cast<float>(xm) : 0.f;
         w2[j] = in ? static_cast<float>(xm * m) : 0.f;
       }
       wave_sync();
+      if (we < t_hi) load(we);
       const uint32_t b0 = max(lo, w), b1 = min(hi, we);
       for (uint32_t k = b0; k < b1; ++k) {
         m0 += w0[k - w];
