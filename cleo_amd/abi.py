@@ -26,7 +26,9 @@ ERR_NAMES = {
 }
 ERR_XI_OVERFLOW = 13
 
-PROB_GOLOVIN, PROB_LONG, PROB_CONST = 0, 1, 2
+PROB_GOLOVIN, PROB_LONG, PROB_CONST, PROB_LOWLIST_COAL, PROB_LOWLIST_BU = 0, 1, 2, 3, 4
+NFRAGS_CONST, NFRAGS_CKE = 0, 1
+FLAG_CONST, FLAG_SU, FLAG_TS, FLAG_STRAUB = 0, 1, 2, 3
 ENACT_COAL, ENACT_BREAKUP, ENACT_COALBURE = 0, 1, 2
 TV_NULL, TV_SIMMEL, TV_ROGERS_YAU, TV_ROGERS_GK = 0, 1, 2, 3
 
@@ -62,7 +64,9 @@ class Solute(C.Structure):
 
 class CollParams(C.Structure):
     _fields_ = [("interval", C.c_uint32), ("prob_kind", C.c_int32), ("prob_param", C.c_double),
-                ("enact_kind", C.c_int32), ("nfrags", C.c_double), ("coalbure_coaleff", C.c_double)]
+                ("enact_kind", C.c_int32), ("nfrags", C.c_double), ("coalbure_coaleff", C.c_double),
+                ("nfrags_kind", C.c_int32), ("flag_kind", C.c_int32), ("prob_tv", C.c_int32),
+                ("nfrags_tv", C.c_int32), ("flag_tv", C.c_int32)]
 
 
 class CondParams(C.Structure):

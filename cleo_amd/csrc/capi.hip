@@ -172,8 +172,36 @@ int prepare_injected(cleo_ctx* c) {
 
 // one collision event over every gridbox: the drops stay in buf[cur], their shuffled order goes to
 // c->order (a pending order of an earlier event is applied first)
+// the kinds a cleo_coll_params names must be ones the kernels know; the Low-List probabilities
+// and the kinetic-energy NFrags / flags read a terminal velocity formula
+bool tv_known(int32_t tv) { return tv == CLEO_TV_NULL || tv == CLEO_TV_SIMMEL || tv == CLEO_TV_ROGERS_YAU || tv == CLEO_TV_ROGERS_GK; }
+const char* coll_params_error(const cleo_coll_params& p) {
+  if (p.interval == 0) return "collision interval 0";
+  if (p.prob_kind < CLEO_PROB_GOLOVIN || p.prob_kind > CLEO_PROB_LOWLIST_BU) return "unknown prob_kind";
+  if (p.enact_kind < CLEO_ENACT_COAL || p.enact_kind > CLEO_ENACT_COALBURE) return "unknown enact_kind";
+  if (p.nfrags_kind != CLEO_NFRAGS_CONST && p.nfrags_kind != CLEO_NFRAGS_CKE) return "unknown nfrags_kind";
+  if (p.flag_kind < CLEO_FLAG_CONST || p.flag_kind > CLEO_FLAG_STRAUB) return "unknown flag_kind";
+  if (!tv_known(p.prob_tv) || !tv_known(p.nfrags_tv) || !tv_known(p.flag_tv)) return "unknown terminal velocity";
+  return nullptr;
+}
+CollCfg make_collcfg(const cleo_coll_params& p) {
+  CollCfg k{};
+  k.prob_kind = p.prob_kind;
+  k.prob_param = p.prob_param;
+  k.enact_kind = p.enact_kind;
+  k.nfrags = p.nfrags;
+  k.coaleff = p.coalbure_coaleff;
+  k.DELT = step2realtime(p.interval);
+  k.nfrags_kind = p.nfrags_kind;
+  k.flag_kind = p.flag_kind;
+  k.prob_tv = p.prob_tv;
+  k.nfrags_tv = p.nfrags_tv;
+  k.flag_tv = p.flag_tv;
+  return k;
+}
+
 int enqueue_collision_event(cleo_ctx* c, const cleo_coll_params& p) {
-  if (p.interval == 0) return fail(c, CLEO_E_ARG, "collision interval 0");
+  if (const char* m = coll_params_error(p)) return fail(c, CLEO_E_ARG, m);
   if (c->rng_mode == 0) return fail(c, CLEO_E_STATE, "no random-number stream set");
   const bool inj = c->rng_mode == 2;
   if (inj) {
@@ -188,7 +216,7 @@ int enqueue_collision_event(cleo_ctx* c, const cleo_coll_params& p) {
   a.nact = c->nact;
   a.vol = c->st.vol;
   a.sol = c->solute;
-  a.k = CollCfg{p.prob_kind, p.prob_param, p.enact_kind, p.nfrags, p.coalbure_coaleff, step2realtime(p.interval)};
+  a.k = make_collcfg(p);
   a.ngbx = c->grid.ngbx;
   a.wpb = wpb_or1(c->tiles.coll_waves_per_block);
   a.ctl = c->ctl;
@@ -674,6 +702,8 @@ int cleo_set_sdm(cleo_ctx* c, const cleo_sdm_params* p) {
   if ((p->enable_cond && p->cond.interval == 0) || (p->enable_colls && p->coll.interval == 0) ||
       (p->enable_motion && p->motion.interval == 0))
     return fail(c, CLEO_E_ARG, "zero interval");
+  if (p->enable_colls)
+    if (const char* m = coll_params_error(p->coll)) return fail(c, CLEO_E_ARG, m);
   c->sdm = *p;
   c->sdm_set = true;
   return CLEO_OK;
@@ -992,8 +1022,7 @@ static int time_stage(cleo_ctx* c, int kern, const cleo_tiles& t, int reps, bool
       a.nact = c->nact;
       a.vol = c->st.vol;
       a.sol = c->solute;
-      a.k = CollCfg{kp.prob_kind, kp.prob_param, kp.enact_kind, kp.nfrags, kp.coalbure_coaleff,
-                    step2realtime(kp.interval)};
+      a.k = make_collcfg(kp);
       a.ngbx = ngbx;
       a.wpb = t.coll_waves_per_block ? t.coll_waves_per_block : 1;
       a.ctl = c->ctl;

@@ -24,6 +24,7 @@ constexpr double CP_DRY = 1004.64;
 constexpr double CP_V = 1865.01;
 constexpr double C_L = 4192.664;
 constexpr double RHO_L = 998.203;
+constexpr double SURFSIGMA = 7.28e-2;  // surface tension of water [J/m^2]
 }  // namespace dimmed
 namespace dimless {
 constexpr double W0 = 1.0;
@@ -493,9 +494,74 @@ struct CollCfg {
   double nfrags;
   double coaleff;
   double DELT;
+  int nfrags_kind, flag_kind;
+  int prob_tv, nfrags_tv, flag_tv;
 };
+
+// ---- collision kinetics (Low and List 1982a), energies in J from dimensionless radii --------
+// surface energy 4 pi sigma R^2 of one drop, of two, and of the coalesced drop of both volumes
+// (collisionkinetics.hpp:41-56, collisionkinetics.cpp:62-71)
+constexpr double kSurfConst = 4.0 * dimmed::SURFSIGMA * kPi * dimless::R0 * dimless::R0;
+__device__ __forceinline__ double surf_energy(double r) { return kSurfConst * (r * r); }
+__device__ __forceinline__ double surf_energy_pair(double r1, double r2) { return kSurfConst * ((r1 * r1) + (r2 * r2)); }
+__device__ __forceinline__ double surf_energy_coal(double r1, double r2) {
+  const double sum = r1 * r1 * r1 + r2 * r2 * r2;
+  return kSurfConst * pow(sum, 2.0 / 3.0);
+}
+// kinetic energy of the collision, (2/3) pi rho_l (r1^3 r2^3 / (r1^3 + r2^3)) (v1 - v2)^2, with
+// the reduced volume written as r1^3 / (1 + (r1/r2)^3) (collisionkinetics.cpp:33-46)
+__device__ __forceinline__ double collision_kinetic_energy(double r1, double r2, double v1, double v2) {
+  constexpr double kCke = dimless::R0 * dimless::R0 * dimless::R0 * 2.0 / 3.0 * dimmed::RHO_L * kPi * dimless::W0 *
+                          dimless::W0;
+  const double q = r1 / r2;
+  const double rratio = (r1 * r1 * r1) / (1 + q * q * q);
+  const double vdiff = v1 - v2;
+  return kCke * rratio * vdiff * vdiff;
+}
+// Straub et al. 2010 eq. 5: exp(-1.15 We), We = cke / coalesced surface energy
+// (coalescence_efficiency.cpp:27-37)
+__device__ __forceinline__ double coaleff_straub(double r1, double r2, double cke) {
+  return exp(-1.15 * (cke / surf_energy_coal(r1, r2)));
+}
+// Low and List 1982a eqs. 4.5-4.6: 0.778 (1 + rs/rl)^-2 exp(-2.62e6 sigma E^2 / S_c) when the
+// total energy E = cke + S_t - S_c is below 5e-6 J, else 0 (lowlistprob.hpp:50-91)
 __device__ __forceinline__ double vol_of(double r) { return 4.0 / 3.0 * kPi * rcubed(r); }
-// golovinprob.cpp:32-43, longhydroprob.cpp:30-48 + hydrodynamicprob.hpp:50-66, constprob.hpp:45-48
+__device__ __forceinline__ double coaleff_lowlist(int tv, double r1, double r2) {
+  const double cke = collision_kinetic_energy(r1, r2, terminal_velocity(tv, r1), terminal_velocity(tv, r2));
+  const double surf_t = surf_energy_pair(r1, r2);
+  const double surf_c = surf_energy_coal(r1, r2);
+  const double etot = cke + surf_t - surf_c;
+  if (!(etot < 5e-6)) return 0.0;
+  const double rsmall = fmin(r1, r2), rbig = fmax(r1, r2);
+  const double alpha = 1 + rsmall / rbig;
+  const double ratio = 1.0 / (alpha * alpha);
+  constexpr double kExp = -2.62e6 * dimmed::SURFSIGMA;
+  return 0.778 * ratio * exp(kExp * etot * etot / surf_c);
+}
+// Long's hydrodynamic kernel (longhydroprob.cpp:30-48, hydrodynamicprob.hpp:50-66) times coaleff
+__device__ __forceinline__ double prob_long(double r1, double r2, double coaleff, double DELT, double VOLUME) {
+  constexpr double k1 = 4.5e4 * dimless::R0 * dimless::R0 * 100 * 100;
+  constexpr double k2 = 3e-4 / dimless::R0 / 100;
+  constexpr double rlim = 5e-5 / dimless::R0;
+  constexpr double colleff_min = 0.001;
+  const double rsmall = fmin(r1, r2);
+  const double rbig = fmax(r1, r2);
+  double colleff = 1.0;
+  if (rbig < rlim) {
+    colleff = k1 * rbig * rbig * (1 - (k2 / rsmall));
+    colleff = fmax(colleff, colleff_min);
+  }
+  const double eff = colleff * coaleff;
+  constexpr double prob_jk_const = kPi * dimless::R0 * dimless::R0 * dimless::W0;
+  const double DELT_DELVOL = DELT / VOLUME;
+  const double sumr = r1 + r2;
+  const double sumrsqrd = sumr * sumr;
+  const double vdiff = fabs(tv_simmel(r1) - tv_simmel(r2));
+  const double hydro_kernel = prob_jk_const * eff * sumrsqrd * vdiff;
+  return hydro_kernel * DELT_DELVOL;
+}
+// golovinprob.cpp:32-43, longhydroprob.cpp:30-48 + hydrodynamicprob.hpp:50-66, constprob.hpp:45-48,
+// lowlistprob.hpp:117-123,144-153
 __device__ __forceinline__ double pair_probability(const CollCfg& k, double r1, double r2, double VOLUME) {
   if (k.prob_kind == CLEO_PROB_GOLOVIN) {
     constexpr double prob_jk_const = 1.5e3 * dimless::R0 * dimless::R0 * dimless::R0;
@@ -503,26 +569,12 @@ __device__ __forceinline__ double pair_probability(const CollCfg& k, double r1, 
     const double kern = prob_jk_const * (vol_of(r1) + vol_of(r2));
     return kern * DELT_DELVOL;
   }
-  if (k.prob_kind == CLEO_PROB_LONG) {
-    constexpr double k1 = 4.5e4 * dimless::R0 * dimless::R0 * 100 * 100;
-    constexpr double k2 = 3e-4 / dimless::R0 / 100;
-    constexpr double rlim = 5e-5 / dimless::R0;
-    constexpr double colleff_min = 0.001;
-    const double rsmall = fmin(r1, r2);
-    const double rbig = fmax(r1, r2);
-    double colleff = 1.0;
-    if (rbig < rlim) {
-      colleff = k1 * rbig * rbig * (1 - (k2 / rsmall));
-      colleff = fmax(colleff, colleff_min);
-    }
-    const double eff = colleff * k.prob_param;
-    constexpr double prob_jk_const = kPi * dimless::R0 * dimless::R0 * dimless::W0;
-    const double DELT_DELVOL = k.DELT / VOLUME;
-    const double sumr = r1 + r2;
-    const double sumrsqrd = sumr * sumr;
-    const double vdiff = fabs(tv_simmel(r1) - tv_simmel(r2));
-    const double hydro_kernel = prob_jk_const * eff * sumrsqrd * vdiff;
-    return hydro_kernel * DELT_DELVOL;
+  if (k.prob_kind == CLEO_PROB_LONG) return prob_long(r1, r2, k.prob_param, k.DELT, VOLUME);
+  if (k.prob_kind == CLEO_PROB_LOWLIST_COAL)
+    return prob_long(r1, r2, 1.0, k.DELT, VOLUME) * coaleff_lowlist(k.prob_tv, r1, r2);
+  if (k.prob_kind == CLEO_PROB_LOWLIST_BU) {
+    const double bueff = 1.0 - coaleff_lowlist(k.prob_tv, r1, r2);
+    return prob_long(r1, r2, 1.0, k.DELT, VOLUME) * bueff;
   }
   return k.prob_param * k.DELT / VOLUME;
 }
@@ -630,23 +682,54 @@ __device__ __forceinline__ int breakup(double nfrags, DropAttrs& d1, DropAttrs& 
   d2.msol = new_msol;
   return err;
 }
+// NFragments of the pair, from the radii before the breakup: ConstNFrags, or the collision
+// kinetic energy fit of Schlottke et al. 2010 fig. 13, 1 / (1.5 - (1e6 min(cke, ckemax))^0.135)
+// + 11/6 (breakup_nfrags.hpp:44-91)
+__device__ __forceinline__ double breakup_nfrags(const CollCfg& k, const DropAttrs& d1, const DropAttrs& d2) {
+  if (k.nfrags_kind != CLEO_NFRAGS_CKE) return k.nfrags;
+  const double cke = collision_kinetic_energy(d1.r, d2.r, terminal_velocity(k.nfrags_tv, d1.r),
+                                              terminal_velocity(k.nfrags_tv, d2.r));
+  constexpr double ckemax = 16.49789599 / 1000000;
+  const double cke_cap = fmin(ckemax, cke);
+  const double g = pow(cke_cap * 1e6, 0.135);
+  return 1.0 / (1.5 - g) + 11.0 / 6.0;
+}
+// CoalBuReFlag: 1 coalescence, 2 breakup, 0 rebound (coalbure_flag.hpp:41-198)
+__device__ __forceinline__ uint32_t coalbure_flag(const CollCfg& k, double phi, const DropAttrs& d1,
+                                                  const DropAttrs& d2) {
+  if (k.flag_kind == CLEO_FLAG_CONST) return phi < k.coaleff ? 1u : 2u;
+  const double r1 = d1.r, r2 = d2.r;
+  const double cke =
+      collision_kinetic_energy(r1, r2, terminal_velocity(k.flag_tv, r1), terminal_velocity(k.flag_tv, r2));
+  if (k.flag_kind == CLEO_FLAG_STRAUB) return phi < coaleff_straub(r1, r2, cke) ? 1u : 2u;
+  if (k.flag_kind == CLEO_FLAG_SU) {
+    if (cke < surf_energy(fmin(r1, r2))) return 0u;
+    return cke < surf_energy_coal(r1, r2) ? 1u : 2u;
+  }
+  // TS: below the small drop's surface energy rebound or coalescence, below the large drop's
+  // coalescence or breakup, above it breakup
+  if (cke < surf_energy(fmin(r1, r2))) return phi < coaleff_straub(r1, r2, cke) ? 1u : 0u;
+  if (cke < surf_energy(fmax(r1, r2))) return phi < coaleff_straub(r1, r2, cke) ? 1u : 2u;
+  return 2u;
+}
 // PairEnactX dispatch; bit0 of the return: drop1 null, bit1: enacted; error via *err
 __device__ __forceinline__ uint32_t enact_pair(const CollCfg& k, DropAttrs& d1, DropAttrs& d2, double prob,
                                                double phi_coll, double phi_out, const Solute& s, int& err) {
   bool null1 = false;
   if (k.enact_kind == CLEO_ENACT_BREAKUP) {
     if (phi_coll < (prob - floor(prob))) {
-      err = breakup(k.nfrags, d1, d2, s);
+      err = breakup(breakup_nfrags(k, d1, d2), d1, d2, s);
       return 2u;
     }
     return 0u;
   }
   const uint64_t gamma = coal_gamma(d1.xi, d2.xi, prob, phi_coll);
   if (gamma == 0) return 0u;
-  if (k.enact_kind == CLEO_ENACT_COAL || phi_out < k.coaleff) {
+  const uint32_t flag = k.enact_kind == CLEO_ENACT_COAL ? 1u : coalbure_flag(k, phi_out, d1, d2);
+  if (flag == 1u) {
     err = coalesce(gamma, d1, d2, s, null1);
-  } else {
-    err = breakup(k.nfrags, d1, d2, s);
+  } else if (flag == 2u) {
+    err = breakup(breakup_nfrags(k, d1, d2), d1, d2, s);
   }
   return 2u | (null1 ? 1u : 0u);
 }

@@ -56,14 +56,6 @@ struct ConstProb {
   int kind() const { return CLEO_PROB_CONST; }
   double param() const { return kernel; }
 };
-struct ConstNFrags {
-  double nfrags;
-  explicit ConstNFrags(double n) : nfrags(n) {}
-};
-struct ConstCoalBuReFlag {
-  double coaleff;
-  explicit ConstCoalBuReFlag(double c) : coaleff(c) {}
-};
 struct NullTerminalVelocity {
   int kind() const { return CLEO_TV_NULL; }
 };
@@ -76,6 +68,86 @@ struct RogersYauTerminalVelocity {
 struct RogersGKTerminalVelocity {
   int kind() const { return CLEO_TV_ROGERS_GK; }
 };
+// the descriptors write their kinds into the C-ABI parameters; probabilities without a terminal
+// velocity leave prob_tv at CLEO_TV_NULL
+template <typename Probability>
+inline void apply_prob(const Probability& prob, cleo_coll_params& p) {
+  p.prob_kind = prob.kind();
+  p.prob_param = prob.param();
+}
+// LowListCoalProb<TerminalVelocity> / LowListBuProb<TerminalVelocity> (lowlistprob.hpp:40-153)
+template <typename TerminalVelocity>
+struct LowListCoalProb {
+  TerminalVelocity terminalv;
+  explicit LowListCoalProb(TerminalVelocity tv) : terminalv(tv) {}
+  int kind() const { return CLEO_PROB_LOWLIST_COAL; }
+  double param() const { return 1.0; }
+  int tv() const { return terminalv.kind(); }
+};
+template <typename TerminalVelocity>
+struct LowListBuProb {
+  TerminalVelocity terminalv;
+  explicit LowListBuProb(TerminalVelocity tv) : terminalv(tv) {}
+  int kind() const { return CLEO_PROB_LOWLIST_BU; }
+  double param() const { return 1.0; }
+  int tv() const { return terminalv.kind(); }
+};
+template <typename TerminalVelocity>
+inline void apply_prob(const LowListCoalProb<TerminalVelocity>& prob, cleo_coll_params& p) {
+  p.prob_kind = prob.kind();
+  p.prob_param = prob.param();
+  p.prob_tv = prob.tv();
+}
+template <typename TerminalVelocity>
+inline void apply_prob(const LowListBuProb<TerminalVelocity>& prob, cleo_coll_params& p) {
+  p.prob_kind = prob.kind();
+  p.prob_param = prob.param();
+  p.prob_tv = prob.tv();
+}
+
+// NFragments (breakup_nfrags.hpp:38-91)
+struct ConstNFrags {
+  double nfrags;
+  explicit ConstNFrags(double n) : nfrags(n) {}
+  void apply(cleo_coll_params& p) const {
+    p.nfrags_kind = CLEO_NFRAGS_CONST;
+    p.nfrags = nfrags;
+  }
+};
+template <typename TerminalVelocity>
+struct CollisionKineticEnergyNFrags {
+  TerminalVelocity terminalv;
+  explicit CollisionKineticEnergyNFrags(TerminalVelocity tv) : terminalv(tv) {}
+  void apply(cleo_coll_params& p) const {
+    p.nfrags_kind = CLEO_NFRAGS_CKE;
+    p.nfrags_tv = terminalv.kind();
+  }
+};
+
+// CoalBuReFlag (coalbure_flag.hpp:41-198)
+struct ConstCoalBuReFlag {
+  double coaleff;
+  explicit ConstCoalBuReFlag(double c) : coaleff(c) {}
+  void apply(cleo_coll_params& p) const {
+    p.flag_kind = CLEO_FLAG_CONST;
+    p.coalbure_coaleff = coaleff;
+  }
+};
+template <int Kind, typename TerminalVelocity>
+struct KineticCoalBuReFlag {
+  TerminalVelocity terminalv;
+  explicit KineticCoalBuReFlag(TerminalVelocity tv) : terminalv(tv) {}
+  void apply(cleo_coll_params& p) const {
+    p.flag_kind = Kind;
+    p.flag_tv = terminalv.kind();
+  }
+};
+template <typename TerminalVelocity>
+using SUCoalBuReFlag = KineticCoalBuReFlag<CLEO_FLAG_SU, TerminalVelocity>;
+template <typename TerminalVelocity>
+using TSCoalBuReFlag = KineticCoalBuReFlag<CLEO_FLAG_TS, TerminalVelocity>;
+template <typename TerminalVelocity>
+using StraubCoalBuReFlag = KineticCoalBuReFlag<CLEO_FLAG_STRAUB, TerminalVelocity>;
 
 // ---- MicrophysicalProcess descriptors -------------------------------------------------------
 // ConstTstepMicrophysics timing (microphysicalprocess.hpp:243-257)
@@ -112,29 +184,38 @@ inline CondensationProcess Condensation(unsigned int interval,
   return p;
 }
 
-template <typename Probability>
 inline CollisionsProcess make_colls(unsigned int interval, const std::function<double(unsigned int)>& int2realtime,
-                                    const Probability& prob, int enact, double nfrags, double coaleff) {
+                                    int enact) {
   require_same(int2realtime(interval), (static_cast<double>(interval) / 100000.0) * 1000.0, "int2realtime");
   CollisionsProcess p{{interval}, {}};
-  p.params = cleo_coll_params{interval, prob.kind(), prob.param(), enact, nfrags, coaleff};
+  p.params = cleo_coll_params{};
+  p.params.interval = interval;
+  p.params.enact_kind = enact;
   return p;
 }
 template <typename Probability>
 inline CollisionsProcess CollCoal(unsigned int interval, const std::function<double(unsigned int)>& int2realtime,
                                   const Probability& prob) {
-  return make_colls(interval, int2realtime, prob, CLEO_ENACT_COAL, 0.0, 0.0);
+  CollisionsProcess p = make_colls(interval, int2realtime, CLEO_ENACT_COAL);
+  apply_prob(prob, p.params);
+  return p;
 }
-template <typename Probability>
+template <typename Probability, typename NFrags>
 inline CollisionsProcess CollBu(unsigned int interval, const std::function<double(unsigned int)>& int2realtime,
-                                const Probability& prob, const ConstNFrags& nfrags) {
-  return make_colls(interval, int2realtime, prob, CLEO_ENACT_BREAKUP, nfrags.nfrags, 0.0);
+                                const Probability& prob, const NFrags& nfrags) {
+  CollisionsProcess p = make_colls(interval, int2realtime, CLEO_ENACT_BREAKUP);
+  apply_prob(prob, p.params);
+  nfrags.apply(p.params);
+  return p;
 }
-template <typename Probability>
+template <typename Probability, typename NFrags, typename Flag>
 inline CollisionsProcess CoalBuRe(unsigned int interval, const std::function<double(unsigned int)>& int2realtime,
-                                  const Probability& prob, const ConstNFrags& nfrags,
-                                  const ConstCoalBuReFlag& flag) {
-  return make_colls(interval, int2realtime, prob, CLEO_ENACT_COALBURE, nfrags.nfrags, flag.coaleff);
+                                  const Probability& prob, const NFrags& nfrags, const Flag& flag) {
+  CollisionsProcess p = make_colls(interval, int2realtime, CLEO_ENACT_COALBURE);
+  apply_prob(prob, p.params);
+  nfrags.apply(p.params);
+  flag.apply(p.params);
+  return p;
 }
 
 // cond >> colls (the order the reference examples use, e.g. rainshaft1d main: cond >> colls)

@@ -115,13 +115,30 @@ typedef struct {
 enum {
   CLEO_PROB_GOLOVIN = 0, /* golovinprob.cpp:32-43 */
   CLEO_PROB_LONG = 1,    /* longhydroprob.cpp:30-48 + hydrodynamicprob.hpp:50-66 */
-  CLEO_PROB_CONST = 2    /* constprob.hpp:32-50 */
+  CLEO_PROB_CONST = 2,   /* constprob.hpp:32-50 */
+  CLEO_PROB_LOWLIST_COAL = 3, /* LowListCoalProb<prob_tv>: Long (coaleff 1) x Low-List coaleff,
+                                 lowlistprob.hpp:40-123 */
+  CLEO_PROB_LOWLIST_BU = 4    /* LowListBuProb<prob_tv>: Long (coaleff 1) x (1 - Low-List coaleff),
+                                 lowlistprob.hpp:136-153 */
 };
 /* Collision enactments (PairEnactX, collisions.hpp:64-67) */
 enum {
   CLEO_ENACT_COAL = 0,     /* DoCoalescence, coalescence.cpp:39-196 */
   CLEO_ENACT_BREAKUP = 1,  /* DoBreakup<ConstNFrags>, breakup.hpp:140-268 */
   CLEO_ENACT_COALBURE = 2  /* DoCoalBuRe<ConstNFrags, ConstCoalBuReFlag>, coalbure.hpp:207-256 */
+};
+/* Fragment numbers of a breakup (NFragments, breakup_nfrags.hpp:30-91) */
+enum {
+  CLEO_NFRAGS_CONST = 0, /* ConstNFrags(nfrags), breakup_nfrags.hpp:38-48 */
+  CLEO_NFRAGS_CKE = 1    /* CollisionKineticEnergyNFrags<nfrags_tv>, breakup_nfrags.hpp:50-91 */
+};
+/* Coalescence / breakup / rebound decision of DoCoalBuRe (CoalBuReFlag, coalbure_flag.hpp:34-198):
+   1 coalescence, 2 breakup, else rebound (nothing changes) */
+enum {
+  CLEO_FLAG_CONST = 0,  /* ConstCoalBuReFlag(coalbure_coaleff), coalbure_flag.hpp:179-198 */
+  CLEO_FLAG_SU = 1,     /* SUCoalBuReFlag<flag_tv> (Szakall and Urbich 2018), coalbure_flag.hpp:41-74 */
+  CLEO_FLAG_TS = 2,     /* TSCoalBuReFlag<flag_tv> (Testik 2011 + Straub 2010), coalbure_flag.hpp:76-147 */
+  CLEO_FLAG_STRAUB = 3  /* StraubCoalBuReFlag<flag_tv> (Straub 2010), coalbure_flag.hpp:149-177 */
 };
 /* Terminal velocity formulas (VelocityFormula, terminalvelocity.hpp:36-124) */
 enum {
@@ -138,6 +155,11 @@ typedef struct {
   int32_t enact_kind;   /* CLEO_ENACT_* */
   double nfrags;        /* ConstNFrags (breakup / coalbure) */
   double coalbure_coaleff; /* ConstCoalBuReFlag coaleff */
+  int32_t nfrags_kind;  /* CLEO_NFRAGS_* (breakup / coalbure); 0 = ConstNFrags(nfrags) */
+  int32_t flag_kind;    /* CLEO_FLAG_* (coalbure); 0 = ConstCoalBuReFlag(coalbure_coaleff) */
+  int32_t prob_tv;      /* CLEO_TV_* of the Low-List probabilities (their template argument) */
+  int32_t nfrags_tv;    /* CLEO_TV_* of CLEO_NFRAGS_CKE */
+  int32_t flag_tv;      /* CLEO_TV_* of the SU / TS / Straub flags */
 } cleo_coll_params;
 
 typedef struct {

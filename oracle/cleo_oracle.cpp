@@ -441,11 +441,81 @@ static inline double prob_long(const Superdrop& a, const Superdrop& b, double co
   const double hydro_kernel = prob_jk_const * eff * sumrsqrd * vdiff;
   return hydro_kernel * DELT_DELVOL;
 }
+
+/* ---------------- collision kinetics (Low and List 1982a) ---------------- */
+/* surface tension energies [J] of one drop, of two and of their coalesced drop
+   (collisionkinetics.hpp:72-93, collisionkinetics.cpp:62-71; surfconst cleoconstants.hpp:102) */
+constexpr double SURF_SIGMA = 7.28e-2;
+constexpr double surfconst = 4.0 * SURF_SIGMA * PI * dlc::R0 * dlc::R0;
+static inline double surfenergy(double r) { return surfconst * (r * r); }
+static inline double total_surfenergy(double r1, double r2) { return surfconst * ((r1 * r1) + (r2 * r2)); }
+static inline double coal_surfenergy(double r1, double r2) {
+  return surfconst * std::pow(r1 * r1 * r1 + r2 * r2 * r2, 2.0 / 3.0);
+}
+/* collisionkinetics.cpp:33-46 */
+static inline double coll_kinetic_energy(double r1, double r2, double v1, double v2) {
+  constexpr double ckeconst = dlc::R0 * dlc::R0 * dlc::R0 * 2.0 / 3.0 * DC::RHO_L * PI * dlc::W0 * dlc::W0;
+  const double r1cubed = r1 * r1 * r1;
+  const double ratio = r1 / r2;
+  const double rratio = r1cubed / (1 + ratio * ratio * ratio);
+  const double vdiff = v1 - v2;
+  return ckeconst * rratio * vdiff * vdiff;
+}
+static inline double cke_of(int tv, const Superdrop& a, const Superdrop& b) {
+  return coll_kinetic_energy(a.attrs.radius, b.attrs.radius, terminalv(tv, a.attrs.radius),
+                             terminalv(tv, b.attrs.radius));
+}
+/* Straub et al. 2010 eq. 5 (coalescence_efficiency.cpp:27-37) */
+static inline double ecoal_straub(double r1, double r2, double cke) {
+  return std::exp(-1.15 * (cke / coal_surfenergy(r1, r2)));
+}
+/* LowListCoalProb::coaleff, Low and List 1982a eqs. 4.5-4.6 (lowlistprob.hpp:50-91) */
+static inline double lowlist_coaleff(int tv, const Superdrop& a, const Superdrop& b) {
+  const double r1 = a.attrs.radius, r2 = b.attrs.radius;
+  const double etot = cke_of(tv, a, b) + total_surfenergy(r1, r2) - coal_surfenergy(r1, r2);
+  if (etot < 5e-6) {
+    const double alpha = 1 + std::fmin(r1, r2) / std::fmax(r1, r2);
+    const double expo = -2.62e6 * SURF_SIGMA * etot * etot / coal_surfenergy(r1, r2);
+    return 0.778 * (1.0 / (alpha * alpha)) * std::exp(expo);
+  }
+  return 0.0;
+}
+/* NFragments: ConstNFrags, or CollisionKineticEnergyNFrags (breakup_nfrags.hpp:38-91) */
+static inline double nfrags_of(const cleo_coll_params& p, const Superdrop& a, const Superdrop& b) {
+  if (p.nfrags_kind != CLEO_NFRAGS_CKE) return p.nfrags;
+  const double cke_cap = std::fmin(16.49789599 / 1000000, cke_of(p.nfrags_tv, a, b));
+  return 1.0 / (1.5 - std::pow(cke_cap * 1e6, 0.135)) + 11.0 / 6.0;
+}
+/* CoalBuReFlag: 1 coalescence, 2 breakup, 0 rebound (coalbure_flag.hpp:41-198) */
+static inline unsigned coalbure_flag(const cleo_coll_params& p, double phi, const Superdrop& a,
+                                     const Superdrop& b) {
+  if (p.flag_kind == CLEO_FLAG_CONST) return phi < p.coalbure_coaleff ? 1 : 2;
+  const double r1 = a.attrs.radius, r2 = b.attrs.radius;
+  const double cke = cke_of(p.flag_tv, a, b);
+  switch (p.flag_kind) {
+    case CLEO_FLAG_SU:
+      if (cke < surfenergy(std::fmin(r1, r2))) return 0;
+      if (cke < coal_surfenergy(r1, r2)) return 1;
+      return 2;
+    case CLEO_FLAG_TS:
+      if (cke < surfenergy(std::fmin(r1, r2))) return phi < ecoal_straub(r1, r2, cke) ? 1 : 0;
+      if (cke < surfenergy(std::fmax(r1, r2))) return phi < ecoal_straub(r1, r2, cke) ? 1 : 2;
+      return 2;
+    default: return phi < ecoal_straub(r1, r2, cke) ? 1 : 2; /* Straub */
+  }
+}
+
 static inline double pair_prob(const cleo_coll_params& p, const Superdrop& a, const Superdrop& b,
                                double DELT, double VOLUME) {
   switch (p.prob_kind) {
     case CLEO_PROB_GOLOVIN: return prob_golovin(a, b, DELT, VOLUME);
     case CLEO_PROB_LONG: return prob_long(a, b, p.prob_param, DELT, VOLUME);
+    case CLEO_PROB_LOWLIST_COAL: /* lowlistprob.hpp:117-123 */
+      return prob_long(a, b, 1.0, DELT, VOLUME) * lowlist_coaleff(p.prob_tv, a, b);
+    case CLEO_PROB_LOWLIST_BU: { /* lowlistprob.hpp:144-153 */
+      const double bueff = 1.0 - lowlist_coaleff(p.prob_tv, a, b);
+      return prob_long(a, b, 1.0, DELT, VOLUME) * bueff;
+    }
     default: return p.prob_param * DELT / VOLUME; /* constprob.hpp:45-48 */
   }
 }
@@ -547,16 +617,17 @@ static inline size_t enact(const cleo_coll_params& p, Superdrop& d1, Superdrop& 
   if (p.enact_kind == CLEO_ENACT_BREAKUP) {
     if (phi_coll < (prob - std::floor(prob))) { /* breakup.hpp:157-166 */
       enacted = true;
-      breakup_pair(p.nfrags, d1, d2, s, e);
+      breakup_pair(nfrags_of(p, d1, d2), d1, d2, s, e);
     }
     return 0;
   }
-  /* CoalBuRe with ConstCoalBuReFlag (coalbure.hpp:207-256, coalbure_flag.hpp:179-198) */
+  /* CoalBuRe (coalbure.hpp:207-256): the flag picks coalescence, breakup or rebound */
   const uint64_t gamma = coal_gamma(d1.attrs.xi, d2.attrs.xi, prob, phi_coll);
   if (gamma != 0) {
     enacted = true;
-    if (phi_out < p.coalbure_coaleff) return coalesce_pair(gamma, d1, d2, s, e);
-    breakup_pair(p.nfrags, d1, d2, s, e);
+    const unsigned flag = coalbure_flag(p, phi_out, d1, d2);
+    if (flag == 1) return coalesce_pair(gamma, d1, d2, s, e);
+    if (flag == 2) breakup_pair(nfrags_of(p, d1, d2), d1, d2, s, e);
   }
   return 0;
 }
@@ -1348,6 +1419,40 @@ double orc_pair_prob(int kind, double param, double r1, double xi1, double r2, d
   p.prob_kind = kind;
   p.prob_param = param;
   return pair_prob(p, a, b, DELT, VOLUME);
+}
+/* collision kinetics of two drops of dimensionless radii r1, r2 */
+double orc_cke(double r1, double r2, double v1, double v2) { return coll_kinetic_energy(r1, r2, v1, v2); }
+void orc_surf_energies(double r1, double r2, double* out) {
+  out[0] = surfenergy(r1);
+  out[1] = total_surfenergy(r1, r2);
+  out[2] = coal_surfenergy(r1, r2);
+}
+static inline void two_drops(double r1, double r2, Superdrop& a, Superdrop& b) {
+  a = Superdrop{};
+  b = Superdrop{};
+  a.attrs.radius = r1;
+  b.attrs.radius = r2;
+  a.attrs.xi = b.attrs.xi = 1;
+}
+double orc_lowlist_coaleff(int tv, double r1, double r2) {
+  Superdrop a, b;
+  two_drops(r1, r2, a, b);
+  return lowlist_coaleff(tv, a, b);
+}
+double orc_nfrags(const cleo_coll_params* p, double r1, double r2) {
+  Superdrop a, b;
+  two_drops(r1, r2, a, b);
+  return nfrags_of(*p, a, b);
+}
+int32_t orc_coalbure_flag(const cleo_coll_params* p, double phi, double r1, double r2) {
+  Superdrop a, b;
+  two_drops(r1, r2, a, b);
+  return static_cast<int32_t>(coalbure_flag(*p, phi, a, b));
+}
+double orc_pair_prob_params(const cleo_coll_params* p, double r1, double r2, double DELT, double VOLUME) {
+  Superdrop a, b;
+  two_drops(r1, r2, a, b);
+  return pair_prob(*p, a, b, DELT, VOLUME);
 }
 /* enact one collision on two drops given as (xi, r, msol) arrays of length 2 (drop1, drop2) */
 int32_t orc_enact_pair(const cleo_coll_params* p, uint64_t* xi, double* r, double* msol, double prob,

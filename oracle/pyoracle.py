@@ -37,6 +37,20 @@ def load(kind="serial"):
     lib.orc_solve_condensation.restype = C.c_double
     lib.orc_pair_prob.argtypes = [C.c_int] + [C.c_double] * 7
     lib.orc_pair_prob.restype = C.c_double
+    from cleo_amd.abi import CollParams
+    pp = C.POINTER(CollParams)
+    lib.orc_cke.argtypes = [C.c_double] * 4
+    lib.orc_cke.restype = C.c_double
+    lib.orc_surf_energies.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double)]
+    lib.orc_surf_energies.restype = None
+    lib.orc_lowlist_coaleff.argtypes = [C.c_int, C.c_double, C.c_double]
+    lib.orc_lowlist_coaleff.restype = C.c_double
+    lib.orc_nfrags.argtypes = [pp, C.c_double, C.c_double]
+    lib.orc_nfrags.restype = C.c_double
+    lib.orc_coalbure_flag.argtypes = [pp, C.c_double, C.c_double, C.c_double]
+    lib.orc_coalbure_flag.restype = C.c_int32
+    lib.orc_pair_prob_params.argtypes = [pp, C.c_double, C.c_double, C.c_double, C.c_double]
+    lib.orc_pair_prob_params.restype = C.c_double
     lib.orc_constants.argtypes = [C.POINTER(C.c_double)]
     lib.orc_constants.restype = None
     lib.orc_num_threads.restype = C.c_int

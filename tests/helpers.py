@@ -19,8 +19,11 @@ def cond_params(interval=100, alter=1, maxniters=50, rtol=1e-6, atol=1e-8, minsu
     return abi.CondParams(interval, alter, maxniters, rtol, atol, minsub, warm, warm_iters)
 
 
-def coll_params(prob=abi.PROB_LONG, param=1.0, enact=abi.ENACT_COALBURE, nfrags=5.0, coaleff=0.85, interval=100):
-    return abi.CollParams(interval, prob, param, enact, nfrags, coaleff)
+def coll_params(prob=abi.PROB_LONG, param=1.0, enact=abi.ENACT_COALBURE, nfrags=5.0, coaleff=0.85, interval=100,
+                nfrags_kind=abi.NFRAGS_CONST, flag_kind=abi.FLAG_CONST, prob_tv=abi.TV_NULL,
+                nfrags_tv=abi.TV_NULL, flag_tv=abi.TV_NULL):
+    return abi.CollParams(interval, prob, param, enact, nfrags, coaleff, nfrags_kind, flag_kind, prob_tv,
+                          nfrags_tv, flag_tv)
 
 
 def motion_params(interval=100, tv=abi.TV_SIMMEL):

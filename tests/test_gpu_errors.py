@@ -53,6 +53,19 @@ def test_breakup_nfrags(gpu, oracle):
         assert _code(d, rc) == 7
 
 
+def test_breakup_cke_nfrags_without_velocity(gpu, oracle):
+    print("case: kinetic-energy nfrags with no terminal velocity (nfrags = 2.5) abort", flush=True)
+    pop = P.coal_breakup(SEED, ndims=(2, 2, 1), mean=60)
+    # CKE = 0 without a terminal velocity, so nfrags = 2/3 + 11/6 and breakup.hpp:198-201 aborts;
+    # coalbure with coaleff 0 makes every enacted collision a breakup
+    p = coll_params(prob=abi.PROB_CONST, param=1e30, enact=abi.ENACT_COALBURE, coaleff=0.0,
+                    nfrags_kind=abi.NFRAGS_CKE, nfrags_tv=abi.TV_NULL)
+    for d in _both(gpu, oracle, pop):
+        d.rng_counter(1)
+        rc = d.collisions(p)
+        assert _code(d, rc) == 7
+
+
 def test_timestep_below_minsubstep(gpu, oracle):
     print("case: delt < MINSUBTSTEP abort", flush=True)
     pop = P.condensation(SEED, ndims=(2, 2, 2), mean=5)

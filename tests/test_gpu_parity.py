@@ -87,6 +87,76 @@ def test_collisions_injected(gpu, oracle, enact, tiled):
     assert g.stats()["coll_events"] == o.stats()["coll_events"]
 
 
+# the Low-List probabilities, the kinetic-energy NFragments and the SU / TS / Straub flags, each
+# with a terminal velocity formula of its own (lowlistprob.hpp, breakup_nfrags.hpp, coalbure_flag.hpp)
+KINETIC_CASES = {
+    "lowlist_coal": dict(prob=abi.PROB_LOWLIST_COAL, prob_tv=abi.TV_SIMMEL, enact=abi.ENACT_COAL),
+    "lowlist_bu_cke": dict(prob=abi.PROB_LOWLIST_BU, prob_tv=abi.TV_SIMMEL, enact=abi.ENACT_BREAKUP,
+                           nfrags_kind=abi.NFRAGS_CKE, nfrags_tv=abi.TV_SIMMEL),
+    "su_cke": dict(prob=abi.PROB_LONG, enact=abi.ENACT_COALBURE, nfrags_kind=abi.NFRAGS_CKE,
+                   nfrags_tv=abi.TV_ROGERS_YAU, flag_kind=abi.FLAG_SU, flag_tv=abi.TV_SIMMEL),
+    "ts": dict(prob=abi.PROB_LONG, enact=abi.ENACT_COALBURE, flag_kind=abi.FLAG_TS, flag_tv=abi.TV_ROGERS_GK),
+    "straub": dict(prob=abi.PROB_LONG, enact=abi.ENACT_COALBURE, flag_kind=abi.FLAG_STRAUB,
+                   flag_tv=abi.TV_SIMMEL),
+}
+
+
+def _rain(seed, ndims=(4, 4, 2), mean=128):
+    """The coalescence/breakup population with radii log-uniform in 20 um .. 2.5 mm, so that the
+    kinetic energies fall on both sides of every flag's boundaries."""
+    pop = P.coal_breakup(seed, ndims=ndims, mean=mean)
+    rng = np.random.default_rng(seed + 1)
+    wet = np.exp(rng.uniform(np.log(20.0), np.log(2500.0), pop.n))
+    pop.supers["radius"] = np.maximum(wet, pop.supers["radius"])
+    return pop
+
+
+@pytest.mark.parametrize("tiled", [0, 1])
+@pytest.mark.parametrize("case", sorted(KINETIC_CASES))
+def test_collisions_kinetic_variants(gpu, oracle, case, tiled):
+    _say(f"case: collisions {case} injected tiled={tiled}")
+    pop = _rain(SEED)
+    g, o = _both(gpu, oracle, pop)
+    _coll_variant(gpu, g, tiled)
+    p = coll_params(**KINETIC_CASES[case])
+    rng = np.random.default_rng(7)
+    for step in range(3):
+        js, ph = injected_stream(o.refs(), rng)
+        g.rng_injected(js, ph)
+        o.rng_injected(js, ph)
+        assert g.collisions(p) == 0
+        assert o.collisions(p) == 0
+        assert g.error()[0] == 0 and o.error()[0] == 0
+        assert_supers_match(g.download(), o.download(), what=f"{case} step {step}")
+    sg, so = g.stats(), o.stats()
+    assert sg["coll_events"] == so["coll_events"] and sg["coll_events"] > 0
+
+
+@pytest.mark.parametrize("case", sorted(KINETIC_CASES))
+def test_collisions_kinetic_variants_counter_rng(gpu, oracle, case):
+    _say(f"case: collisions {case} counter RNG, 4 events")
+    pop = _rain(SEED + 5, ndims=(5, 3, 3), mean=90)
+    g, o = _both(gpu, oracle, pop)
+    g.rng_counter(31)
+    o.rng_counter(31)
+    p = coll_params(**KINETIC_CASES[case])
+    for _ in range(4):
+        assert g.collisions(p) == 0
+        assert o.collisions(p) == 0
+    assert g.error()[0] == 0 and o.error()[0] == 0
+    assert_supers_match(g.download(), o.download(), what=case)
+
+
+def test_collision_params_rejected(gpu):
+    _say("case: unknown collision kinds rejected")
+    pop = P.coal_breakup(SEED, ndims=(2, 2, 2), mean=10)
+    g = make_domain(gpu, pop)
+    g.rng_counter(1)
+    for bad in (dict(prob=7), dict(enact=5), dict(nfrags_kind=3), dict(flag_kind=9), dict(prob_tv=8),
+                dict(flag_tv=-1), dict(interval=0)):
+        assert g.collisions(coll_params(**bad)) == abi.CLEO_E_ARG, bad
+
+
 @pytest.mark.parametrize("tiled", [0, 1])
 def test_collisions_counter_rng_golovin_box(gpu, oracle, tiled):
     _say(f"case: Golovin box 8192 drops, counter RNG, 5 steps (HBM scratch path) tiled={tiled}")
