@@ -562,6 +562,9 @@ __device__ __forceinline__ double prob_long(double r1, double r2, double coaleff
 }
 // golovinprob.cpp:32-43, longhydroprob.cpp:30-48 + hydrodynamicprob.hpp:50-66, constprob.hpp:45-48,
 // lowlistprob.hpp:117-123,144-153
+// Kinetic = false compiles out the Low-List probabilities and the kinetic-energy NFragments and
+// flags (launches whose CollCfg names none of them)
+template <bool Kinetic>
 __device__ __forceinline__ double pair_probability(const CollCfg& k, double r1, double r2, double VOLUME) {
   if (k.prob_kind == CLEO_PROB_GOLOVIN) {
     constexpr double prob_jk_const = 1.5e3 * dimless::R0 * dimless::R0 * dimless::R0;
@@ -570,11 +573,13 @@ __device__ __forceinline__ double pair_probability(const CollCfg& k, double r1, 
     return kern * DELT_DELVOL;
   }
   if (k.prob_kind == CLEO_PROB_LONG) return prob_long(r1, r2, k.prob_param, k.DELT, VOLUME);
-  if (k.prob_kind == CLEO_PROB_LOWLIST_COAL)
-    return prob_long(r1, r2, 1.0, k.DELT, VOLUME) * coaleff_lowlist(k.prob_tv, r1, r2);
-  if (k.prob_kind == CLEO_PROB_LOWLIST_BU) {
-    const double bueff = 1.0 - coaleff_lowlist(k.prob_tv, r1, r2);
-    return prob_long(r1, r2, 1.0, k.DELT, VOLUME) * bueff;
+  if constexpr (Kinetic) {
+    if (k.prob_kind == CLEO_PROB_LOWLIST_COAL)
+      return prob_long(r1, r2, 1.0, k.DELT, VOLUME) * coaleff_lowlist(k.prob_tv, r1, r2);
+    if (k.prob_kind == CLEO_PROB_LOWLIST_BU) {
+      const double bueff = 1.0 - coaleff_lowlist(k.prob_tv, r1, r2);
+      return prob_long(r1, r2, 1.0, k.DELT, VOLUME) * bueff;
+    }
   }
   return k.prob_param * k.DELT / VOLUME;
 }
@@ -712,24 +717,34 @@ __device__ __forceinline__ uint32_t coalbure_flag(const CollCfg& k, double phi, 
   if (cke < surf_energy(fmax(r1, r2))) return phi < coaleff_straub(r1, r2, cke) ? 1u : 2u;
   return 2u;
 }
+// true when a CollCfg needs the Kinetic instantiations
+inline bool coll_is_kinetic(const CollCfg& k) {
+  return k.prob_kind == CLEO_PROB_LOWLIST_COAL || k.prob_kind == CLEO_PROB_LOWLIST_BU ||
+         k.nfrags_kind != CLEO_NFRAGS_CONST || k.flag_kind != CLEO_FLAG_CONST;
+}
 // PairEnactX dispatch; bit0 of the return: drop1 null, bit1: enacted; error via *err
+template <bool Kinetic>
 __device__ __forceinline__ uint32_t enact_pair(const CollCfg& k, DropAttrs& d1, DropAttrs& d2, double prob,
                                                double phi_coll, double phi_out, const Solute& s, int& err) {
   bool null1 = false;
   if (k.enact_kind == CLEO_ENACT_BREAKUP) {
     if (phi_coll < (prob - floor(prob))) {
-      err = breakup(breakup_nfrags(k, d1, d2), d1, d2, s);
+      err = breakup(Kinetic ? breakup_nfrags(k, d1, d2) : k.nfrags, d1, d2, s);
       return 2u;
     }
     return 0u;
   }
   const uint64_t gamma = coal_gamma(d1.xi, d2.xi, prob, phi_coll);
   if (gamma == 0) return 0u;
-  const uint32_t flag = k.enact_kind == CLEO_ENACT_COAL ? 1u : coalbure_flag(k, phi_out, d1, d2);
+  uint32_t flag = 1u;
+  if (k.enact_kind != CLEO_ENACT_COAL) {
+    if constexpr (Kinetic) flag = coalbure_flag(k, phi_out, d1, d2);
+    else flag = phi_out < k.coaleff ? 1u : 2u;
+  }
   if (flag == 1u) {
     err = coalesce(gamma, d1, d2, s, null1);
   } else if (flag == 2u) {
-    err = breakup(breakup_nfrags(k, d1, d2), d1, d2, s);
+    err = breakup(Kinetic ? breakup_nfrags(k, d1, d2) : k.nfrags, d1, d2, s);
   }
   return 2u | (null1 ? 1u : 0u);
 }

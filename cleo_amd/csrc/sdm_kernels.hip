@@ -258,6 +258,7 @@ struct PairOut {
   bool enacted;
   int err;
 };
+template <bool Kinetic>
 __device__ __forceinline__ PairOut coll_pair_eval(const CollArgs& a, uint32_t g, uint32_t n, uint32_t jj, uint32_t ka,
                                                   uint32_t kb, uint64_t event) {
   const uint32_t npairs = n / 2;
@@ -271,7 +272,7 @@ __device__ __forceinline__ PairOut coll_pair_eval(const CollArgs& a, uint32_t g,
   // lane-varying flag would put both drops in scratch memory)
   DropAttrs d1{a_first ? A.xi : B.xi, a_first ? A.r : B.r, a_first ? A.msol : B.msol};
   DropAttrs d2{a_first ? B.xi : A.xi, a_first ? B.r : A.r, a_first ? B.msol : A.msol};
-  const double prob_jk = pair_probability(a.k, d1.r, d2.r, VOLUME);
+  const double prob_jk = pair_probability<Kinetic>(a.k, d1.r, d2.r, VOLUME);
   const double prob = scale_p * static_cast<double>(d1.xi) * prob_jk;
   double phi_coll, phi_out;
   if (a.inj_phi) {
@@ -285,7 +286,7 @@ __device__ __forceinline__ PairOut coll_pair_eval(const CollArgs& a, uint32_t g,
     phi_out = u53(r4[2], r4[3]);
   }
   o.err = 0;
-  const uint32_t res = enact_pair(a.k, d1, d2, prob, phi_coll, phi_out, a.sol, o.err);
+  const uint32_t res = enact_pair<Kinetic>(a.k, d1, d2, prob, phi_coll, phi_out, a.sol, o.err);
   o.enacted = (res >> 1) & 1u;
   o.nullA = (res & 1u) && a_first;
   o.nullB = (res & 1u) && !a_first;
@@ -302,9 +303,10 @@ __device__ __forceinline__ void coll_store(const CollArgs& a, uint32_t k, const 
 }
 // pair jj of the drops at ka, kb: the slots 2jj, 2jj+1 of the logical order, and the new attributes
 // when it was enacted; true when one of the two became null; `events` counts enactments
+template <bool Kinetic>
 __device__ __forceinline__ bool coll_pair(const CollArgs& a, uint32_t g, uint32_t base, uint32_t n, uint32_t jj,
                                           uint32_t ka, uint32_t kb, uint64_t event, uint32_t& events) {
-  const PairOut o = coll_pair_eval(a, g, n, jj, ka, kb, event);
+  const PairOut o = coll_pair_eval<Kinetic>(a, g, n, jj, ka, kb, event);
   if (o.err) set_error(a.ctl, o.err, g, base + 2 * jj);
   a.order[base + 2 * jj] = ka;
   a.order[base + 2 * jj + 1] = kb;
@@ -355,7 +357,7 @@ __device__ __forceinline__ void coll_order_nulls(const CollArgs& a, uint32_t g, 
 
 // The permutation `perm` and the drawn indices `jb` live in LDS (kLds) or, for gridboxes above
 // kCollLdsCap drops, in HBM scratch; the two instantiations keep LDS accesses as ds_* instructions.
-template <bool kLds>
+template <bool kLds, bool Kinetic>
 __device__ __forceinline__ void coll_gbx(const CollArgs& a, int lane, uint32_t g, uint32_t base, uint32_t nfull,
                                          uint32_t n, uint32_t* perm, uint32_t* jb, uint32_t (&acc)[3]) {
   wave_sync();  // the previous gridbox of this wave is done with perm / jb
@@ -409,7 +411,7 @@ __device__ __forceinline__ void coll_gbx(const CollArgs& a, int lane, uint32_t g
   for (uint32_t c = 0; c < npairs; c += kWave) {
     const uint32_t jj = c + lane;
     bool nul = false;
-    if (jj < npairs) nul = coll_pair(a, g, base, n, jj, base + perm[2 * jj], base + perm[2 * jj + 1], event, events);
+    if (jj < npairs) nul = coll_pair<Kinetic>(a, g, base, n, jj, base + perm[2 * jj], base + perm[2 * jj + 1], event, events);
     nulls += static_cast<uint32_t>(__popcll(__ballot(nul)));
   }
   // unpaired last drop and the inactive tail (nulls of earlier sub-steps) keep their places
@@ -420,6 +422,7 @@ __device__ __forceinline__ void coll_gbx(const CollArgs& a, int lane, uint32_t g
   acc[2] += nulls;
 }
 
+template <bool Kinetic>
 __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_collisions(CollArgs a) {
   extern __shared__ uint32_t lds_dyn[];
   const int lane = lane_id();
@@ -431,10 +434,10 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_collisions(CollArgs a
     const uint32_t n = a.nact[g];
     if (n <= kCollLdsCap) {
       uint32_t* perm = lds_dyn + static_cast<uint32_t>(w) * 2u * kCollLdsCap;
-      coll_gbx<true>(a, lane, g, base, nfull, n, perm, perm + kCollLdsCap, acc);
+      coll_gbx<true, Kinetic>(a, lane, g, base, nfull, n, perm, perm + kCollLdsCap, acc);
     } else {
       uint32_t* perm = a.big + 2ull * base;
-      coll_gbx<false>(a, lane, g, base, nfull, n, perm, perm + n, acc);
+      coll_gbx<false, Kinetic>(a, lane, g, base, nfull, n, perm, perm + n, acc);
     }
   }
   if (lane == 0)
@@ -481,6 +484,7 @@ __device__ __forceinline__ uint32_t tile_owner(const uint32_t* pre, uint32_t f) 
   return lo;
 }
 
+template <bool Kinetic>
 __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_coll_tiles(CollArgs a) {
   extern __shared__ uint32_t lds_dyn[];
   const int lane = lane_id();
@@ -541,7 +545,7 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_coll_tiles(CollArgs a
       const uint32_t i = tile_owner(L.ppre, f);
       const uint32_t jj = f - L.ppre[i], bi = L.base[i];
       const uint8_t* pi = pb + L.loff[i];
-      if (coll_pair(a, tile * kWave + i, bi, L.n[i], jj, bi + pi[2 * jj], bi + pi[2 * jj + 1], event, events))
+      if (coll_pair<Kinetic>(a, tile * kWave + i, bi, L.n[i], jj, bi + pi[2 * jj], bi + pi[2 * jj + 1], event, events))
         atomicAdd(&L.nulls[i], 1u);
     }
     // the unpaired last drop and the inactive tail of the lane's own gridbox keep their places
@@ -565,7 +569,7 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_coll_tiles(CollArgs a
       const int i = __ffsll(static_cast<unsigned long long>(mb)) - 1;
       mb &= mb - 1;
       const uint32_t bb = readlane_u32(base, i), nb = readlane_u32(n, i);
-      coll_gbx<false>(a, lane, tile * kWave + static_cast<uint32_t>(i), bb, readlane_u32(nfull, i), nb,
+      coll_gbx<false, Kinetic>(a, lane, tile * kWave + static_cast<uint32_t>(i), bb, readlane_u32(nfull, i), nb,
                       a.big + 2ull * bb, a.big + 2ull * bb + nb, acc);
     }
   }
@@ -1511,7 +1515,10 @@ void launch_collisions(cleo_ctx* c, const CollArgs& args) {
     const unsigned nb = gbx_blocks((a.ngbx + kWave - 1) / kWave, a.wpb, c->tiles.coll_grid_waves);
     a.stat = stat_slots(c, static_cast<uint64_t>(nb) * a.wpb);
     kev_mark(c, 0);
-    hipLaunchKernelGGL(k_coll_tiles, dim3(nb), dim3(64 * a.wpb), a.wpb * kCollTileLdsBytes, c->stream, a);
+    if (coll_is_kinetic(a.k))
+      hipLaunchKernelGGL(k_coll_tiles<true>, dim3(nb), dim3(64 * a.wpb), a.wpb * kCollTileLdsBytes, c->stream, a);
+    else
+      hipLaunchKernelGGL(k_coll_tiles<false>, dim3(nb), dim3(64 * a.wpb), a.wpb * kCollTileLdsBytes, c->stream, a);
     kev_mark(c, 1);
     launch_stats_reduce(c, nb * a.wpb, ST_COLL_PAIRS, 3);
     return;
@@ -1520,7 +1527,10 @@ void launch_collisions(cleo_ctx* c, const CollArgs& args) {
   a.stat = stat_slots(c, static_cast<uint64_t>(nb) * a.wpb);
   const size_t lds = static_cast<size_t>(a.wpb) * 2u * kCollLdsCap * sizeof(uint32_t);
   kev_mark(c, 0);
-  hipLaunchKernelGGL(k_collisions, dim3(nb), dim3(64 * a.wpb), lds, c->stream, a);
+  if (coll_is_kinetic(a.k))
+    hipLaunchKernelGGL(k_collisions<true>, dim3(nb), dim3(64 * a.wpb), lds, c->stream, a);
+  else
+    hipLaunchKernelGGL(k_collisions<false>, dim3(nb), dim3(64 * a.wpb), lds, c->stream, a);
   kev_mark(c, 1);
   launch_stats_reduce(c, nb * a.wpb, ST_COLL_PAIRS, 3);
 }
