@@ -309,6 +309,32 @@ def test_graph_replay_collisions_only(gpu):
     assert_supers_match(a.download(), b.download(), exact_fp=True, what="graph colls")
 
 
+@pytest.mark.parametrize("case", ["ts", "lowlist_bu_cke"])
+def test_graph_replay_kinetic_collisions(gpu, oracle, case):
+    _say(f"case: hipGraph replay, collisions {case}, against eager steps and the oracle")
+    pop = _rain(SEED + 9, ndims=(4, 4, 4), mean=100)
+    a, b, o = make_domain(gpu, pop), make_domain(gpu, pop), make_domain(oracle, pop)
+    s = sdm_params(coll=coll_params(**KINETIC_CASES[case]))
+    for d in (a, b, o):
+        d.set_sdm(s)
+        d.rng_counter(4)
+    for step in range(3):
+        assert a.run_step(step * 100, (step + 1) * 100) == 0
+        assert o.run_step(step * 100, (step + 1) * 100) == 0
+    assert gpu.lib.cleo_run_steps_graph(b.h, 0, 100, 3) == 0
+    assert a.error()[0] == 0 and b.error()[0] == 0 and o.error()[0] == 0
+    assert_supers_match(a.download(), b.download(), exact_fp=True, what=f"graph {case}")
+    assert_supers_match(a.download(), o.download(), what=f"oracle {case}")
+
+
+def test_set_sdm_rejects_unknown_collision_kinds(gpu):
+    _say("case: cleo_set_sdm with an unknown flag kind")
+    pop = P.coal_breakup(SEED, ndims=(2, 2, 2), mean=10)
+    d = make_domain(gpu, pop)
+    s = sdm_params(coll=coll_params(flag_kind=11))
+    assert gpu.lib.cleo_set_sdm(d.h, abi.ctypes_byref(s)) == abi.CLEO_E_ARG
+
+
 # Collisions leave the drops in place and record their shuffled order; with motion every second
 # microphysics step, condensation and collisions run while such an order is pending (it is applied
 # first), and the next sort applies the order the last collisions left.
