@@ -84,6 +84,22 @@ class SDMParams(C.Structure):
                 ("cond", CondParams), ("coll", CollParams), ("motion", MotionParams)]
 
 
+PROCESS_CONDENSATION, PROCESS_COLLISIONS = 0, 1
+MAX_PROCESSES = 8
+
+
+class Process(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("cond", CondParams), ("coll", CollParams)]
+
+
+def cond_process(cp):
+    return Process(PROCESS_CONDENSATION, cp, CollParams())
+
+
+def coll_process(kp):
+    return Process(PROCESS_COLLISIONS, CondParams(), kp)
+
+
 class Error(C.Structure):
     _fields_ = [("code", C.c_int32), ("gbxindex", C.c_uint32), ("drop", C.c_uint64)]
 
@@ -163,6 +179,7 @@ _SIGS = {
     "set_rng_counter": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
     "set_rng_injected": (C.c_int, [C.c_void_p, _u32p, C.c_uint64, _dp, C.c_uint64]),
     "set_sdm": (C.c_int, [C.c_void_p, C.POINTER(SDMParams)]),
+    "set_sdm_chain": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "sort": (C.c_int, [C.c_void_p]),
     "motion": (C.c_int, [C.c_void_p, C.POINTER(MotionParams)]),
     "condensation": (C.c_int, [C.c_void_p, C.POINTER(CondParams)]),
@@ -290,6 +307,12 @@ class Domain:
 
     def set_sdm(self, p):
         self.check(self.b.set_sdm(self.h, C.byref(p)), "set_sdm")
+
+    def set_sdm_chain(self, procs, motion=None):
+        """microphysics procs[0] >> procs[1] >> ... (a list of Process), motion a MotionParams or None"""
+        arr = (Process * max(1, len(procs)))(*procs)
+        m = C.byref(motion) if motion is not None else None
+        return self.b.set_sdm_chain(self.h, C.cast(arr, C.c_void_p), len(procs), m)
 
     def sort(self):
         self.check(self.b.sort(self.h), "sort")

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstring>
 #include <fstream>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -99,6 +100,29 @@ int cleo_read_initsupers(const char* path, cleo_supers_soa* out, uint64_t cap, u
   return CLEO_OK;
 }
 
+int cleo_read_initsupers_padded(const char* path, uint64_t maxnsupers, cleo_supers_soa* out, uint64_t* ninit) {
+  if (!path || !out || !ninit) return CLEO_E_ARG;
+  uint64_t n = 0;
+  int rc = cleo_read_initsupers(path, nullptr, 0, &n);
+  if (rc != CLEO_OK) return rc;
+  // InitSupersFromBinary: maxnsupers < initnsupers is an error (init_supers_from_binary.hpp:75-79)
+  if (n > maxnsupers) return CLEO_E_ARG;
+  rc = cleo_read_initsupers(path, out, maxnsupers, &n);
+  if (rc != CLEO_OK) return rc;
+  // add_uninitialised_superdrops_data (init_supers_from_binary.cpp:44-63): out of the domain, NaN
+  // attributes and coordinates, and the values signaling_NaN gives integers (0) for xi and sdId
+  const double nan = std::numeric_limits<double>::signaling_NaN();
+  for (uint64_t k = n; k < maxnsupers; ++k) {
+    out->sdgbxindex[k] = CLEO_OOB_GBXINDEX;
+    out->coord3[k] = out->coord1[k] = out->coord2[k] = nan;
+    out->radius[k] = out->msol[k] = nan;
+    out->xi[k] = 0;
+    out->sdid[k] = 0;
+  }
+  *ninit = n;
+  return CLEO_OK;
+}
+
 int cleo_read_gridfile(const char* path, uint32_t ndims[3], double* zhalf, double* xhalf, double* yhalf) {
   if (!path || !ndims) return CLEO_E_ARG;
   std::ifstream f;
@@ -110,7 +134,11 @@ int cleo_read_gridfile(const char* path, uint32_t ndims[3], double* zhalf, doubl
   if (!read_var(f, meta[0], nd)) return CLEO_E_FILE;
   for (int a = 0; a < 3; ++a)
     if (nd[a] == 0 || nd[a] > 0xFFFFFFFFull) return CLEO_E_FILE;
-  const uint64_t ngbx = nd[0] * nd[1] * nd[2];
+  // a product that wraps, or more gridboxes than the file can hold bounds for, is not a layout
+  uint64_t nzx = 0, ngbx = 0;
+  if (__builtin_mul_overflow(nd[0], nd[1], &nzx) || __builtin_mul_overflow(nzx, nd[2], &ngbx) ||
+      ngbx > fsize / 48)
+    return CLEO_E_FILE;
   // gbx_bounds_from_binary.cpp:38-44: one index and six bounds per gridbox
   if (meta[1].nvar != ngbx || meta[2].nvar != 6 * ngbx) return CLEO_E_FILE;
   std::vector<uint32_t> idx(ngbx);

@@ -298,59 +298,73 @@ int enqueue_motion_condensation(cleo_ctx* c, const cleo_motion_params& mp, const
   return launch_check(c);
 }
 
+uint32_t process_interval(const cleo_process& q) {
+  return q.kind == CLEO_PROCESS_CONDENSATION ? q.cond.interval : q.coll.interval;
+}
+
 // SDMMethods::run_step(t_mdl, t_mdl_next) (sdmmethods.hpp:319-332): host loop over SDM sub-steps;
 // every gridbox follows the same schedule, so each process runs as one launch over all gridboxes.
+// On each sub-step the processes that are on step run in chain order (CombinedMicrophysicalProcess,
+// microphysicalprocess.hpp:65-120); the next sub-step is the earliest next step of any of them.
 int enqueue_run_step(cleo_ctx* c, uint32_t t_mdl, uint32_t t_mdl_next) {
-  const cleo_sdm_params& p = c->sdm;
-  const bool mot = p.enable_motion != 0, cond = p.enable_cond != 0, coll = p.enable_colls != 0;
+  const cleo_ctx::Chain& p = c->sdm;
   const uint32_t UMAX = 0xFFFFFFFFu;
   auto next_of = [](uint32_t t, uint32_t iv) { return ((t / iv) + 1) * iv; };
+  auto on_step = [&](uint32_t i, uint32_t t) {
+    const uint32_t iv = process_interval(p.proc[i]);
+    return t % iv == 0 && iv != UMAX;
+  };
+  auto qcond_now = [&]() {
+    QcondArgs q{c->buf[c->cur], c->off, c->st, c->solute, c->grid.ngbx, wpb_or1(c->tiles.cond_waves_per_block), c->ctl};
+    launch_qcond(c, q);
+  };
   uint32_t t_sdm = t_mdl;
   while (t_sdm < t_mdl_next) {
-    const uint32_t next_move = mot ? next_of(t_sdm, p.motion.interval) : UMAX;
+    const uint32_t next_move = p.motion_on ? next_of(t_sdm, p.motion.interval) : UMAX;
     const uint32_t t_sdm_next = std::min(t_mdl_next, next_move);
     bool pending_qcond = false;
-    bool cond_done = false;  // the condensation at t_sdm already ran inside the sort
-    if (mot && t_sdm % p.motion.interval == 0) {
-      // a condensation due at once is solved in the sort's gather (thread-per-drop variants; not
-      // while guesses are recorded, whose positions are those of the sorted store); otherwise
-      // motion and sort run alone and qcond is recomputed afterwards
-      const bool fuse = cond && t_sdm % p.cond.interval == 0 &&
+    int fused = -1;  // the process whose step at t_sdm already ran inside the sort
+    if (p.motion_on && t_sdm % p.motion.interval == 0) {
+      // when the first process on step at once is a condensation, it is solved in the sort's gather
+      // (thread-per-drop variants; not while guesses are recorded, whose positions are those of the
+      // sorted store); otherwise motion and sort run alone and qcond is recomputed afterwards
+      int first = -1;
+      for (uint32_t i = 0; i < p.n && first < 0; ++i)
+        if (on_step(i, t_sdm)) first = static_cast<int>(i);
+      const bool fuse = first >= 0 && p.proc[first].kind == CLEO_PROCESS_CONDENSATION &&
                         (c->tiles.motion_flat == 1 || c->tiles.motion_flat == 3) && c->tiles.cond_flat == 1 &&
-                        !(p.cond.warm_start && c->rec_guess);
-      const int rc = fuse ? enqueue_motion_condensation(c, p.motion, p.cond) : enqueue_motion(c, p.motion, false);
+                        !(p.proc[first].cond.warm_start && c->rec_guess);
+      const int rc = fuse ? enqueue_motion_condensation(c, p.motion, p.proc[first].cond)
+                          : enqueue_motion(c, p.motion, false);
       if (rc != CLEO_OK) return rc;
       pending_qcond = !fuse;
-      cond_done = fuse;
+      fused = fuse ? first : -1;
     }
     bool any_coll = false;
     for (uint32_t subt = t_sdm; subt < t_sdm_next;) {
-      const bool on_cond = cond && subt % p.cond.interval == 0 && !(cond_done && subt == t_sdm);
-      const bool on_coll = coll && subt % p.coll.interval == 0;
-      if (on_cond) {
-        const int rc = enqueue_condensation(c, p.cond, pending_qcond);
+      for (uint32_t i = 0; i < p.n; ++i) {
+        if (!on_step(i, subt) || (static_cast<int>(i) == fused && subt == t_sdm)) continue;
+        const cleo_process& q = p.proc[i];
+        int rc;
+        if (q.kind == CLEO_PROCESS_CONDENSATION) {
+          rc = enqueue_condensation(c, q.cond, pending_qcond);  // with the qcond recompute first if due
+        } else {
+          if (pending_qcond) qcond_now();  // qcond after the sort comes before any drop changes
+          rc = enqueue_collision_event(c, q.coll);
+          any_coll = true;
+        }
         if (rc != CLEO_OK) return rc;
         pending_qcond = false;
       }
-      if (pending_qcond) {  // qcond after the sort comes before any drop changes
-        QcondArgs q{c->buf[c->cur], c->off, c->st, c->solute, c->grid.ngbx, wpb_or1(c->tiles.cond_waves_per_block),
-                    c->ctl};
-        launch_qcond(c, q);
+      if (pending_qcond) {
+        qcond_now();
         pending_qcond = false;
       }
-      if (on_coll) {
-        const int rc = enqueue_collision_event(c, p.coll);
-        if (rc != CLEO_OK) return rc;
-        any_coll = true;
-      }
-      const uint32_t a = cond ? next_of(subt, p.cond.interval) : UMAX;
-      const uint32_t b = coll ? next_of(subt, p.coll.interval) : UMAX;
-      subt = std::min(a, b);
+      uint32_t nxt = UMAX;
+      for (uint32_t i = 0; i < p.n; ++i) nxt = std::min(nxt, next_of(subt, process_interval(p.proc[i])));
+      subt = nxt;
     }
-    if (pending_qcond) {
-      QcondArgs q{c->buf[c->cur], c->off, c->st, c->solute, c->grid.ngbx, wpb_or1(c->tiles.cond_waves_per_block), c->ctl};
-      launch_qcond(c, q);
-    }
+    if (pending_qcond) qcond_now();
     if (any_coll) neighbour_sort(c, c->cur, 1 - c->cur, &c->ctl->resort_flag, true);
     t_sdm = t_sdm_next;
   }
@@ -697,16 +711,43 @@ int cleo_set_rng_injected(cleo_ctx* c, const uint32_t* j, uint64_t nj, const dou
   return CLEO_OK;
 }
 
-int cleo_set_sdm(cleo_ctx* c, const cleo_sdm_params* p) {
-  if (!c || !p) return CLEO_E_ARG;
-  if ((p->enable_cond && p->cond.interval == 0) || (p->enable_colls && p->coll.interval == 0) ||
-      (p->enable_motion && p->motion.interval == 0))
-    return fail(c, CLEO_E_ARG, "zero interval");
-  if (p->enable_colls)
-    if (const char* m = coll_params_error(p->coll)) return fail(c, CLEO_E_ARG, m);
-  c->sdm = *p;
+int cleo_set_sdm_chain(cleo_ctx* c, const cleo_process* procs, uint32_t nprocs, const cleo_motion_params* motion) {
+  if (!c || (nprocs && !procs)) return CLEO_E_ARG;
+  if (nprocs > CLEO_MAX_PROCESSES) return fail(c, CLEO_E_ARG, "more processes than CLEO_MAX_PROCESSES");
+  for (uint32_t i = 0; i < nprocs; ++i) {
+    const cleo_process& q = procs[i];
+    if (q.kind == CLEO_PROCESS_CONDENSATION) {
+      if (q.cond.interval == 0) return fail(c, CLEO_E_ARG, "zero interval");
+    } else if (q.kind == CLEO_PROCESS_COLLISIONS) {
+      if (const char* m = coll_params_error(q.coll)) return fail(c, CLEO_E_ARG, m);
+    } else {
+      return fail(c, CLEO_E_ARG, "unknown process kind");
+    }
+  }
+  if (motion && motion->interval == 0) return fail(c, CLEO_E_ARG, "zero interval");
+  if (motion && !tv_known(motion->terminalv)) return fail(c, CLEO_E_ARG, "unknown terminal velocity");
+  c->sdm = cleo_ctx::Chain{};
+  for (uint32_t i = 0; i < nprocs; ++i) c->sdm.proc[i] = procs[i];
+  c->sdm.n = nprocs;
+  c->sdm.motion_on = motion != nullptr;
+  if (motion) c->sdm.motion = *motion;
   c->sdm_set = true;
   return CLEO_OK;
+}
+
+int cleo_set_sdm(cleo_ctx* c, const cleo_sdm_params* p) {
+  if (!c || !p) return CLEO_E_ARG;
+  cleo_process q[2]{};
+  uint32_t n = 0;
+  if (p->enable_cond) {
+    q[n].kind = CLEO_PROCESS_CONDENSATION;
+    q[n++].cond = p->cond;
+  }
+  if (p->enable_colls) {
+    q[n].kind = CLEO_PROCESS_COLLISIONS;
+    q[n++].coll = p->coll;
+  }
+  return cleo_set_sdm_chain(c, q, n, p->enable_motion ? &p->motion : nullptr);
 }
 
 int cleo_sort(cleo_ctx* c) {
@@ -720,6 +761,7 @@ int cleo_sort(cleo_ctx* c) {
 
 int cleo_motion(cleo_ctx* c, const cleo_motion_params* p) {
   if (!c || !p) return CLEO_E_ARG;
+  if (!tv_known(p->terminalv)) return fail(c, CLEO_E_ARG, "unknown terminal velocity");
   return enqueue_motion(c, *p, true);
 }
 
@@ -752,24 +794,29 @@ int cleo_collisions(cleo_ctx* c, const cleo_coll_params* p) {
 int cleo_run_step(cleo_ctx* c, uint32_t t_mdl, uint32_t t_mdl_next) {
   if (!c) return CLEO_E_ARG;
   if (!c->sdm_set) return fail(c, CLEO_E_STATE, "cleo_set_sdm not called");
-  if (c->sdm.enable_cond &&
-      step2dimlesstime(c->sdm.cond.interval) < c->sdm.cond.minsubdelt_s / dimless::TIME0)
-    return set_substep_error(c);
+  for (uint32_t i = 0; i < c->sdm.n; ++i) {
+    const cleo_process& q = c->sdm.proc[i];
+    if (q.kind == CLEO_PROCESS_CONDENSATION &&
+        step2dimlesstime(q.cond.interval) < q.cond.minsubdelt_s / dimless::TIME0)
+      return set_substep_error(c);
+  }
   return enqueue_run_step(c, t_mdl, t_mdl_next);
 }
 
 int cleo_run_steps_graph(cleo_ctx* c, uint32_t t_mdl, uint32_t span, uint32_t nsteps) {
   if (!c || span == 0) return CLEO_E_ARG;
   if (!c->sdm_set) return fail(c, CLEO_E_STATE, "cleo_set_sdm not called");
-  if (c->rng_mode != 1 && c->sdm.enable_colls) return fail(c, CLEO_E_STATE, "graph replay needs the counter RNG");
-  const cleo_sdm_params& p = c->sdm;
+  const cleo_ctx::Chain& p = c->sdm;
+  if (c->rng_mode != 1 && p.first_coll()) return fail(c, CLEO_E_STATE, "graph replay needs the counter RNG");
   auto aligned = [&](bool on, uint32_t iv) { return !on || (span % iv == 0 && t_mdl % iv == 0); };
-  if (!aligned(p.enable_cond, p.cond.interval) || !aligned(p.enable_colls, p.coll.interval) ||
-      !aligned(p.enable_motion, p.motion.interval))
-    return fail(c, CLEO_E_ARG, "graph span must be a multiple of every process interval");
-  if (p.enable_cond && p.cond.warm_start) {
-    const int rc = mlp_check_ready(c);
-    if (rc != CLEO_OK) return rc;
+  bool ok = aligned(p.motion_on, p.motion.interval);
+  for (uint32_t i = 0; i < p.n; ++i) ok = ok && aligned(true, process_interval(p.proc[i]));
+  if (!ok) return fail(c, CLEO_E_ARG, "graph span must be a multiple of every process interval");
+  for (uint32_t i = 0; i < p.n; ++i) {
+    if (p.proc[i].kind == CLEO_PROCESS_CONDENSATION && p.proc[i].cond.warm_start) {
+      const int rc = mlp_check_ready(c);
+      if (rc != CLEO_OK) return rc;
+    }
   }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   // The launches a step enqueues depend on the store's state when it starts (which buffer is
@@ -972,11 +1019,12 @@ static int time_stage(cleo_ctx* c, int kern, const cleo_tiles& t, int reps, bool
   HIP_TRY(c, hipMemcpyAsync(saved_nact, c->nact, ngbx * 4ull, hipMemcpyDeviceToDevice, c->stream));
   DevCtl saved_ctl;
   HIP_TRY(c, ctx_copy(c, &saved_ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
-  const cleo_cond_params cp = c->sdm_set ? c->sdm.cond
-                                         : cleo_cond_params{100, 1, 50, 1e-6, 1e-8, 0.001, 0, 10};
-  const cleo_coll_params kp =
-      c->sdm_set ? c->sdm.coll : cleo_coll_params{100, CLEO_PROB_LONG, 1.0, CLEO_ENACT_COALBURE, 5.0, 0.85};
-  const cleo_motion_params mp = c->sdm_set ? c->sdm.motion : cleo_motion_params{100, CLEO_TV_SIMMEL};
+  const cleo_cond_params cp = c->sdm.first_cond() ? *c->sdm.first_cond()
+                                                   : cleo_cond_params{100, 1, 50, 1e-6, 1e-8, 0.001, 0, 10};
+  const cleo_coll_params kp = c->sdm.first_coll()
+                                  ? *c->sdm.first_coll()
+                                  : cleo_coll_params{100, CLEO_PROB_LONG, 1.0, CLEO_ENACT_COALBURE, 5.0, 0.85};
+  const cleo_motion_params mp = c->sdm.motion_on ? c->sdm.motion : cleo_motion_params{100, CLEO_TV_SIMMEL};
   const cleo_tiles tiles0 = c->tiles;
   c->tiles = t;
   int rc = CLEO_OK;

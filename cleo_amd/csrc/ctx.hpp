@@ -97,7 +97,23 @@ struct cleo_ctx {
   uint32_t* big_scratch = nullptr;
   uint64_t big_scratch_n = 0;
   // configuration
-  cleo_sdm_params sdm{};
+  // the SDMMethods configuration: microphysics proc[0] >> ... >> proc[n-1], then motion
+  struct Chain {
+    uint32_t n = 0;
+    cleo_process proc[CLEO_MAX_PROCESSES]{};
+    bool motion_on = false;
+    cleo_motion_params motion{};
+    const cleo_cond_params* first_cond() const {
+      for (uint32_t i = 0; i < n; ++i)
+        if (proc[i].kind == CLEO_PROCESS_CONDENSATION) return &proc[i].cond;
+      return nullptr;
+    }
+    const cleo_coll_params* first_coll() const {
+      for (uint32_t i = 0; i < n; ++i)
+        if (proc[i].kind == CLEO_PROCESS_COLLISIONS) return &proc[i].coll;
+      return nullptr;
+    }
+  } sdm;
   bool sdm_set = false;
   cleo_tiles tiles{1, 1, 1, 0, 0, 0, 0, 0, 0, 0};
   double* gs = nullptr;  // 2*ngbx: per-gridbox saturation ratio and diffusion factor

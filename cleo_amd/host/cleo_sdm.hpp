@@ -17,6 +17,7 @@
 #include <functional>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "cleo_hip.h"
@@ -218,33 +219,42 @@ inline CollisionsProcess CoalBuRe(unsigned int interval, const std::function<dou
   return p;
 }
 
-// cond >> colls (the order the reference examples use, e.g. rainshaft1d main: cond >> colls)
+// CombinedMicrophysicalProcess (microphysicalprocess.hpp:65-120): the processes of a chain in
+// order; operator>> (microphysicalprocess.hpp:132-134) joins any two processes or chains, so any
+// number of processes in any order can be composed (cond >> colls, coal >> bu, cond >> coal >> bu).
+// NullMicrophysicalProcess is the empty chain.
 struct CombinedMicrophysicalProcess {
-  bool has_cond = false, has_colls = false;
-  cleo_cond_params cond{};
-  cleo_coll_params coll{};
+  std::vector<cleo_process> chain;
 };
-inline CombinedMicrophysicalProcess operator>>(const CondensationProcess& a, const CollisionsProcess& b) {
-  CombinedMicrophysicalProcess m;
-  m.has_cond = m.has_colls = true;
-  m.cond = a.params;
-  m.coll = b.params;
-  return m;
-}
 inline CombinedMicrophysicalProcess as_microphysics(const CondensationProcess& a) {
-  CombinedMicrophysicalProcess m;
-  m.has_cond = true;
-  m.cond = a.params;
-  return m;
+  cleo_process q{};
+  q.kind = CLEO_PROCESS_CONDENSATION;
+  q.cond = a.params;
+  return {{q}};
 }
 inline CombinedMicrophysicalProcess as_microphysics(const CollisionsProcess& b) {
-  CombinedMicrophysicalProcess m;
-  m.has_colls = true;
-  m.coll = b.params;
-  return m;
+  cleo_process q{};
+  q.kind = CLEO_PROCESS_COLLISIONS;
+  q.coll = b.params;
+  return {{q}};
 }
 inline CombinedMicrophysicalProcess as_microphysics(const CombinedMicrophysicalProcess& m) { return m; }
 inline CombinedMicrophysicalProcess as_microphysics(const NullMicrophysicalProcess&) { return {}; }
+
+template <typename T>
+inline constexpr bool is_microphysical_process_v =
+    std::is_same_v<T, CondensationProcess> || std::is_same_v<T, CollisionsProcess> ||
+    std::is_same_v<T, CombinedMicrophysicalProcess> || std::is_same_v<T, NullMicrophysicalProcess>;
+
+template <typename A, typename B,
+          typename = std::enable_if_t<is_microphysical_process_v<A> && is_microphysical_process_v<B>>>
+inline CombinedMicrophysicalProcess operator>>(const A& a, const B& b) {
+  CombinedMicrophysicalProcess m = as_microphysics(a);
+  const CombinedMicrophysicalProcess n = as_microphysics(b);
+  m.chain.insert(m.chain.end(), n.chain.begin(), n.chain.end());
+  if (m.chain.size() > CLEO_MAX_PROCESSES) throw std::invalid_argument("more processes than CLEO_MAX_PROCESSES");
+  return m;
+}
 
 // ---- Motion -----------------------------------------------------------------------------------
 struct PredCorrMotion : ConstTstep {
@@ -330,12 +340,11 @@ class SDMMethods {
   SDMMethods(unsigned int couplstep, GpuDomain& domain, const Microphys& microphys, const Motion& motion)
       : couplstep_(couplstep), domain_(domain) {
     const CombinedMicrophysicalProcess m = as_microphysics(microphys);
-    p_.enable_cond = m.has_cond;
-    p_.enable_colls = m.has_colls;
-    p_.cond = m.cond;
-    p_.coll = m.coll;
     set_motion(motion);
-    check(domain_.ctx(), cleo_set_sdm(domain_.ctx(), &p_), "cleo_set_sdm");
+    check(domain_.ctx(),
+          cleo_set_sdm_chain(domain_.ctx(), m.chain.data(), static_cast<uint32_t>(m.chain.size()),
+                             motion_on_ ? &motion_ : nullptr),
+          "cleo_set_sdm_chain");
   }
   unsigned int get_couplstep() const { return couplstep_; }
   unsigned int next_couplstep(unsigned int t_mdl) const { return ((t_mdl / couplstep_) + 1) * couplstep_; }
@@ -352,13 +361,14 @@ class SDMMethods {
 
  private:
   void set_motion(const PredCorrMotion& m) {
-    p_.enable_motion = 1;
-    p_.motion = m.params;
+    motion_on_ = true;
+    motion_ = m.params;
   }
-  void set_motion(const NullMotion&) { p_.enable_motion = 0; }
+  void set_motion(const NullMotion&) { motion_on_ = false; }
   unsigned int couplstep_;
   GpuDomain& domain_;
-  cleo_sdm_params p_{};
+  bool motion_on_ = false;
+  cleo_motion_params motion_{};
 };
 
 // model-step conversions (libs/initialise/timesteps.hpp:34-59)

@@ -124,8 +124,9 @@ enum {
 /* Collision enactments (PairEnactX, collisions.hpp:64-67) */
 enum {
   CLEO_ENACT_COAL = 0,     /* DoCoalescence, coalescence.cpp:39-196 */
-  CLEO_ENACT_BREAKUP = 1,  /* DoBreakup<ConstNFrags>, breakup.hpp:140-268 */
-  CLEO_ENACT_COALBURE = 2  /* DoCoalBuRe<ConstNFrags, ConstCoalBuReFlag>, coalbure.hpp:207-256 */
+  CLEO_ENACT_BREAKUP = 1,  /* DoBreakup<NFragments per nfrags_kind>, breakup.hpp:140-268 */
+  CLEO_ENACT_COALBURE = 2  /* DoCoalBuRe<NFragments per nfrags_kind, CoalBuReFlag per flag_kind>,
+                              coalbure.hpp:207-256 */
 };
 /* Fragment numbers of a breakup (NFragments, breakup_nfrags.hpp:30-91) */
 enum {
@@ -178,8 +179,9 @@ typedef struct {
   int32_t terminalv;     /* CLEO_TV_* */
 } cleo_motion_params;
 
-/* SDMMethods configuration: Condensation >> Collisions microphysics plus motion
-   (libs/runcleo/sdmmethods.hpp:110-333). enable_* = 0 gives the reference's Null* types. */
+/* SDMMethods configuration with the microphysics Condensation >> Collisions plus motion
+   (libs/runcleo/sdmmethods.hpp:110-333). enable_* = 0 gives the reference's Null* types.
+   cleo_set_sdm(p) is cleo_set_sdm_chain with the enabled ones of {cond, coll} in that order. */
 typedef struct {
   int32_t enable_cond;
   int32_t enable_colls;
@@ -188,6 +190,21 @@ typedef struct {
   cleo_coll_params coll;
   cleo_motion_params motion;
 } cleo_sdm_params;
+
+/* One member of a chain of microphysical processes (MicrophysicalProcess,
+   libs/superdrops/microphysicalprocess.hpp:31-53): a ConstTstepMicrophysics of condensation
+   (Condensation, condensation.hpp:141-150) or of collisions (CollCoal / CollBu / CoalBuRe,
+   coalescence.hpp, breakup.hpp, coalbure.hpp), with its own interval in its params. */
+enum {
+  CLEO_PROCESS_CONDENSATION = 0,
+  CLEO_PROCESS_COLLISIONS = 1
+};
+#define CLEO_MAX_PROCESSES 8
+typedef struct {
+  int32_t kind;           /* CLEO_PROCESS_* */
+  cleo_cond_params cond;  /* read when kind is CLEO_PROCESS_CONDENSATION */
+  cleo_coll_params coll;  /* read when kind is CLEO_PROCESS_COLLISIONS */
+} cleo_process;
 
 typedef struct {
   int32_t code;      /* CLEO_ERR_* of the first failure */
@@ -246,6 +263,16 @@ int cleo_set_rng_injected(cleo_ctx* ctx, const uint32_t* shuffle_j, uint64_t nsh
 
 /* ---- hot path ----------------------------------------------------------------------- */
 int cleo_set_sdm(cleo_ctx* ctx, const cleo_sdm_params* params);
+/* SDMMethods with the microphysics procs[0] >> procs[1] >> ... >> procs[nprocs-1]
+   (CombinedMicrophysicalProcess and operator>>, microphysicalprocess.hpp:65-134: on each
+   sub-step every process that is on step runs, in chain order, on every gridbox; the next
+   sub-step is the earliest next step of any process) and motion (NULL: NullMotion).  Any order
+   and any number of processes up to CLEO_MAX_PROCESSES; nprocs = 0 is NullMicrophysicalProcess.
+   Each collision process is its own collision event: it draws from its own counter-RNG event
+   (the events advance in chain order), and an injected stream covers one event, so an injected
+   chain is run process by process with cleo_collisions. */
+int cleo_set_sdm_chain(cleo_ctx* ctx, const cleo_process* procs, uint32_t nprocs,
+                       const cleo_motion_params* motion);
 /* SortSupersBySdgbxindex + set_gridboxes_refs (sortsupers.hpp:93-226, movesupersindomain.hpp:170-175) */
 int cleo_sort(cleo_ctx* ctx);
 /* MoveSupersInDomain::run_step body with NullBoundaryConditions (movesupersindomain.hpp:234-301) */
@@ -293,6 +320,14 @@ int cleo_download_guess(cleo_ctx* ctx, double* z, uint64_t n);
    host arrays of `out` (except sdid, filled too) must hold `cap` >= n entries.  The drops go to
    the GPU with cleo_upload_supers. */
 int cleo_read_initsupers(const char* path, cleo_supers_soa* out, uint64_t cap, uint64_t* n);
+/* InitSupersFromBinary::fetch_data on one process (init_supers_from_binary.hpp:66-95,
+   init_supers_from_binary.cpp:44-63): the file's drops (as cleo_read_initsupers), then
+   maxnsupers - n uninitialised drops, out of the domain (CLEO_OOB_GBXINDEX) with NaN radius, solute
+   mass and coordinates and xi = sdId = 0, so that the host arrays of `out` (maxnsupers entries each)
+   hold exactly maxnsupers drops, as the reference's totsupers does.  *ninit = n.  CLEO_E_ARG when
+   the file holds more than maxnsupers drops.  Upload the result with cleo_upload_supers(ctx, out,
+   maxnsupers). */
+int cleo_read_initsupers_padded(const char* path, uint64_t maxnsupers, cleo_supers_soa* out, uint64_t* ninit);
 /* Gridbox boundaries (gbx_bounds_from_binary.cpp:29-58): variable 0 ndims (3 uint64: nz, nx, ny),
    1 gridbox indices (uint32), 2 six bounds per gridbox [zlow, zup, xlow, xup, ylow, yup]
    (double, dimensionless).  Writes ndims and, when the arrays are non-null, the face coordinates

@@ -17,12 +17,16 @@
  * Storage mirrors the reference: an array of 72-byte Superdrop records (superdrop.hpp:36-60)
  * with per-gridbox refs (supersingbx.hpp:32-102).
  *
- * Parity status of each part (what pins it is in tests/test_oracle_*.py):
+ * Parity status of each part (what pins it: tests/test_oracle.py, tests/oracle_pins.py,
+ * tests/test_collision_kinetics.py, tests/test_observers.py, tests/test_coupling.py):
  *   pinned by closed forms / invariants / brute force: units and constants, Newton solve,
- *   thermodynamic feedback, Golovin/Long/Const probabilities, Simmel terminal velocity,
- *   coalescence / breakup / coalbure enactment, Fisher-Yates with injected indices,
- *   stable counting sort and refs, predictor-corrector motion and gridbox update,
- *   Philox4x32-10 (published known-answer vectors).
+ *   Golovin/Long/Const/Low-List probabilities, Simmel terminal velocity, coalescence / breakup /
+ *   coalbure enactment, Fisher-Yates with injected indices, stable counting sort and refs,
+ *   predictor-corrector motion and gridbox update, Philox4x32-10 (published known-answer vectors);
+ *   pinned by SI evaluations of the published equations (each constant perturbed by 1e-6 fails
+ *   one, tests/test_oracle_mutations.py): saturation pressure, saturation ratio, diffusion
+ *   factors, ventilation factor, Koehler factors, Rogers-Yau and Rogers-GK terminal velocities,
+ *   dry-air density (through the qcond recompute) and the State update of condensation.
  *   Parity unpinned: Kokkos Random_XorShift64_Pool streams (replaced by the injected /
  *   Philox streams on both sides) and Kokkos sort_team's permutation of a gridbox after
  *   nulls appear (restated as a stable partition).
@@ -676,7 +680,9 @@ struct Ctx {
   uint64_t seed = 0, event = 0;
   std::vector<uint32_t> inj_j;
   std::vector<double> inj_phi;
-  cleo_sdm_params sdm{};
+  std::vector<cleo_process> procs; /* microphysics procs[0] >> procs[1] >> ... */
+  bool motion_on = false;
+  cleo_motion_params motion{};
   bool sdm_set = false;
   ErrWord err;
   cleo_stats stats{};
@@ -1029,28 +1035,36 @@ static bool injected_offsets(const Ctx& c, std::vector<uint64_t>& oj, std::vecto
   return oj[c.ngbx] == c.inj_j.size() && op[c.ngbx] == c.inj_phi.size();
 }
 
-/* one microphysics pass over [t_sdm, t_next) for every gridbox (sdmmethods.hpp:70-92,184-230) */
-static int microphysics(Ctx& c, uint32_t t_sdm, uint32_t t_next, bool cond_on, const cleo_cond_params& cp,
-                        bool coll_on, const cleo_coll_params& kp) {
+static uint32_t interval_of(const cleo_process& q) {
+  return q.kind == CLEO_PROCESS_CONDENSATION ? q.cond.interval : q.coll.interval;
+}
+
+/* one microphysics pass over [t_sdm, t_next) for every gridbox (sdmmethods.hpp:70-92,184-230) of
+   the chain procs[0] >> procs[1] >> ... (CombinedMicrophysicalProcess, microphysicalprocess.hpp:
+   65-120): next_step is the minimum over the chain, on each sub-step the processes on step run in
+   chain order */
+static int microphysics(Ctx& c, uint32_t t_sdm, uint32_t t_next, const std::vector<cleo_process>& procs) {
   auto next_step = [&](uint32_t subt) -> uint32_t {
-    uint32_t a = 0xFFFFFFFFu, b = 0xFFFFFFFFu;
-    if (cond_on) a = ((subt / cp.interval) + 1) * cp.interval;
-    if (coll_on) b = ((subt / kp.interval) + 1) * kp.interval;
-    return std::min(a, b);
+    uint32_t t = 0xFFFFFFFFu;
+    for (const cleo_process& q : procs) t = std::min(t, ((subt / interval_of(q)) + 1) * interval_of(q));
+    return t;
   };
-  auto on_cond = [&](uint32_t subt) { return cond_on && (subt % cp.interval == 0) && cp.interval != 0xFFFFFFFFu; };
-  auto on_coll = [&](uint32_t subt) { return coll_on && (subt % kp.interval == 0) && kp.interval != 0xFFFFFFFFu; };
-  /* collision events of this pass, in sub-step order; the same for every gridbox */
-  std::vector<uint32_t> subts;
-  for (uint32_t subt = t_sdm; subt < t_next; subt = next_step(subt)) subts.push_back(subt);
+  auto on_step = [&](const cleo_process& q, uint32_t subt) {
+    return (subt % interval_of(q) == 0) && interval_of(q) != 0xFFFFFFFFu;
+  };
+  /* (sub-step, process) pairs of this pass in order; the same for every gridbox */
+  std::vector<std::pair<uint32_t, size_t>> runs;
+  for (uint32_t subt = t_sdm; subt < t_next; subt = next_step(subt))
+    for (size_t i = 0; i < procs.size(); ++i)
+      if (on_step(procs[i], subt)) runs.emplace_back(subt, i);
   uint64_t nevents = 0;
-  for (uint32_t subt : subts) nevents += on_coll(subt) ? 1 : 0;
+  for (const auto& r : runs) nevents += procs[r.second].kind == CLEO_PROCESS_COLLISIONS ? 1 : 0;
   std::vector<uint64_t> oj, op;
-  if (coll_on && nevents > 0 && c.rng_mode == 2) {
+  if (nevents > 0 && c.rng_mode == 2) {
     if (nevents != 1) return fail(&c, CLEO_E_STATE, "injected RNG covers exactly one collision event");
     if (!injected_offsets(c, oj, op)) return fail(&c, CLEO_E_ARG, "injected stream size mismatch");
   }
-  if (coll_on && nevents > 0 && c.rng_mode == 0) return fail(&c, CLEO_E_STATE, "no RNG set");
+  if (nevents > 0 && c.rng_mode == 0) return fail(&c, CLEO_E_STATE, "no RNG set");
   uint64_t iters = 0, subs = 0, restarts = 0, ndrops = 0;
   CollCounters cc;
   bool any_change = false;
@@ -1061,15 +1075,15 @@ static int microphysics(Ctx& c, uint32_t t_sdm, uint32_t t_next, bool cond_on, c
     const uint32_t g = static_cast<uint32_t>(gi);
     uint64_t ev = 0;
     CollCounters lc;
-    for (uint32_t subt : subts) {
-      if (on_cond(subt)) {
+    for (const auto& r : runs) {
+      const cleo_process& q = procs[r.second];
+      if (q.kind == CLEO_PROCESS_CONDENSATION) {
         ndrops += c.nact[g];
-        cond_gbx(c, g, cp, iters, subs, restarts);
-      }
-      if (on_coll(subt)) {
+        cond_gbx(c, g, q.cond, iters, subs, restarts);
+      } else {
         const uint64_t j0 = oj.empty() ? 0 : oj[g];
         const uint64_t p0 = op.empty() ? 0 : op[g];
-        any_change = coll_gbx(c, g, kp, c.event + ev, j0, p0, lc) || any_change;
+        any_change = coll_gbx(c, g, q.coll, c.event + ev, j0, p0, lc) || any_change;
         ++ev;
       }
     }
@@ -1293,11 +1307,33 @@ int orc_set_cond_guess(orc_ctx* h, const double* z, uint64_t n) {
   else c.guess.clear();
   return CLEO_OK;
 }
-int orc_set_sdm(orc_ctx* h, const cleo_sdm_params* p) {
+int orc_set_sdm_chain(orc_ctx* h, const cleo_process* procs, uint32_t n, const cleo_motion_params* motion) {
   Ctx& c = *reinterpret_cast<Ctx*>(h);
-  c.sdm = *p;
+  if (n > CLEO_MAX_PROCESSES || (n && !procs)) return fail(&c, CLEO_E_ARG, "process list");
+  for (uint32_t i = 0; i < n; ++i) {
+    if (procs[i].kind != CLEO_PROCESS_CONDENSATION && procs[i].kind != CLEO_PROCESS_COLLISIONS)
+      return fail(&c, CLEO_E_ARG, "unknown process kind");
+    if (interval_of(procs[i]) == 0) return fail(&c, CLEO_E_ARG, "zero interval");
+  }
+  if (motion && motion->interval == 0) return fail(&c, CLEO_E_ARG, "zero interval");
+  c.procs.assign(procs, procs + n);
+  c.motion_on = motion != nullptr;
+  if (motion) c.motion = *motion;
   c.sdm_set = true;
   return CLEO_OK;
+}
+int orc_set_sdm(orc_ctx* h, const cleo_sdm_params* p) {
+  cleo_process q[2]{};
+  uint32_t n = 0;
+  if (p->enable_cond) {
+    q[n].kind = CLEO_PROCESS_CONDENSATION;
+    q[n++].cond = p->cond;
+  }
+  if (p->enable_colls) {
+    q[n].kind = CLEO_PROCESS_COLLISIONS;
+    q[n++].coll = p->coll;
+  }
+  return orc_set_sdm_chain(h, q, n, p->enable_motion ? &p->motion : nullptr);
 }
 int orc_sort(orc_ctx* h) {
   Ctx& c = *reinterpret_cast<Ctx*>(h);
@@ -1317,30 +1353,30 @@ int orc_condensation(orc_ctx* h, const cleo_cond_params* p) {
     return CLEO_E_DEVICE;
   }
   if (c.rec_niters) c.niters.assign(c.sup.size(), 0);
-  cleo_coll_params none{};
-  none.interval = 1;
-  return microphysics(c, 0, 1, true, *p, false, none);
+  cleo_process q{};
+  q.kind = CLEO_PROCESS_CONDENSATION;
+  q.cond = *p;
+  return microphysics(c, 0, 1, {q});
 }
 int orc_collisions(orc_ctx* h, const cleo_coll_params* p) {
   Ctx& c = *reinterpret_cast<Ctx*>(h);
   if (p->interval == 0) return fail(&c, CLEO_E_ARG, "interval 0");
-  cleo_cond_params none{};
-  none.interval = 1;
-  return microphysics(c, 0, 1, false, none, true, *p);
+  cleo_process q{};
+  q.kind = CLEO_PROCESS_COLLISIONS;
+  q.coll = *p;
+  return microphysics(c, 0, 1, {q});
 }
 int orc_run_step(orc_ctx* h, uint32_t t_mdl, uint32_t t_mdl_next) {
   Ctx& c = *reinterpret_cast<Ctx*>(h);
   if (!c.sdm_set) return fail(&c, CLEO_E_STATE, "cleo_set_sdm not called");
-  const cleo_sdm_params& p = c.sdm;
   if (c.rec_niters) c.niters.assign(c.sup.size(), 0);
   uint32_t t_sdm = t_mdl;
   while (t_sdm < t_mdl_next) {
     uint32_t next_move = 0xFFFFFFFFu;
-    if (p.enable_motion) next_move = ((t_sdm / p.motion.interval) + 1) * p.motion.interval;
+    if (c.motion_on) next_move = ((t_sdm / c.motion.interval) + 1) * c.motion.interval;
     const uint32_t t_sdm_next = std::min(t_mdl_next, next_move);
-    if (p.enable_motion && (t_sdm % p.motion.interval == 0)) do_motion(c, p.motion);
-    const int rc = microphysics(c, t_sdm, t_sdm_next, p.enable_cond != 0, p.cond, p.enable_colls != 0,
-                                p.coll);
+    if (c.motion_on && (t_sdm % c.motion.interval == 0)) do_motion(c, c.motion);
+    const int rc = microphysics(c, t_sdm, t_sdm_next, c.procs);
     if (rc != CLEO_OK) return rc;
     t_sdm = t_sdm_next;
   }
@@ -1391,6 +1427,16 @@ double orc_diffusion_factor(double press, double temp, double psat) {
 double orc_ventilation_factor(double r) { return ventilation_factor(r); }
 double orc_supersaturation_ratio(double press, double qvap, double psat) {
   return supersaturation_ratio(press, qvap, psat);
+}
+/* Koehler a and b of one drop of solute mass msol (thermodynamic_equations.hpp:92-107) */
+void orc_kohler_factors(double temp, double msol, double rho_sol, double mr_sol, double ionic, double* ab) {
+  Superdrop d{};
+  d.attrs.msol = msol;
+  Solute s;
+  s.rho_sol = rho_sol;
+  s.mr_sol = mr_sol;
+  s.ionic = ionic;
+  kohler_factors(d, temp, s, ab[0], ab[1]);
 }
 /* one drop through solve_condensation; returns new radius, writes iteration count */
 double orc_solve_condensation(double s_ratio, double akoh, double bkoh, double ffactor, double rprev,

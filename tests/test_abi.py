@@ -30,8 +30,8 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     # sizes of the POD structs as gcc lays out include/cleo_hip.h on x86-64
     sizes = [C.sizeof(x) for x in (abi.DomainDesc, abi.SupersSoA, abi.GbxState, abi.CollParams, abi.CondParams,
-                                   abi.SDMParams, abi.Error, abi.Stats)]
-    assert sizes == [56, 64, 88, 64, 48, 136, 16, 72]
+                                   abi.SDMParams, abi.Error, abi.Stats, abi.Process)]
+    assert sizes == [56, 64, 88, 64, 48, 136, 16, 72, 120]
 
 
 def test_oracle_exports_mirror(oracle):

@@ -1,11 +1,15 @@
 """CLEO's binary initialisation files (initialise/readbinary.cpp:22-99, init_all_supers_from_binary.cpp:
-57-100, gbx_bounds_from_binary.cpp:29-58) read by cleo_read_initsupers / cleo_read_gridfile.
+57-100, init_supers_from_binary.cpp:44-63, gbx_bounds_from_binary.cpp:29-58) read by
+cleo_read_initsupers, cleo_read_initsupers_padded and cleo_read_gridfile.
 
-The files are written here by `write_cleo_binary`, a restatement of the layout cleopy/writebinary.py
-packs (little-endian, unpadded: 4 uint32, the global metadata string, per variable 3 uint32 + 2
-chars + 1 double, then the data); the reference holds no binary files of its own.  The readers
-are host code, so these tests need no GPU, except the last, which uploads what was read and
-checks the store against a direct upload."""
+The fixtures tests/golden/*.dat were written by the reference's own writers (cleopy's
+writebinary.writebinary and create_gbxboundaries.write_gridboxboundaries_binary, run by
+tests/golden/make_binfiles.py), with what was written kept in tests/golden/binfiles_expected.npz;
+they pin the readers bit for bit.  Other cases are written here by `write_cleo_binary`, a
+restatement of the layout cleopy/writebinary.py packs (little-endian, unpadded: 4 uint32, the
+global metadata string, per variable 3 uint32 + 2 chars + 1 double, then the data), checked against
+the fixtures.  The readers are host code, so these tests need no GPU, except the last, which uploads
+a padded store read from a fixture to the GPU and to the oracle and compares a step of both."""
 import ctypes as C
 import os
 import struct
@@ -15,7 +19,8 @@ import pytest
 
 from cleo_amd import abi
 from cleo_amd import populations as P
-from tests.helpers import SEED, make_domain
+from tests.helpers import (SEED, assert_state_match, assert_supers_match, coll_params, cond_params, make_domain,
+                           sdm_params)
 
 _FMT = {np.dtype(np.uint32): "I", np.dtype(np.uint64): "Q", np.dtype(np.float64): "d"}
 
@@ -68,6 +73,90 @@ def read_gridfile(so, path):
     halves = [np.zeros(int(n) + 1) for n in nd]
     rc = so.cleo_read_gridfile(path.encode(), nd, *(h.ctypes.data_as(C.c_void_p) for h in halves))
     return rc, (tuple(int(n) for n in nd), halves)
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+INIT_KEYS = ("sdgbxindex", "xi", "radius", "msol", "coord3", "coord1", "coord2")
+
+
+def _expected():
+    return np.load(os.path.join(GOLDEN, "binfiles_expected.npz"))
+
+
+def read_initsupers_padded(so, path, maxn):
+    arrs = abi.empty_supers(maxn)
+    s = abi.soa_view(arrs)
+    n = C.c_uint64()
+    so.cleo_read_initsupers_padded.argtypes = [C.c_char_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_uint64)]
+    rc = so.cleo_read_initsupers_padded(path.encode(), maxn, C.byref(s), C.byref(n))
+    return rc, arrs, n.value
+
+
+@pytest.mark.parametrize("name", ["grid_uneven3d", "grid_delta3d", "grid_box0d"])
+def test_reference_written_gridfiles(name):
+    """Faces read from the reference writer's file reproduce every gridbox's six bounds and index
+    as the reference computed them (dimless_gridboxboundaries), bit for bit."""
+    so = _lib()
+    e = _expected()
+    rc, (nd, halves) = read_gridfile(so, os.path.join(GOLDEN, name + ".dat"))
+    assert rc == abi.CLEO_OK
+    assert nd == tuple(int(v) for v in e[name + "_ndims"])
+    zh, xh, yh = halves
+    nz, nx, ny = nd
+    got = []
+    for g in range(nz * nx * ny):
+        k, i, j = g % nz, (g // nz) % nx, g // (nz * nx)
+        got += [zh[k], zh[k + 1], xh[i], xh[i + 1], yh[j], yh[j + 1]]
+    assert np.array_equal(np.asarray(got), e[name + "_bounds"])
+    assert np.array_equal(np.arange(nz * nx * ny, dtype=np.uint32), e[name + "_idxs"])
+
+
+@pytest.mark.parametrize("name", ["initsupers3d", "initsupers0d"])
+def test_reference_written_initsupers(name, tmp_path):
+    so = _lib()
+    e = _expected()
+    path = os.path.join(GOLDEN, name + ".dat")
+    rc, got = read_initsupers(so, path)
+    assert rc == abi.CLEO_OK
+    n = len(e[f"{name}_xi"])
+    for k in INIT_KEYS:
+        want = e[f"{name}_{k}"]
+        if k.startswith("coord") and len(want) == 0:
+            assert not got[k].any(), k  # a 0-D file: coordinates read as 0
+        else:
+            assert np.array_equal(got[k], want), k
+    assert np.array_equal(got["sdid"], np.arange(n, dtype=np.uint64))
+    # the restated writer produces the reference writer's bytes for the same variables and the same
+    # global metadata string
+    ref = open(path, "rb").read()
+    nchars = struct.unpack_from("<I", ref, 4)[0]
+    mine = str(tmp_path / "mine.dat")
+    write_cleo_binary(mine, [e[f"{name}_{k}"] for k in INIT_KEYS], units=[b" ", b" ", b"m", b"g", b"m", b"m", b"m"],
+                      scales=[1.0, 1.0, P.R0, P.MASS0, 1000.0, 1000.0, 1000.0],
+                      metastr=ref[16:16 + nchars].decode())
+    assert open(mine, "rb").read() == ref
+
+
+def test_padded_initsupers():
+    """The file's drops, then maxnsupers - n drops out of the domain with NaN attributes and
+    coordinates and xi = sdId = 0 (init_supers_from_binary.cpp:44-63); more drops than maxnsupers is
+    an error (init_supers_from_binary.hpp:75-79)."""
+    so = _lib()
+    e = _expected()
+    path = os.path.join(GOLDEN, "initsupers3d.dat")
+    n = len(e["initsupers3d_xi"])
+    rc, got, ninit = read_initsupers_padded(so, path, n + 5)
+    assert rc == abi.CLEO_OK and ninit == n
+    for k in INIT_KEYS:
+        assert np.array_equal(got[k][:n], e[f"initsupers3d_{k}"]), k
+    assert np.all(got["sdgbxindex"][n:] == abi.OOB)
+    assert np.all(got["xi"][n:] == 0) and np.all(got["sdid"][n:] == 0)
+    for k in ("radius", "msol", "coord3", "coord1", "coord2"):
+        assert np.all(np.isnan(got[k][n:])), k
+    rc, _, _ = read_initsupers_padded(so, path, n)
+    assert rc == abi.CLEO_OK
+    rc, _, _ = read_initsupers_padded(so, path, n - 1)
+    assert rc == abi.CLEO_E_ARG
 
 
 def _initsupers_arrays(pop):
@@ -162,20 +251,40 @@ def test_gridfile_rejects_non_cartesian_bounds(tmp_path):
     assert rc == abi.CLEO_E_FILE
 
 
-@pytest.mark.gpu
-def test_gpu_upload_from_binary_files(gpu, tmp_path):
-    print("case: upload from CLEO binary files", flush=True)
+def test_gridfile_rejects_wrapping_gridbox_count(tmp_path):
+    """ndims whose product wraps a 64-bit integer (2^22 * 2^22 * 2^20 = 2^64) are not a layout."""
     so = _lib()
-    pop = P.coupled(SEED, ndims=(6, 5, 4), mean=30)
-    ipath, gpath = str(tmp_path / "initsupers.dat"), str(tmp_path / "grid.dat")
-    write_cleo_binary(ipath, _initsupers_arrays(pop))
-    write_cleo_binary(gpath, _gridfile_arrays(pop.ndims, pop.halves))
-    rc, sup = read_initsupers(so, ipath)
+    path = str(tmp_path / "grid.dat")
+    write_cleo_binary(path, [np.asarray([2 ** 22, 2 ** 22, 2 ** 20], np.uint64), np.zeros(0, np.uint32),
+                             np.zeros(0, np.float64)])
+    rc, _ = read_gridfile(so, path)
+    assert rc == abi.CLEO_E_FILE
+
+
+@pytest.mark.gpu
+def test_gpu_upload_from_binary_files(gpu, oracle):
+    """A padded store read from the reference-written fixtures goes to the GPU and to the oracle;
+    both hold the same store and refs, and one condensation + collision step of both agrees."""
+    print("case: padded store from the reference-written CLEO binary files, GPU against the oracle", flush=True)
+    so = _lib()
+    rc, (nd, halves) = read_gridfile(so, os.path.join(GOLDEN, "grid_uneven3d.dat"))
     assert rc == abi.CLEO_OK
-    rc, (nd, halves) = read_gridfile(so, gpath)
-    assert rc == abi.CLEO_OK
-    from_files = P.Population(nd, halves, sup, pop.state, pop.solute, periodic=pop.periodic)
-    a = make_domain(gpu, from_files).download()
-    b = make_domain(gpu, pop).download()
-    for k, _ in abi.SUPER_FIELDS:
-        assert np.array_equal(a[k], b[k]), k
+    n = len(_expected()["initsupers3d_xi"])
+    rc, sup, ninit = read_initsupers_padded(so, os.path.join(GOLDEN, "initsupers3d.dat"), n + 40)
+    assert rc == abi.CLEO_OK and ninit == n
+    ngbx = int(np.prod(nd))
+    state = P.condensation(SEED, ndims=nd, mean=2).state
+    zh, xh, yh = halves
+    gi = np.arange(ngbx)
+    k, i, j = gi % nd[0], (gi // nd[0]) % nd[1], gi // (nd[0] * nd[1])
+    state["volume"][:] = (zh[k + 1] - zh[k]) * (xh[i + 1] - xh[i]) * (yh[j + 1] - yh[j])
+    pop = P.Population(nd, halves, sup, state, P.NACL)
+    g, o = make_domain(gpu, pop), make_domain(oracle, pop)
+    assert_supers_match(g.download(), o.download(), exact_fp=True, what="padded upload")
+    assert np.array_equal(g.refs(), o.refs())
+    for d in (g, o):
+        d.rng_counter(9)
+        d.set_sdm(sdm_params(cond=cond_params(), coll=coll_params()))
+        assert d.run_step(0, 100) == 0, d.error()
+    assert_supers_match(g.download(), o.download(), what="padded step")
+    assert_state_match(g.state(), o.state(), what="padded step")

@@ -10,7 +10,7 @@ import pytest
 from cleo_amd import abi
 from cleo_amd import populations as P
 from tests.helpers import (SEED, assert_state_match, assert_supers_match, coll_params, cond_params,
-                           injected_stream, make_domain, motion_params, sdm_params)
+                           injected_stream, make_domain, motion_params, rain_population, sdm_params)
 
 pytestmark = pytest.mark.gpu
 
@@ -101,14 +101,7 @@ KINETIC_CASES = {
 }
 
 
-def _rain(seed, ndims=(4, 4, 2), mean=128):
-    """The coalescence/breakup population with radii log-uniform in 20 um .. 2.5 mm, so that the
-    kinetic energies fall on both sides of every flag's boundaries."""
-    pop = P.coal_breakup(seed, ndims=ndims, mean=mean)
-    rng = np.random.default_rng(seed + 1)
-    wet = np.exp(rng.uniform(np.log(20.0), np.log(2500.0), pop.n))
-    pop.supers["radius"] = np.maximum(wet, pop.supers["radius"])
-    return pop
+_rain = rain_population
 
 
 @pytest.mark.parametrize("tiled", [0, 1])
@@ -333,6 +326,10 @@ def test_set_sdm_rejects_unknown_collision_kinds(gpu):
     d = make_domain(gpu, pop)
     s = sdm_params(coll=coll_params(flag_kind=11))
     assert gpu.lib.cleo_set_sdm(d.h, abi.ctypes_byref(s)) == abi.CLEO_E_ARG
+    # an unknown terminal velocity of motion is rejected too, by cleo_set_sdm and cleo_motion
+    s = sdm_params(motion=motion_params(tv=9))
+    assert gpu.lib.cleo_set_sdm(d.h, abi.ctypes_byref(s)) == abi.CLEO_E_ARG
+    assert d.motion(motion_params(tv=-2)) == abi.CLEO_E_ARG
 
 
 # Collisions leave the drops in place and record their shuffled order; with motion every second
