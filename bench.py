@@ -22,6 +22,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baseline's OpenMP threads: one per core, packed (set before any OpenMP runtime starts)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 METRIC = "superdroplet-steps/sec on 1 GPU (plain vs MLP warm-start), Newton iters/drop"
 HBM_PEAK_GBS = 8000.0
@@ -77,6 +80,21 @@ def stage_kernel(stage, tiles):
     return {0: "k_motion", 1: "k_motion_flat", 2: "k_motion_scatter", 3: "k_motion_keys"}[tiles.motion_flat]
 
 
+def host_cpu():
+    """CPU model and the cores this process may use (for the CPU baseline's record)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(model=model, cpus_available=len(os.sched_getaffinity(0)), omp_proc_bind=os.environ.get("OMP_PROC_BIND"),
+                omp_places=os.environ.get("OMP_PLACES"), omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +108,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--tiles", type=str, default=None,
                     help="launch tiling as 8 comma-separated cleo_tiles fields (skips the autotuner)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the coalescence/breakup, condensation and learned-condensation workloads")
+    ap.add_argument("--config-steps", type=int, default=10)
     ap.add_argument("--copy-probe", action="store_true",
                     help="negate one 1 GiB device buffer first (reference point for byte counters)")
     return ap.parse_args()
@@ -305,6 +326,181 @@ def learned_leg(args, b, d, sdm, t, timed_steps, torch):
     return res
 
 
+# ------------------------------------------------------------------------------------------------
+# the other workloads: collisions alone, condensation alone, condensation with the learned start
+
+
+def _device_domain(torch, b, arrs, st, halves, ndims, solute, seed=20261202):
+    from cleo_amd import abi
+    from cleo_amd import populations as P
+    L = b.lib
+    n = int(arrs["xi"].numel())
+    soa = abi.SupersSoA()
+    soa.sdgbxindex = _ptr(arrs["sdgbxindex"], C.c_uint32)
+    soa.coord3, soa.coord1, soa.coord2 = (_ptr(arrs[k], C.c_double) for k in ("coord3", "coord1", "coord2"))
+    soa.xi = _ptr(arrs["xi"], C.c_uint64)
+    soa.radius, soa.msol = _ptr(arrs["radius"], C.c_double), _ptr(arrs["msol"], C.c_double)
+    soa.sdid = _ptr(arrs["sdid"], C.c_uint64)
+    d = abi.Domain(b, n, ndims, *halves)
+    d.check(L.cleo_upload_supers_device(d.h, C.byref(soa), n), "upload_supers_device")
+    d.ntot = n
+    d.set_state(st)
+    d.set_solute(*P.solute_dimless(solute))
+    d.rng_counter(seed)
+    return d, n
+
+
+def _graph_steps(d, L, torch, t0, k, span=100):
+    """k steps replayed as a hipGraph, HIP events on the library's stream: (ms per step, stats)."""
+    d.reset_stats()
+    d.check(L.cleo_synchronize(d.h), "sync")
+    L.cleo_event_record(d.h, 8)
+    d.check(L.cleo_run_steps_graph(d.h, t0, span, k), "graph steps")
+    L.cleo_event_record(d.h, 9)
+    d.check(L.cleo_synchronize(d.h), "sync")
+    ms = C.c_float()
+    d.check(L.cleo_event_elapsed(d.h, 8, 9, C.byref(ms)), "event_elapsed")
+    return float(ms.value) / k, d.stats()
+
+
+def _host_arrays(arrs):
+    """device-generator output (torch, on the CPU) as the numpy SoA the oracle uploads"""
+    out = {}
+    for k, v in arrs.items():
+        a = v.numpy()
+        out[k] = a.view(np.uint32) if k == "sdgbxindex" else (a.view(np.uint64) if k in ("xi", "sdid") else a)
+    return out
+
+
+def _cpu_leg(torch, gen, ndims, solute, procs, steps, warm=1, **kw):
+    """OpenMP oracle on a bounded sample of a workload: superdroplet-steps/s and the sample text."""
+    from cleo_amd import abi
+    from cleo_amd import populations as P
+    from oracle import pyoracle
+    b = pyoracle.load("omp")
+    arrs, st, halves = gen(torch, "cpu", ndims=ndims, **kw)
+    sup = _host_arrays(arrs)
+    d = abi.Domain(b, len(sup["xi"]), ndims, *halves)
+    d.upload(sup)
+    d.set_state(st)
+    d.set_solute(*P.solute_dimless(solute))
+    d.rng_counter(20261202)
+    assert d.set_sdm_chain(procs) == 0
+    for s_ in range(warm):
+        d.run_step(100 * s_, 100 * (s_ + 1))
+    d.reset_stats()
+    t0 = time.perf_counter()
+    for s_ in range(warm, warm + steps):
+        rc = d.run_step(100 * s_, 100 * (s_ + 1))
+        if rc != 0:
+            raise RuntimeError(f"oracle run_step rc={rc}")
+    dt = time.perf_counter() - t0
+    n = len(sup["xi"])
+    st_ = d.stats()
+    d.close()
+    return dict(value=n * steps / dt, unit="superdroplet-steps/s", cores=int(b.lib.orc_num_threads()), kind="port",
+                sample=f"{ndims[0]}x{ndims[1]}x{ndims[2]} gridboxes ({n} drops), {steps} steps after {warm} "
+                       f"warm-up, {dt:.1f} s, OpenMP oracle"), st_
+
+
+def run_configs(args):
+    import torch
+
+    from cleo_amd import abi, lib
+    from cleo_amd import populations as P
+
+    b = lib.load()
+    L = b.lib
+    dev = torch.device("cuda:0")
+    k = args.config_steps
+    out = {}
+
+    # collision-coalescence + breakup alone: 64^3 gridboxes of 50 m, Poisson(128) drops, Long kernel,
+    # DoCoalBuRe<ConstNFrags(5), ConstCoalBuReFlag(0.85)>, dt = 1 s
+    kp = abi.CollParams(100, abi.PROB_LONG, 1.0, abi.ENACT_COALBURE, 5.0, 0.85)
+    ndims = (64, 64, 64)
+    arrs, st, halves = P.coal_breakup_device(torch, dev, ndims=ndims)
+    d, n = _device_domain(torch, b, arrs, st, halves, ndims, P.NACL)
+    del arrs
+    torch.cuda.empty_cache()
+    assert d.set_sdm_chain([abi.coll_process(kp)]) == 0
+    d.check(L.cleo_run_steps_graph(d.h, 0, 100, 2), "warmup")
+    ms, stats = _graph_steps(d, L, torch, 200, k)
+    d.close()
+    cpu, _ = _cpu_leg(torch, P.coal_breakup_device, (64, 64, 16), P.NACL, [abi.coll_process(kp)], 10)
+    val = n * 1e3 / ms
+    out["coal_breakup_64cubed"] = dict(
+        workload="64x64x64 gridboxes of 50 m, Poisson(128) drops, Long kernel, CoalBuRe(nfrags 5, coaleff 0.85), "
+                 "counter RNG, hipGraph", ndrops=n, ms_per_step=ms, value=val, unit="superdroplet-steps/s",
+        coll_events_per_step=stats["coll_events"] / k,
+        hbm_model_bytes_per_drop=48, hbm_model_GBs=48 * n / (ms / 1e3) / 1e9, cpu_baseline=cpu,
+        gpu_over_cpu=val / cpu["value"])
+
+    # condensation alone: 64^3 gridboxes, Poisson(122), ammonium sulphate at Koehler equilibrium, per
+    # gridbox T, p, S; FP64 Newton rtol 1e-6, atol 1e-8, maxniters 50, thermodynamic feedback
+    cp = abi.CondParams(100, 1, 50, 1e-6, 1e-8, 0.001, 0, 10)
+    arrs, st, halves = P.condensation_device(torch, dev, ndims=ndims)
+    d, n = _device_domain(torch, b, arrs, st, halves, ndims, P.AMMONIUM_SULPHATE)
+    dl, _ = _device_domain(torch, b, arrs, st, halves, ndims, P.AMMONIUM_SULPHATE)  # twin for the learned start
+    del arrs
+    torch.cuda.empty_cache()
+    for x in (d, dl):
+        assert x.set_sdm_chain([abi.cond_process(cp)]) == 0
+        x.check(L.cleo_run_steps_graph(x.h, 0, 100, 2), "warmup")
+    tiles = abi.Tiles()
+    d.check(L.cleo_autotune(d.h, C.byref(tiles)), "autotune")
+    L.cleo_set_tiles(dl.h, C.byref(tiles))
+    ms, stats = _graph_steps(d, L, torch, 200, k)
+    iters = stats["newton_iters"] / max(stats["cond_drops"], 1)
+    cpu, cst = _cpu_leg(torch, P.condensation_device, (64, 64, 8), P.AMMONIUM_SULPHATE, [abi.cond_process(cp)], 3)
+    # FP64 operation count of one drop's solve: the setup (saturation ratio, diffusion and ventilation
+    # factors, Koehler factors, critical radius and timestep) and each Newton update (residual and
+    # derivative), with a pow / exp / sqrt counted as 20 operations
+    flop = 400 + 220 * iters
+    val = n * 1e3 / ms
+    out["condensation_64cubed"] = dict(
+        workload="64x64x64 gridboxes of 50 m, Poisson(122) drops, ammonium sulphate at Koehler equilibrium, "
+                 "T 270-300 K, p 7e4-1e5 Pa, S-1 in [-0.05, 0.01], FP64 Newton (rtol 1e-6, atol 1e-8), hipGraph",
+        ndrops=n, ms_per_step=ms, value=val, unit="superdroplet-steps/s", newton_iters_per_drop=iters,
+        fp64_flop_per_drop_est=flop, fp64_TFLOPs=flop * n / (ms / 1e3) / 1e12, fp64_peak_TFLOPs=FP64_PEAK_TFLOPS,
+        hbm_model_bytes_per_drop=32, hbm_model_GBs=32 * n / (ms / 1e3) / 1e9, cpu_baseline=cpu,
+        cpu_newton_iters_per_drop=cst["newton_iters"] / max(cst["cond_drops"], 1), gpu_over_cpu=val / cpu["value"],
+        tiles={f: int(getattr(tiles, f)) for f, _ in tiles._fields_})
+
+    # the same population with the learned warm start: MLP trained on 10M samples of the plain solver
+    # (Adam lr 1e-3, batch 65536, 20 epochs), Newton unchanged, restart from the plain guess after 10
+    cap = 10_000_000
+    xd = torch.zeros((cap, 8), dtype=torch.float32, device="cuda")
+    yd = torch.zeros(cap, dtype=torch.float32, device="cuda")
+    nw = C.c_uint64()
+    dl.check(L.cleo_mlp_make_samples(dl.h, C.byref(cp), xd.data_ptr(), yd.data_ptr(), cap, C.byref(nw)), "samples")
+    tp = abi.MLPTrainParams(20, 65536, 1e-3, 0.9, 0.999, 1e-8, 20261202)
+    loss = C.c_float()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dl.check(L.cleo_mlp_train(dl.h, xd.data_ptr(), yd.data_ptr(), int(nw.value), C.byref(tp), C.byref(loss)), "train")
+    t_train = time.perf_counter() - t0
+    del xd, yd
+    torch.cuda.empty_cache()
+    cpl = abi.CondParams(100, 1, 50, 1e-6, 1e-8, 0.001, 1, 10)
+    assert dl.set_sdm_chain([abi.cond_process(cpl)]) == 0
+    msl, statsl = _graph_steps(dl, L, torch, 200, k)
+    itl = statsl["newton_iters"] / max(statsl["cond_drops"], 1)
+    rp, rl = d.download()["radius"], dl.download()["radius"]
+    rel = np.abs(rl - rp) / rp
+    d.close()
+    dl.close()
+    out["condensation_learned_64cubed"] = dict(
+        workload="the condensation population and solver with the MLP warm start (6-64-64-1 SiLU, bf16 weights, "
+                 "fp32 accumulation)", ndrops=n, ms_per_step=msl, value=n * 1e3 / msl, unit="superdroplet-steps/s",
+        newton_iters_per_drop=itl, plain_newton_iters_per_drop=iters, iters_cut_vs_plain=1.0 - itl / iters,
+        warm_restarts_per_step=statsl["warm_restarts"] / k, plain_ms_per_step=ms,
+        radius_rel_diff_vs_plain={"mean": float(rel.mean()), "max": float(rel.max()),
+                                  "after_steps": 2 + k},
+        train={"samples": int(nw.value), "seconds": t_train, "final_mse_normalised": float(loss.value)})
+    return out
+
+
 def run_cpu(args):
     from cleo_amd import abi
     from cleo_amd import populations as P
@@ -329,10 +525,30 @@ def run_cpu(args):
             raise RuntimeError(f"oracle run_step rc={rc}")
     dt = time.perf_counter() - t0
     val = pop.n * args.cpu_steps / dt
-    return dict(value=val, unit="superdroplet-steps/s", cores=int(b.lib.orc_num_threads()), kind="port",
+    cores = int(b.lib.orc_num_threads())
+    d.close()
+    # the same code on one thread, one step of a 128 x 128 x 2 slab of the workload
+    one = P.coupled(ndims=(128, 128, 2))
+    d1 = abi.Domain(b, one.n, one.ndims, *one.halves)
+    d1.upload(one.supers)
+    d1.set_state(one.state)
+    d1.set_solute(*P.solute_dimless(one.solute))
+    d1.set_sdm(sdm)
+    d1.rng_counter(20261202)
+    b.lib.orc_set_num_threads(1)
+    d1.run_step(0, 100)
+    t1 = time.perf_counter()
+    d1.run_step(100, 200)
+    dt1 = time.perf_counter() - t1
+    b.lib.orc_set_num_threads(cores)
+    d1.close()
+    return dict(value=val, unit="superdroplet-steps/s", cores=cores, kind="port",
                 sample=f"coupled workload on 128x128x{args.cpu_ny} gridboxes ({pop.n} drops, Poisson(64)), "
                        f"{args.cpu_steps} steps after 2 warm-up steps, {dt:.1f} s, OpenMP oracle "
-                       f"(gridbox-parallel, 72-byte AoS, atomic counting sort)")
+                       f"(gridbox-parallel, 72-byte AoS, atomic counting sort, g++ -O3 -march=x86-64-v3)",
+                one_thread=dict(value=one.n / dt1, unit="superdroplet-steps/s",
+                                sample=f"one step of 128x128x2 gridboxes ({one.n} drops) after one warm-up step"),
+                host=host_cpu())
 
 
 def main():
@@ -360,6 +576,7 @@ def main():
     achieved = alg_bytes / (km[dom] / 1e3) / 1e9
     traffic = measured_traffic(dom, g["ndims"])
     cpu = None if args.no_cpu else run_cpu(args)
+    configs = None if args.no_configs else run_configs(args)
     out = {
         "metric": METRIC,
         "value": value,
@@ -386,6 +603,9 @@ def main():
         "stage_ms": g["stage_ms"],
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "profiles/traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE of this kernel "
+                                       "on this workload, scaled by a copy kernel of known size (not measured in "
+                                       "this run)" if traffic is not None else None,
                      "algorithmic_bytes": alg_bytes, "kernel_ms": km[dom]},
         "kernel_ms": km,
         # MassMomentsObserver's reduction: read xi, r, msol | write mom0, mom1, mom2 per gridbox
@@ -410,6 +630,8 @@ def main():
         }
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
+    if configs:
+        out["configs"] = configs
     if g["err"][0] != 0:
         out["device_error"] = g["err"]
     print(json.dumps(out), flush=True)

@@ -1569,29 +1569,29 @@ static size_t scan_words_rec(uint64_t len) {
 }
 size_t scan_tmp_words(uint64_t len) { return scan_words_rec(len) + 16; }
 
-static void scan_rec(cleo_ctx* c, const uint32_t* in, uint64_t len, uint32_t stride, uint32_t offset, uint32_t* out,
-                     const uint32_t* flag, uint32_t* tmp) {
+static void scan_rec(hipStream_t stream, const uint32_t* in, uint64_t len, uint32_t stride, uint32_t offset,
+                     uint32_t* out, const uint32_t* flag, uint32_t* tmp) {
   if (len == 0) {
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, c->stream, out, 0u, flag);
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, stream, out, 0u, flag);
     return;
   }
   const uint64_t nb = (len + kScanTile - 1) / kScanTile;
   if (nb == 1) {
-    hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(kScanThreads), 0, c->stream, in, len, stride, offset,
+    hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(kScanThreads), 0, stream, in, len, stride, offset,
                        static_cast<const uint32_t*>(nullptr), out, flag);
     return;
   }
   uint32_t* partial = tmp;
   uint32_t* pscan = tmp + nb;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, c->stream, in, len, stride,
+  hipLaunchKernelGGL(k_scan_reduce, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, stream, in, len, stride,
                      offset, partial, flag);
-  scan_rec(c, partial, nb, 1u, 0u, pscan, flag, tmp + nb + nb + 1);
-  hipLaunchKernelGGL(k_scan_apply, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, c->stream, in, len, stride,
+  scan_rec(stream, partial, nb, 1u, 0u, pscan, flag, tmp + nb + nb + 1);
+  hipLaunchKernelGGL(k_scan_apply, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, stream, in, len, stride,
                      offset, static_cast<const uint32_t*>(pscan), out, flag);
 }
 void scan_u32(cleo_ctx* c, const uint32_t* in, uint64_t len, uint32_t stride, uint32_t offset, uint32_t* out,
               const uint32_t* flag) {
-  scan_rec(c, in, len, stride, offset, out, flag, c->scan_tmp);
+  scan_rec(c->stream, in, len, stride, offset, out, flag, c->scan_tmp);
 }
 
 // argument block of the neighbour-sort kernels reading `src` and writing `dst`
@@ -1747,36 +1747,66 @@ void motion_sort_gather(cleo_ctx* c, const MotionArgs& args, const SoA& dst, int
   if (commit_dst >= 0) nsort_commit(c, commit_dst);
 }
 
+uint64_t radix_scratch_words(uint64_t n) {
+  const uint64_t ntiles = (n + 255) / 256;
+  return 4 * n + (32 * ntiles + 1) + scan_tmp_words(16 * ntiles);
+}
+RadixScratch radix_scratch_at(uint32_t* base, uint64_t n) {
+  const uint64_t ntiles = (n + 255) / 256;
+  RadixScratch sc{};
+  sc.key[0] = base;
+  sc.key[1] = base + n;
+  sc.idx[0] = base + 2 * n;
+  sc.idx[1] = base + 3 * n;
+  sc.cnt = base + 4 * n;
+  sc.scan_tmp = sc.cnt + 32 * ntiles + 1;
+  return sc;
+}
+
+// stable LSD radix sort of the drops of `src` by sdgbxindex (keys >= ngbx in one bucket after every
+// gridbox, as _get_count_position, sortsupers.hpp:44-48), gathered into `dst`, with off[0..ngbx]
+// the first position of each bucket (find_refs, findrefs.hpp)
+void radix_sort_soa(hipStream_t stream, const SoA& src, const SoA& dst, uint64_t n, uint32_t ngbx, uint32_t* off,
+                    const RadixScratch& sc) {
+  const uint32_t ntiles = static_cast<uint32_t>((n + 255) / 256);
+  if (n == 0) {
+    hipLaunchKernelGGL(k_bucket_bounds, dim3(1), dim3(256), 0, stream, static_cast<const uint32_t*>(sc.key[0]), 0ull,
+                       ngbx, off);
+    return;
+  }
+  hipLaunchKernelGGL(k_rs_init, dim3(stride_blocks(n)), dim3(256), 0, stream, src.gbx, n, ngbx, sc.key[0], sc.idx[0]);
+  int passes = 1;
+  while (passes < 8 && (static_cast<uint64_t>(ngbx) >> (4 * passes)) != 0) ++passes;
+  int cur = 0;
+  uint32_t* counts = sc.cnt;
+  uint32_t* offs = sc.cnt + 16ull * ntiles;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = 4 * p;
+    hipLaunchKernelGGL(k_rs_count, dim3(ntiles), dim3(256), 0, stream, sc.key[cur], n, shift, ntiles, counts);
+    scan_rec(stream, counts, 16ull * ntiles, 1u, 0u, offs, nullptr, sc.scan_tmp);
+    hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(256), 0, stream, sc.key[cur], sc.idx[cur], n, shift, ntiles,
+                       static_cast<const uint32_t*>(offs), sc.key[1 - cur], sc.idx[1 - cur]);
+    cur = 1 - cur;
+  }
+  hipLaunchKernelGGL(k_gather_soa, dim3(stride_blocks(n)), dim3(256), 0, stream, src, dst,
+                     static_cast<const uint32_t*>(sc.idx[cur]), n);
+  hipLaunchKernelGGL(k_bucket_bounds, dim3(stride_blocks(n + 1)), dim3(256), 0, stream,
+                     static_cast<const uint32_t*>(sc.key[cur]), n, ngbx, off);
+}
+
 void generic_sort(cleo_ctx* c) {
   const uint64_t n = c->ntot;
   const uint32_t ngbx = c->grid.ngbx;
   const int src = c->cur, dst = 1 - c->cur;
-  const uint32_t ntiles = static_cast<uint32_t>((n + 255) / 256);
-  if (n > 0) {
-    hipLaunchKernelGGL(k_rs_init, dim3(stride_blocks(n)), dim3(256), 0, c->stream, c->buf[src].gbx, n, ngbx,
-                       c->rs_key[0], c->rs_idx[0]);
-    int passes = 1;
-    while (passes < 8 && (static_cast<uint64_t>(ngbx) >> (4 * passes)) != 0) ++passes;
-    int cur = 0;
-    uint32_t* counts = c->rs_cnt;
-    uint32_t* offs = c->rs_cnt + 16ull * ntiles;
-    for (int p = 0; p < passes; ++p) {
-      const int shift = 4 * p;
-      hipLaunchKernelGGL(k_rs_count, dim3(ntiles), dim3(256), 0, c->stream, c->rs_key[cur], n, shift, ntiles, counts);
-      scan_u32(c, counts, 16ull * ntiles, 1u, 0u, offs, nullptr);
-      hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(256), 0, c->stream, c->rs_key[cur], c->rs_idx[cur], n, shift,
-                         ntiles, static_cast<const uint32_t*>(offs), c->rs_key[1 - cur], c->rs_idx[1 - cur]);
-      cur = 1 - cur;
-    }
-    hipLaunchKernelGGL(k_gather_soa, dim3(stride_blocks(n)), dim3(256), 0, c->stream, c->buf[src], c->buf[dst],
-                       static_cast<const uint32_t*>(c->rs_idx[cur]), n);
-    hipLaunchKernelGGL(k_bucket_bounds, dim3(stride_blocks(n + 1)), dim3(256), 0, c->stream,
-                       static_cast<const uint32_t*>(c->rs_key[cur]), n, ngbx, c->off);
-    c->cur = dst;
-  } else {
-    hipLaunchKernelGGL(k_bucket_bounds, dim3(1), dim3(256), 0, c->stream, static_cast<const uint32_t*>(c->rs_key[0]),
-                       0ull, ngbx, c->off);
+  RadixScratch sc{};
+  for (int i = 0; i < 2; ++i) {
+    sc.key[i] = c->rs_key[i];
+    sc.idx[i] = c->rs_idx[i];
   }
+  sc.cnt = c->rs_cnt;
+  sc.scan_tmp = c->scan_tmp;
+  radix_sort_soa(c->stream, c->buf[src], c->buf[dst], n, ngbx, c->off, sc);
+  if (n > 0) c->cur = dst;
   hipLaunchKernelGGL(k_set_nact, dim3(nblocks(ngbx, 256)), dim3(256), 0, c->stream, c->off, ngbx, c->nact,
                      static_cast<const uint32_t*>(nullptr));
   hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, static_cast<const uint32_t*>(nullptr));

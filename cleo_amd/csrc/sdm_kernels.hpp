@@ -175,6 +175,18 @@ void motion_sort_gather(cleo_ctx* c, const MotionArgs& args, const SoA& dst, int
 void motion_sort_fused(cleo_ctx* c, const MotionArgs& args, const SoA& dst, int commit_dst);
 // generic stable sort of buf[cur] into buf[1-cur]; swaps cur; sets off and nact
 void generic_sort(cleo_ctx* c);
+// the generic sort without a context: scratch of radix_scratch_words(n) u32 words, laid out by
+// radix_scratch_at (two key and two index arrays of n words, the digit counts, the scan's partials)
+struct RadixScratch {
+  uint32_t* key[2];
+  uint32_t* idx[2];
+  uint32_t* cnt;
+  uint32_t* scan_tmp;
+};
+uint64_t radix_scratch_words(uint64_t n);
+RadixScratch radix_scratch_at(uint32_t* base, uint64_t n);
+void radix_sort_soa(hipStream_t stream, const SoA& src, const SoA& dst, uint64_t n, uint32_t ngbx, uint32_t* off,
+                    const RadixScratch& sc);
 void launch_copy_oob_region(cleo_ctx* c, int src, int dst);
 // applies a pending collision order to the store (buf[cur] then holds the drops in order)
 void materialize_order(cleo_ctx* c);

@@ -71,8 +71,10 @@ def _koehler_ab(temp, msol, sol):
 
 
 def equilibrium_radius(s_ratio, temp, msol, sol, rdry):
-    """Wet radius with 1 + a/r - b/r^3 = S on the stable branch r < r_crit; S above the
-    activation supersaturation is clamped to just below it."""
+    """Wet radius with 1 + a/r - b/r^3 = S on the stable branch r < r_crit.  Above the activation
+    supersaturation no equilibrium exists: such a drop starts on the growing branch at 1.5 r_crit
+    (a drop left just below r_crit needs more than 50 Newton iterations in its first solve from the
+    reference's 1 mm guess, which the reference aborts on)."""
     akoh, bkoh = _koehler_ab(temp, msol, sol)
     rcrit = np.sqrt(3.0 * bkoh / akoh)
     scrit = 1.0 + np.sqrt(4.0 * akoh ** 3 / 27.0 / bkoh)
@@ -86,7 +88,7 @@ def equilibrium_radius(s_ratio, temp, msol, sol, rdry):
         neg = f(mid) < 0
         lo = np.where(neg, mid, lo)
         hi = np.where(neg, hi, mid)
-    return np.maximum(0.5 * (lo + hi), rdry)
+    return np.where(s_ratio >= scrit, 1.5 * rcrit, np.maximum(0.5 * (lo + hi), rdry))
 
 
 class Population:
@@ -236,16 +238,10 @@ def coupled(seed=20261202, ndims=(128, 128, 128), spacing_m=25.0, mean=64.0, sol
     return pop
 
 
-def coupled_device(torch, device, seed=20261202, ndims=(128, 128, 128), spacing_m=25.0, mean=64.0,
-                   solute=AMMONIUM_SULPHATE, wmax_mps=2.0):
-    """The `coupled` workload drawn on the GPU with torch (same distributions, different
-    random stream): returns (arrays dict of device tensors, state dict of numpy arrays, halves).
-    sdgbxindex is int32 and xi, sdid int64 tensors; their bits are the u32 / u64 the C ABI reads."""
-    gen = torch.Generator(device=device)
-    gen.manual_seed(seed)
+def _device_drops(torch, device, gen, ndims, spacing_m, mean):
+    """Poisson counts per gridbox, the drops' gridboxes (int32) and coordinates, on the GPU."""
     ngbx = int(np.prod(ndims))
     nz, nx, ny = ndims
-    halves = uniform_grid(ndims, spacing_m)
     counts = torch.poisson(torch.full((ngbx,), float(mean), device=device, dtype=torch.float32),
                            generator=gen).to(torch.int64)
     gidx = torch.repeat_interleave(torch.arange(ngbx, device=device, dtype=torch.int64), counts)
@@ -261,7 +257,12 @@ def coupled_device(torch, device, seed=20261202, ndims=(128, 128, 128), spacing_
         del u, lo, c
     del iz, ix, iy
     out["sdgbxindex"] = gidx.to(torch.int32)
-    del gidx
+    return out, n
+
+
+def _device_coal_breakup_attrs(torch, device, gen, out, n, solute):
+    """coal_breakup's attributes on the GPU: log-uniform solute, bimodal lognormal radii raised to
+    the dry radius, xi ~ U{1e6..1e8}."""
     rho_sol, _, _ = solute_dimless(solute)
     lmin, lmax = np.log(1e-18), np.log(1e-15)
     msol = torch.exp(lmin + (lmax - lmin) * torch.rand(n, device=device, dtype=torch.float64, generator=gen)) / MASS0
@@ -276,6 +277,71 @@ def coupled_device(torch, device, seed=20261202, ndims=(128, 128, 128), spacing_
     del radius, rdry
     out["xi"] = torch.randint(1_000_000, 100_000_001, (n,), device=device, dtype=torch.int64, generator=gen)
     out["sdid"] = torch.arange(n, device=device, dtype=torch.int64)
+
+
+def coal_breakup_device(torch, device, seed=20261202, ndims=(64, 64, 64), spacing_m=50.0, mean=128.0,
+                        solute=NACL):
+    """The `coal_breakup` workload drawn on the GPU (same distributions, torch's random stream):
+    (arrays dict of device tensors, state dict of numpy arrays, halves), as coupled_device."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    out, n = _device_drops(torch, device, gen, ndims, spacing_m, mean)
+    _device_coal_breakup_attrs(torch, device, gen, out, n, solute)
+    ngbx = int(np.prod(ndims))
+    return out, _state(ngbx, spacing_m ** 3 / VOL0, 288.15, 1e5, 1.0), uniform_grid(ndims, spacing_m)
+
+
+def condensation_device(torch, device, seed=20261202, ndims=(64, 64, 64), spacing_m=50.0, mean=122.0,
+                        solute=AMMONIUM_SULPHATE):
+    """The `condensation` workload drawn on the GPU: lognormal dry radii, wet radius at Koehler
+    equilibrium with the gridbox's supersaturation (as equilibrium_radius: the same bisection, and
+    1.5 r_crit for activated drops)."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    out, n = _device_drops(torch, device, gen, ndims, spacing_m, mean)
+    ngbx = int(np.prod(ndims))
+    rng = np.random.default_rng(seed + 1)
+    temp_K, press_Pa, s_ratio = _thermo(rng, ngbx)
+    st = _state(ngbx, spacing_m ** 3 / VOL0, temp_K, press_Pa, s_ratio)
+    rho_sol, mr_sol, ionic = solute_dimless(solute)
+    rdry = torch.exp(np.log(0.04e-6) + np.log(1.5) * torch.randn(n, device=device, dtype=torch.float64,
+                                                                  generator=gen)) / R0
+    msol = 4.0 / 3.0 * np.pi * rdry ** 3 * rho_sol
+    rdry = torch.pow(3.0 / (4.0 * np.pi) * msol / rho_sol, 1.0 / 3.0)
+    g = out["sdgbxindex"].to(torch.int64)
+    temp = torch.from_numpy(st["temp"]).to(device)[g]
+    s = torch.from_numpy(s_ratio).to(device)[g]
+    del g
+    akoh = 3.3e-7 / (TEMP0 * R0) / temp
+    bkoh = 4.3e-6 * RHO0 / MR_DRY * msol * ionic / mr_sol
+    rcrit = torch.sqrt(3.0 * bkoh / akoh)
+    scrit = 1.0 + torch.sqrt(4.0 * akoh ** 3 / 27.0 / bkoh)
+    s_eff = torch.minimum(s, 1.0 + (scrit - 1.0) * 0.999)
+    lo, hi = rdry * 1.000001, rcrit
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        neg = (1.0 + akoh / mid - bkoh / mid ** 3 - s_eff) < 0
+        lo = torch.where(neg, mid, lo)
+        hi = torch.where(neg, hi, mid)
+    out["radius"] = torch.where(s >= scrit, 1.5 * rcrit, torch.maximum(0.5 * (lo + hi), rdry))
+    out["msol"] = msol
+    del lo, hi, mid, neg, akoh, bkoh, rcrit, scrit, s_eff, temp, s
+    out["xi"] = torch.randint(1_000_000, 100_000_001, (n,), device=device, dtype=torch.int64, generator=gen)
+    out["sdid"] = torch.arange(n, device=device, dtype=torch.int64)
+    return out, st, uniform_grid(ndims, spacing_m)
+
+
+def coupled_device(torch, device, seed=20261202, ndims=(128, 128, 128), spacing_m=25.0, mean=64.0,
+                   solute=AMMONIUM_SULPHATE, wmax_mps=2.0):
+    """The `coupled` workload drawn on the GPU with torch (same distributions, different
+    random stream): returns (arrays dict of device tensors, state dict of numpy arrays, halves).
+    sdgbxindex is int32 and xi, sdid int64 tensors; their bits are the u32 / u64 the C ABI reads."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    ngbx = int(np.prod(ndims))
+    halves = uniform_grid(ndims, spacing_m)
+    out, n = _device_drops(torch, device, gen, ndims, spacing_m, mean)
+    _device_coal_breakup_attrs(torch, device, gen, out, n, solute)
     rng = np.random.default_rng(seed + 1)
     temp_K, press_Pa, s_ratio = _thermo(rng, ngbx)
     st = _state(ngbx, spacing_m ** 3 / VOL0, temp_K, press_Pa, s_ratio)

@@ -1156,6 +1156,14 @@ int orc_num_threads(void) {
   return 1;
 #endif
 }
+/* threads of the OpenMP build's later parallel regions (the 1-thread CPU baseline) */
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
 
 int orc_upload_supers(orc_ctx* h, const cleo_supers_soa* s, uint64_t ntot) {
   Ctx& c = *reinterpret_cast<Ctx*>(h);

@@ -54,5 +54,7 @@ def load(kind="serial"):
     lib.orc_constants.argtypes = [C.POINTER(C.c_double)]
     lib.orc_constants.restype = None
     lib.orc_num_threads.restype = C.c_int
+    lib.orc_set_num_threads.argtypes = [C.c_int]
+    lib.orc_set_num_threads.restype = None
     lib.orc_set_cond_guess.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_uint64]
     return b

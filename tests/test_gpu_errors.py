@@ -1,5 +1,9 @@
 """Failure paths: each reference Kokkos::abort site forced from the host gives the same error
-code in the device error word as in the oracle (every kernel still exits normally)."""
+code in the device error word as in the oracle (every kernel still exits normally).
+
+Sites not forced: xi = 0 (superdrop_attrs.hpp:156-160) and coalescence undefined
+(coalescence.cpp:108-112,131-143) are reachable only with a multiplicity of 0 in a colliding
+pair, which divides by zero in the reference's gamma before them (xi1 / xi2, coalescence.cpp:75)."""
 import numpy as np
 import pytest
 
@@ -7,7 +11,126 @@ from cleo_amd import abi
 from cleo_amd import populations as P
 from tests.helpers import SEED, coll_params, cond_params, make_domain, motion_params
 
-pytestmark = pytest.mark.gpu
+
+def _two_drops(xi, r, msol, ndims=(1, 1, 1), gbx=(0, 0), coords=None):
+    """A tiny population of given drops in a domain of 20 m gridboxes with a mild State."""
+    n = len(xi)
+    halves = P.uniform_grid(ndims, 20.0)
+    c = coords if coords is not None else [np.full(n, 0.01), np.full(n, 0.01), np.full(n, 0.01)]
+    sup = dict(sdgbxindex=np.asarray(gbx, np.uint32), coord3=np.asarray(c[0], np.float64),
+               coord1=np.asarray(c[1], np.float64), coord2=np.asarray(c[2], np.float64),
+               xi=np.asarray(xi, np.uint64), radius=np.asarray(r, np.float64), msol=np.asarray(msol, np.float64),
+               sdid=np.arange(n, dtype=np.uint64))
+    ngbx = int(np.prod(ndims))
+    st = P.condensation(SEED, ndims=ndims, mean=1).state
+    st["volume"][:] = 20.0 ** 3 / P.VOL0
+    return P.Population(ndims, halves, sup, st, P.NACL)
+
+
+def _dry_radius(msol):
+    rho_sol = P.solute_dimless(P.NACL)[0]
+    return (3.0 / (4.0 * np.pi) * msol / rho_sol) ** (1.0 / 3.0)
+
+
+def _oracle_code(oracle, pop, fn):
+    d = make_domain(oracle, pop)
+    fn(d)
+    return d.error()[0]
+
+
+def test_oracle_codes_of_forced_sites(oracle):
+    """The oracle's codes for the forced cases below (the GPU tests compare against these)."""
+    assert _oracle_code(oracle, _temp_zero_pop(), lambda d: d.condensation(cond_params(maxniters=5))) == 5
+    assert _oracle_code(oracle, _below_dry_pop(), _coalesce_const) == 2
+    assert _oracle_code(oracle, _wet_below_dry_pop(), lambda d: d.condensation(cond_params())) == 3
+    assert _oracle_code(oracle, _misplaced_pop(), lambda d: d.motion(motion_params(tv=abi.TV_NULL))) == 9
+
+
+def _temp_zero_pop():
+    pop = P.condensation(SEED, ndims=(2, 2, 2), mean=6)
+    pop.state["temp"][3] = 0.0  # thermodynamic_equations.cpp:40-42
+    return pop
+
+
+def _below_dry_pop():
+    """drop1 (xi 1005, r 0.001, no solute) coalesces into drop2 (xi 10) whose radius is below its dry
+    radius: gamma = min(huge, 1005 // 10) = 100, the different case, and drop2's new radius
+    (r2^3 + 100 * 1e-9)^(1/3) stays below dry (superdrop_attrs.hpp:175-181)."""
+    msol2 = 1.0
+    return _two_drops([1005, 10], [1e-3, 0.5 * _dry_radius(msol2)], [0.0, msol2])
+
+
+def _coalesce_const(d):
+    d.rng_counter(3)
+    return d.collisions(coll_params(prob=abi.PROB_CONST, param=1e30, enact=abi.ENACT_COAL))
+
+
+def _wet_below_dry_pop():
+    """a drop whose radius is below its dry radius has negative condensate
+    (superdrop_attrs.hpp:207-213)"""
+    msol = np.array([0.2, 1e-6])
+    return _two_drops([100, 100], [0.5 * _dry_radius(msol[0]), 2.0], msol)
+
+
+def _misplaced_pop():
+    """a drop whose coordinates lie three gridboxes above its sdgbxindex: motion's one-step
+    gridbox update leaves it outside its new gridbox's bounds (cartesian_motion.hpp:44-52)"""
+    z = [0.071, 0.005]  # gridbox faces every 0.02 in z: 0.071 is in gridbox 3
+    return _two_drops([100, 100], [5.0, 5.0], [1e-6, 1e-6], ndims=(5, 1, 1), gbx=(0, 0),
+                      coords=[z, [0.01, 0.01], [0.01, 0.01]])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", [0, 1])
+def test_temperature_not_positive(gpu, oracle, variant):
+    print(f"case: T = 0 in one gridbox, condensation variant {variant}", flush=True)
+    pop = _temp_zero_pop()
+    g, o = _both(gpu, oracle, pop)
+    t = abi.Tiles(1, 1, 1, variant, 0, 0, 0, 0, 0)
+    assert gpu.lib.cleo_set_tiles(g.h, abi.ctypes_byref(t)) == 0
+    for d in (g, o):
+        d.condensation(cond_params(maxniters=5))
+        assert d.error()[:2] == (5, 3)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tiled", [0, 1])
+def test_radius_below_dry_after_coalescence(gpu, oracle, tiled):
+    print(f"case: coalescence leaves a radius below the dry radius, tiled={tiled}", flush=True)
+    pop = _below_dry_pop()
+    g, o = _both(gpu, oracle, pop)
+    t = abi.Tiles(1, 1, 1, 0, 0, 0, 0, 0, tiled)
+    assert gpu.lib.cleo_set_tiles(g.h, abi.ctypes_byref(t)) == 0
+    for d in (g, o):
+        _coalesce_const(d)
+        assert d.error()[0] == 2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_negative_condensate(gpu, oracle, variant):
+    print(f"case: radius below dry radius in condensation, variant {variant}", flush=True)
+    pop = _wet_below_dry_pop()
+    g, o = _both(gpu, oracle, pop)
+    t = abi.Tiles(1, 1, 1, variant, 0, 0, 0, 0, 0)
+    assert gpu.lib.cleo_set_tiles(g.h, abi.ctypes_byref(t)) == 0
+    for d in (g, o):
+        d.condensation(cond_params())
+        assert d.error()[0] == 3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("motion_flat", [0, 1])
+def test_bounds_check_after_motion(gpu, oracle, motion_flat):
+    print(f"case: drop outside its gridbox after motion, motion_flat={motion_flat}", flush=True)
+    pop = _misplaced_pop()
+    g, o = _both(gpu, oracle, pop)
+    t = abi.Tiles(1, 1, 1, 0, motion_flat, 0, 0, 0, 0)
+    assert gpu.lib.cleo_set_tiles(g.h, abi.ctypes_byref(t)) == 0
+    for d in (g, o):
+        d.motion(motion_params(tv=abi.TV_NULL))
+        assert d.error()[0] == 9
+
 
 
 def _both(gpu, oracle, pop):
@@ -18,6 +141,7 @@ def _code(d, rc):
     return d.error()[0]
 
 
+@pytest.mark.gpu
 def test_cfl_violation(gpu, oracle):
     print("case: CFL abort", flush=True)
     pop = P.coupled(SEED, ndims=(4, 4, 2), mean=10)
@@ -28,6 +152,7 @@ def test_cfl_violation(gpu, oracle):
         assert _code(d, rc) == 8
 
 
+@pytest.mark.gpu
 @pytest.mark.parametrize("variant", [0, 1, 2])
 def test_newton_maxiters(gpu, oracle, variant):
     print(f"case: Newton maxniters abort, condensation variant {variant}", flush=True)
@@ -42,6 +167,7 @@ def test_newton_maxiters(gpu, oracle, variant):
         assert _code(d, rc) == 4
 
 
+@pytest.mark.gpu
 def test_breakup_nfrags(gpu, oracle):
     print("case: breakup nfrags <= 2.5 abort", flush=True)
     pop = P.coal_breakup(SEED, ndims=(2, 2, 1), mean=60)
@@ -53,6 +179,7 @@ def test_breakup_nfrags(gpu, oracle):
         assert _code(d, rc) == 7
 
 
+@pytest.mark.gpu
 def test_breakup_cke_nfrags_without_velocity(gpu, oracle):
     print("case: kinetic-energy nfrags with no terminal velocity (nfrags = 2.5) abort", flush=True)
     pop = P.coal_breakup(SEED, ndims=(2, 2, 1), mean=60)
@@ -66,6 +193,7 @@ def test_breakup_cke_nfrags_without_velocity(gpu, oracle):
         assert _code(d, rc) == 7
 
 
+@pytest.mark.gpu
 def test_timestep_below_minsubstep(gpu, oracle):
     print("case: delt < MINSUBTSTEP abort", flush=True)
     pop = P.condensation(SEED, ndims=(2, 2, 2), mean=5)
@@ -75,6 +203,7 @@ def test_timestep_below_minsubstep(gpu, oracle):
         assert rc != 0 and d.error()[0] == 11
 
 
+@pytest.mark.gpu
 def test_bad_injected_index_is_an_error(gpu):
     print("case: injected Fisher-Yates index out of range", flush=True)
     pop = P.coal_breakup(SEED, ndims=(1, 1, 1), mean=6)
