@@ -404,17 +404,69 @@ def _cpu_leg(torch, gen, ndims, solute, procs, steps, warm=1, **kw):
 
 
 def run_configs(args):
+    """The other workloads, each on its own: a device error in one is reported under its name."""
     import torch
 
-    from cleo_amd import abi, lib
-    from cleo_amd import populations as P
+    from cleo_amd import lib
 
     b = lib.load()
+    out = {}
+    for leg in (_leg_golovin_box, _leg_coal_breakup, _leg_condensation):
+        try:
+            out.update(leg(args, b, torch))
+        except RuntimeError as e:
+            out[leg.__name__[5:]] = {"error": str(e)}
+        torch.cuda.empty_cache()
+    return out
+
+
+def _leg_golovin_box(args, b, torch):
+    from cleo_amd import abi
+    from cleo_amd import populations as P
+    from oracle import pyoracle
+    L = b.lib
+    out = {}
+    # the Golovin box (the reference's CPU-runnable case): 1 gridbox of 1e6 m^3, 8192 drops, Golovin
+    # kernel, 3600 steps of 1 s replayed as hipGraphs, against the oracle on the same box
+    gbox = P.golovin_box()
+    kg = abi.CollParams(100, abi.PROB_GOLOVIN, 0.0, abi.ENACT_COAL, 5.0, 0.85)
+    res = {}
+    for name, backend in (("gpu", b), ("cpu", pyoracle.load("omp"))):
+        dd = abi.Domain(backend, gbox.n, gbox.ndims, *gbox.halves)
+        dd.upload(gbox.supers)
+        dd.set_state(gbox.state)
+        dd.rng_counter(20261202)
+        assert dd.set_sdm_chain([abi.coll_process(kg)]) == 0
+        t0 = time.perf_counter()
+        if name == "gpu":
+            d_ms = _graph_steps(dd, L, torch, 0, 3600)[0]
+            res[name] = d_ms
+        else:
+            for s_ in range(3600):
+                dd.run_step(100 * s_, 100 * (s_ + 1))
+            res[name] = (time.perf_counter() - t0) * 1e3 / 3600
+        xi_end = float(np.sum(dd.download()["xi"], dtype=np.float64))
+        res[name + "_xi_ratio"] = xi_end / float(np.sum(gbox.supers["xi"], dtype=np.float64))
+        dd.close()
+    out["golovin_box"] = dict(
+        workload="1 gridbox of 1e6 m^3, 8192 drops, exponential-in-volume radii (30.531 um), Golovin kernel, "
+                 "3600 steps of 1 s, counter RNG, hipGraph", ndrops=gbox.n, ms_per_step=res["gpu"],
+        value=gbox.n * 1e3 / res["gpu"], unit="superdroplet-steps/s",
+        real_droplets_left=res["gpu_xi_ratio"],
+        cpu_baseline=dict(value=gbox.n * 1e3 / res["cpu"], unit="superdroplet-steps/s", cores=1, kind="port",
+                          sample="the same 3600 steps (one gridbox runs on one thread), OpenMP oracle",
+                          real_droplets_left=res["cpu_xi_ratio"]),
+        note="one gridbox of 8192 drops is far too small to fill the GPU: the step is launch-bound")
+    return out
+
+
+def _leg_coal_breakup(args, b, torch):
+    from cleo_amd import abi
+    from cleo_amd import populations as P
     L = b.lib
     dev = torch.device("cuda:0")
     k = args.config_steps
     out = {}
-
     # collision-coalescence + breakup alone: 64^3 gridboxes of 50 m, Poisson(128) drops, Long kernel,
     # DoCoalBuRe<ConstNFrags(5), ConstCoalBuReFlag(0.85)>, dt = 1 s
     kp = abi.CollParams(100, abi.PROB_LONG, 1.0, abi.ENACT_COALBURE, 5.0, 0.85)
@@ -435,7 +487,17 @@ def run_configs(args):
         coll_events_per_step=stats["coll_events"] / k,
         hbm_model_bytes_per_drop=48, hbm_model_GBs=48 * n / (ms / 1e3) / 1e9, cpu_baseline=cpu,
         gpu_over_cpu=val / cpu["value"])
+    return out
 
+
+def _leg_condensation(args, b, torch):
+    from cleo_amd import abi
+    from cleo_amd import populations as P
+    L = b.lib
+    dev = torch.device("cuda:0")
+    k = args.config_steps
+    ndims = (64, 64, 64)
+    out = {}
     # condensation alone: 64^3 gridboxes, Poisson(122), ammonium sulphate at Koehler equilibrium, per
     # gridbox T, p, S; FP64 Newton rtol 1e-6, atol 1e-8, maxniters 50, thermodynamic feedback
     cp = abi.CondParams(100, 1, 50, 1e-6, 1e-8, 0.001, 0, 10)

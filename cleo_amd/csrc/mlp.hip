@@ -9,8 +9,9 @@
 // (32x32x8, bf16 operands): A[i][k] from lane (i + 32*(k/4)) element k%4; B[k][j] from lane
 // (j + 32*(k/4)) element k%4; D[i][j] in lane (j + 32*((i/4)%2)) accumulator 4*(i/8) + i%4.
 //
-// Training: each wave computes the loss gradient of its 64 samples; per-wave partial gradients
-// are summed in a fixed order by one kernel that also applies Adam, so a run is deterministic.
+// Training: each wave computes the loss gradient of its 64 samples; the per-wave partial gradients
+// are summed in a fixed order, in chunks of 32 waves and then the chunk sums in chunk order (the
+// second kernel also applies Adam), so a run is deterministic.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -255,14 +256,27 @@ __global__ void __launch_bounds__(64) k_mlp_grad(TrainArgs a) {
   }
 }
 
-// grad[p] = sum over waves in order; then Adam on the fp32 master weights and refreshed bf16 copies
-__global__ void k_mlp_reduce_adam(const float* partial, uint32_t nwaves, float* grad, float* loss, float* master,
+// first level of the gradient sum: chunk c of kReduceChunk waves, summed in wave order, for every
+// parameter (and the loss, entry NPARAM); one thread per (parameter, chunk)
+constexpr uint32_t kReduceChunk = 32;
+__global__ void k_mlp_reduce_chunks(const float* partial, uint32_t nwaves, float* chunks) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > NPARAM) return;
+  const uint32_t w0 = blockIdx.y * kReduceChunk, w1 = min(w0 + kReduceChunk, nwaves);
+  float s = 0.0f;
+  for (uint32_t w = w0; w < w1; ++w) s += partial[static_cast<uint64_t>(w) * (NPARAM + 1) + p];
+  chunks[static_cast<uint64_t>(blockIdx.y) * (NPARAM + 1) + p] = s;
+}
+
+// grad[p] = the chunk sums added in chunk order (a fixed order: training is deterministic); then
+// Adam on the fp32 master weights
+__global__ void k_mlp_reduce_adam(const float* chunks, uint32_t nchunks, float* grad, float* loss, float* master,
                                   float* m, float* v, int32_t apply, float lr, float b1, float b2, float eps,
                                   float bc1, float bc2) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p > NPARAM) return;
   float s = 0.0f;
-  for (uint32_t w = 0; w < nwaves; ++w) s += partial[static_cast<uint64_t>(w) * (NPARAM + 1) + p];
+  for (uint32_t w = 0; w < nchunks; ++w) s += chunks[static_cast<uint64_t>(w) * (NPARAM + 1) + p];
   if (p == NPARAM) {
     *loss = s;
     return;
@@ -357,14 +371,27 @@ struct MLPDev {
   MLPWeights weights() const { return MLPWeights{w1p, w2, w2t, master}; }
 };
 
+// per-wave partial gradients, followed by room for the chunk sums of k_mlp_reduce_chunks
+static uint32_t reduce_nchunks(uint32_t waves) { return (waves + kReduceChunk - 1) / kReduceChunk; }
 static hipError_t ensure_partial(MLPDev* m, uint32_t waves) {
   if (m->partial_waves >= waves) return hipSuccess;
   hipFree(m->partial);
   m->partial = nullptr;
   m->partial_waves = 0;
-  hipError_t e = hipMalloc(reinterpret_cast<void**>(&m->partial), static_cast<size_t>(waves) * (NPARAM + 1) * 4);
+  const size_t rows = static_cast<size_t>(waves) + reduce_nchunks(waves);
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&m->partial), rows * (NPARAM + 1) * 4);
   if (e == hipSuccess) m->partial_waves = waves;
   return e;
+}
+// the two-level gradient sum of `waves` per-wave partials, then Adam when `apply`
+static void launch_reduce_adam(hipStream_t s, MLPDev* m, uint32_t waves, int32_t apply, float lr, float b1, float b2,
+                               float eps, float bc1, float bc2) {
+  const uint32_t nch = reduce_nchunks(waves);
+  float* ch = m->partial + static_cast<uint64_t>(m->partial_waves) * (NPARAM + 1);
+  hipLaunchKernelGGL(k_mlp_reduce_chunks, dim3((NPARAM + 256) / 256, nch), dim3(256), 0, s,
+                     static_cast<const float*>(m->partial), waves, ch);
+  hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((NPARAM + 256) / 256), dim3(256), 0, s, static_cast<const float*>(ch),
+                     nch, m->grad, m->loss, m->master, m->adam_m, m->adam_v, apply, lr, b1, b2, eps, bc1, bc2);
 }
 
 hipError_t mlp_create(cleo_ctx* c) {
@@ -543,9 +570,7 @@ int cleo_mlp_loss_grad(cleo_ctx* c, const float* x, const float* y, uint64_t n, 
   MLP_TRY(c, ensure_partial(m, waves));
   TrainArgs a{x, y, nullptr, 0, static_cast<uint32_t>(n), 2.0f / static_cast<float>(n), m->weights(), m->partial};
   hipLaunchKernelGGL(k_mlp_grad, dim3(waves), dim3(64), 0, c->stream, a);
-  hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((NPARAM + 256) / 256), dim3(256), 0, c->stream,
-                     static_cast<const float*>(m->partial), waves, m->grad, m->loss, m->master, m->adam_m, m->adam_v, 0,
-                     0.f, 0.f, 0.f, 0.f, 1.f, 1.f);
+  launch_reduce_adam(c->stream, m, waves, 0, 0.f, 0.f, 0.f, 0.f, 1.f, 1.f);
   MLP_TRY(c, hipGetLastError());
   MLP_TRY(c, hipStreamSynchronize(c->stream));
   float l = 0.f;
@@ -650,9 +675,7 @@ int cleo_mlp_train(cleo_ctx* c, const float* x, const float* y, uint64_t n, cons
       hipLaunchKernelGGL(k_mlp_grad, dim3(waves), dim3(64), 0, c->stream, a);
       const float bc1 = 1.0f - std::pow(tp->beta1, static_cast<float>(step));
       const float bc2 = 1.0f - std::pow(tp->beta2, static_cast<float>(step));
-      hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((NPARAM + 256) / 256), dim3(256), 0, c->stream,
-                         static_cast<const float*>(m->partial), waves, m->grad, m->loss, m->master, m->adam_m,
-                         m->adam_v, 1, tp->lr, tp->beta1, tp->beta2, tp->eps, bc1, bc2);
+      launch_reduce_adam(c->stream, m, waves, 1, tp->lr, tp->beta1, tp->beta2, tp->eps, bc1, bc2);
       launch_pack(c);
     }
   }

@@ -15,6 +15,13 @@ def load():
     if _backend is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} missing: run `python -m cleo_amd.build` (no CPU fallback exists)")
+        # torch bundles its own HIP runtime, loaded under another file name than libamdhip64.so.7.
+        # Imported first, it provides that soname and the library binds to it; imported after the
+        # library, it would start a second runtime in the process, which finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(LIB_PATH)
         b = Backend(lib, "cleo_")
         lib.cleo_device_info.argtypes = [C.c_char_p, C.c_size_t]

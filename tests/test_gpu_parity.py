@@ -433,3 +433,85 @@ def test_multiblock_scans_coupled(gpu, oracle, tiles):
         assert g.error()[0] == 0 and o.error()[0] == 0, (g.error(), o.error())
     assert_supers_match(g.download(), o.download(), what="multiblock")
     assert np.array_equal(g.refs(), o.refs())
+
+
+def test_coupled_twenty_steps(gpu, oracle):
+    """20 coupled steps (motion, sort, condensation, collisions) as a hipGraph with the bench's
+    tiling, against 20 oracle steps: drift between the two stays inside the parity bar."""
+    _say("case: 20 coupled steps, 12x10x8 gridboxes, hipGraph with thread-per-drop tiles")
+    pop = P.coupled(SEED + 11, ndims=(12, 10, 8), mean=48)
+    g, o = _both(gpu, oracle, pop)
+    t = abi.Tiles(1, 1, 1, 1, 1, 65536, 32768, 32768, 1, 8)
+    assert gpu.lib.cleo_set_tiles(g.h, abi.ctypes_byref(t)) == 0
+    s = sdm_params(cond=cond_params(), coll=coll_params(), motion=motion_params())
+    for d in (g, o):
+        d.set_sdm(s)
+        d.rng_counter(31)
+    assert gpu.lib.cleo_run_steps_graph(g.h, 0, 100, 20) == 0
+    for step in range(20):
+        assert o.run_step(step * 100, (step + 1) * 100) == 0
+    assert g.error()[0] == 0 and o.error()[0] == 0
+    assert_supers_match(g.download(), o.download(), what="20 steps")
+    assert np.array_equal(g.refs(), o.refs())
+    assert_state_match(g.state(), o.state(), what="20 steps")
+    sg, so = g.stats(), o.stats()
+    for k in ("cond_drops", "coll_pairs", "coll_events", "nulls", "motion_drops"):
+        assert sg[k] == so[k], k
+
+
+def test_coupled_large_domain(gpu, oracle):
+    """A 64 x 64 x 32 slab of the coupled workload (8.4M drops), two steps against the oracle."""
+    _say("case: 2 coupled steps, 64x64x32 gridboxes (8.4M drops), autotuned tiles")
+    pop = P.coupled(SEED + 12, ndims=(32, 64, 64), mean=64)
+    g, o = _both(gpu, oracle, pop)
+    s = sdm_params(cond=cond_params(), coll=coll_params(), motion=motion_params())
+    for d in (g, o):
+        d.set_sdm(s)
+        d.rng_counter(41)
+    tiles = abi.Tiles()
+    assert gpu.lib.cleo_autotune(g.h, abi.ctypes_byref(tiles)) == 0
+    for step in range(2):
+        assert g.run_step(step * 100, (step + 1) * 100) == 0
+        assert o.run_step(step * 100, (step + 1) * 100) == 0
+    assert g.error()[0] == 0 and o.error()[0] == 0
+    assert_supers_match(g.download(), o.download(), what="large domain")
+    assert np.array_equal(g.refs(), o.refs())
+    assert_state_match(g.state(), o.state(), what="large domain")
+
+
+COND_TILINGS = [(0, 0, 0), (0, 8192, 0), (1, 0, 0), (1, 32768, 6), (1, 8192, 12), (2, 0, 0), (2, 8192, 0),
+                (2, 32768, 0)]
+
+
+def test_condensation_workload_tilings(gpu, oracle):
+    """The condensation workload's population (1M drops) stepped 3 times by condensation alone under
+    every condensation variant, grid and deferral budget, and after the autotuner, against the
+    oracle."""
+    pop = P.condensation(SEED + 13, ndims=(32, 32, 8), mean=122)
+    o = make_domain(oracle, pop)
+    s = sdm_params(cond=cond_params())
+    o.set_sdm(s)
+    for step in range(3):
+        assert o.run_step(step * 100, (step + 1) * 100) == 0
+    want, want_state = o.download(), o.state()
+    for tiling in COND_TILINGS + ["autotune"]:
+        _say(f"case: condensation workload 1M drops, 3 steps, tiling {tiling}")
+        g = make_domain(gpu, pop)
+        g.set_sdm(s)
+        if tiling == "autotune":
+            assert g.run_step(0, 100) == 0
+            tiles = abi.Tiles()
+            assert gpu.lib.cleo_autotune(g.h, abi.ctypes_byref(tiles)) == 0
+            _say(f"  autotuned {[getattr(tiles, f) for f, _ in tiles._fields_]}")
+            first = 1
+        else:
+            flat, gw, defer = tiling
+            t = abi.Tiles(1, 1, 1, flat, 0, gw, 0, 0, 0, defer)
+            assert gpu.lib.cleo_set_tiles(g.h, abi.ctypes_byref(t)) == 0
+            first = 0
+        for step in range(first, 3):
+            assert g.run_step(step * 100, (step + 1) * 100) == 0, (tiling, g.error())
+        assert g.error()[0] == 0, (tiling, g.error())
+        assert_supers_match(g.download(), want, what=f"tiling {tiling}")
+        assert_state_match(g.state(), want_state, what=f"tiling {tiling}")
+        g.close()
