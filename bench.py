@@ -334,6 +334,8 @@ def _device_domain(torch, b, arrs, st, halves, ndims, solute, seed=20261202):
     from cleo_amd import abi
     from cleo_amd import populations as P
     L = b.lib
+    # the library copies on its own stream: the generator's kernels on torch's stream finish first
+    torch.cuda.synchronize()
     n = int(arrs["xi"].numel())
     soa = abi.SupersSoA()
     soa.sdgbxindex = _ptr(arrs["sdgbxindex"], C.c_uint32)

@@ -118,6 +118,7 @@ CondArgs cond_args(cleo_ctx* c, const cleo_cond_params& p, bool fuse_qcond) {
   a.niters = c->rec_niters ? c->niters : nullptr;
   a.warm = nullptr;
   a.warm_limit = p.warm_restart_iters;
+  a.gs = c->gs;
   return a;
 }
 
@@ -429,11 +430,13 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   c->sp_slots = std::max<uint64_t>(ngbx + 4, 4 * ((maxn + 255) / 256) + 4);
   HIP_TRY(c, dalloc(&c->sp, kStatSlotsMax * c->sp_slots));
   // deferred drops of the condensation: per-wave segments (waves of a drop launch times a quarter
-  // of their drops + 16), the shared list they move to (as large) and its counter
+  // of their drops + 16), the shared list they move to and its counter.  The list has room for
+  // every drop (a launch defers each drop at most once), so no deferred drop is ever solved in
+  // the launch that deferred it.
   HIP_TRY(c, dalloc(&c->dcount, 1));
-  c->dlist_cap = static_cast<uint32_t>(std::min<uint64_t>(maxn / 4 + 20 * c->sp_slots + 64, 0xFFFFFFFFull));
+  c->dlist_cap = static_cast<uint32_t>(std::min<uint64_t>(maxn + 64, 0xFFFFFFFFull));
   HIP_TRY(c, dalloc(&c->dlist, c->dlist_cap));
-  HIP_TRY(c, dalloc(&c->dseg, c->dlist_cap));
+  HIP_TRY(c, dalloc(&c->dseg, std::min<uint64_t>(maxn / 4 + 20 * c->sp_slots + 64, 0xFFFFFFFFull)));
   HIP_TRY(c, dalloc(&c->perm, std::max<uint64_t>(maxn, 1)));
   HIP_TRY(c, dalloc(&c->dstep, std::max<uint64_t>(maxn, 1)));
   HIP_TRY(c, dalloc(&c->order, std::max<uint64_t>(maxn, 1)));
@@ -983,6 +986,29 @@ int cleo_download_guess(cleo_ctx* c, double* out, uint64_t n) {
   if (!c->guess) return fail(c, CLEO_E_STATE, "guess recording not enabled");
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   HIP_TRY(c, ctx_copy(c, out, c->guess, std::min<uint64_t>(n, c->maxn) * 8, hipMemcpyDeviceToHost));
+  return CLEO_OK;
+}
+
+int cleo_thermo_eval(cleo_ctx* c, uint64_t n, const double* temp, const double* press, const double* qvap,
+                     double* psat, double* s_ratio, double* ffactor) {
+  if (!c || (n && (!temp || !press || !qvap || !psat || !s_ratio || !ffactor))) return CLEO_E_ARG;
+  if (n == 0) return CLEO_OK;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  double* d = nullptr;
+  HIP_TRY(c, dalloc(&d, 6 * n));
+  const size_t b = n * 8;
+  hipError_t e = ctx_copy(c, d, temp, b, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = ctx_copy(c, d + n, press, b, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = ctx_copy(c, d + 2 * n, qvap, b, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    launch_thermo_eval(c, n, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = ctx_copy(c, psat, d + 3 * n, b, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = ctx_copy(c, s_ratio, d + 4 * n, b, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = ctx_copy(c, ffactor, d + 5 * n, b, hipMemcpyDeviceToHost);
+  hipFree(d);
+  HIP_TRY(c, e);
   return CLEO_OK;
 }
 

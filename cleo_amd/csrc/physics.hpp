@@ -152,6 +152,57 @@ __device__ __forceinline__ bool radius_below_dry(double r, double msol, const So
   return r - dryradius(msol, s) <= -1e-12 / dimless::R0;
 }
 
+// ---- correctly rounded exp ------------------------------------------------------------------
+// Double-double values hi + lo (|lo| <= ulp(hi) / 2) and their error-free sums and products.
+struct DD {
+  double hi, lo;
+};
+__device__ __forceinline__ DD dd_norm(double a, double b) {  // |a| >= |b|
+  const double s = a + b;
+  return DD{s, b - (s - a)};
+}
+__device__ __forceinline__ DD dd_add_d(DD a, double b) {  // a + b for any magnitudes
+  const double s = a.hi + b;
+  const double bb = s - a.hi;
+  const double e = (a.hi - (s - bb)) + (b - bb);
+  return dd_norm(s, e + a.lo);
+}
+__device__ __forceinline__ DD dd_mul(DD a, DD b) {
+  const double p = a.hi * b.hi;
+  const double e = fma(a.hi, b.hi, -p) + (a.hi * b.lo + a.lo * b.hi);
+  return dd_norm(p, e);
+}
+__device__ __forceinline__ DD dd_div_n(DD a, double n) {  // a / n for a small integer n
+  const double q1 = a.hi / n;
+  const double rem = fma(-q1, n, a.hi);  // a.hi - q1 n exactly
+  return dd_norm(q1, (rem + a.lo) / n);
+}
+// e^x rounded to nearest, for the arguments of saturation_pressure (|x| well below 700): x = k ln2
+// + r with ln2 in two parts (k ln2_hi exact through an FMA), e^(r / 256) by its Taylor series to
+// the 10th term in double-double arithmetic (|r / 256| < 1.4e-3, remainder below 2^-110), squared 8
+// times (relative error about 2^-95), times 2^k.  The result is the correctly rounded e^x unless
+// e^x lies within ~2^-95 relative of a midpoint between doubles.  glibc's exp errs by at most about
+// 0.51 ulp, so it too returns the correctly rounded value except where e^x lies within about 0.01
+// ulp of a midpoint, and the two agree bit for bit outside those arguments; the device library's
+// exp is accurate to about an ulp instead.
+__device__ __forceinline__ double exp_rn(double x) {
+#pragma clang fp contract(off)
+  constexpr double kLn2Hi = 6.93147180559945286e-01, kLn2Lo = 2.31904681384629956e-17;
+  constexpr double kInvLn2 = 1.4426950408889634;
+  const double k = rint(x * kInvLn2);
+  const double p = k * kLn2Hi;
+  const double pe = fma(k, kLn2Hi, -p);  // k ln2_hi - p exactly
+  DD r = dd_add_d(DD{x - p, 0.0}, -(pe + k * kLn2Lo));  // x - p is exact (Sterbenz) for k != 0
+  r.hi = ldexp(r.hi, -8);
+  r.lo = ldexp(r.lo, -8);
+  DD t{1.0, 0.0};
+#pragma nounroll
+  for (int n = 10; n >= 1; --n) t = dd_add_d(dd_div_n(dd_mul(t, r), static_cast<double>(n)), 1.0);
+#pragma nounroll
+  for (int i = 0; i < 8; ++i) t = dd_mul(t, t);
+  return ldexp(t.hi + t.lo, static_cast<int>(k));
+}
+
 // ---- thermodynamics: thermodynamic_equations.{hpp,cpp} -------------------------------------
 __device__ __forceinline__ double dry_air_density(double press, double temp, double qvap) {
   const double rgas_equiv = dimless::Rgas_dry + dimless::Rgas_v * qvap;
@@ -166,7 +217,7 @@ __device__ __forceinline__ double supersaturation_ratio(double press, double qva
 __device__ __forceinline__ double saturation_pressure(double temp) {
   constexpr double A = 17.4146, B = 33.639, TREF = 273.16, PREF = 611.655;
   const double T = temp * dimless::TEMP0;
-  return (PREF * exp(A * (T - TREF) / (T - B))) / dimless::P0;
+  return (PREF * exp_rn(A * (T - TREF) / (T - B))) / dimless::P0;
 }
 __device__ __forceinline__ double diffusion_factor(double press, double temp, double psat) {
   constexpr double A = 7.11756e-5;
@@ -258,58 +309,96 @@ __device__ __forceinline__ void ode_g_dg(const ODEConsts& c, double subdelt, dou
   const double alpha_d = a_r - 3.0 * b_r3;
   dg = 1 - alpha_d * beta_d;
 }
+// g(z) exactly as the reference spells it (impliciteuler.cpp:270-281): square root and divisions
+// rounded as IEEE operations, no contraction, pow(r, 3.0) as the correctly rounded cube.  Used only
+// to decide a convergence test that the fast evaluation leaves too close to call.
+__device__ __forceinline__ double ode_g_reference(const ODEConsts& c, double subdelt, double rprev, double rsqrd) {
+#pragma clang fp contract(off)
+  const double radius = sqrt(rsqrd);
+  const double alpha = c.s_ratio - 1 - c.akoh / radius + c.bkoh / cube_rn(radius);
+  const double beta = 2.0 * subdelt / (rsqrd * c.ffactor_fv);
+  const double q = rprev / radius;
+  const double gamma = q * q;
+  return 1 - gamma - alpha * beta;
+}
+// check_for_convergence (impliciteuler.hpp:214-219) after the update z0 -> z1, from the reference's g
+__device__ __forceinline__ bool newton_converged_reference(double rtol, double atol, const ODEConsts& c,
+                                                           double subdelt, double rprev, double z0, double z1) {
+  const double r0 = ode_g_reference(c, subdelt, rprev, z0), r1 = ode_g_reference(c, subdelt, rprev, z1);
+  return fabs(r1 - r0) < rtol * fabs(r1) + atol;
+}
+// check_for_convergence (impliciteuler.hpp:214-219) after the update z0 -> z1, with g0, g1 from the
+// fast evaluation: 1 converged, 0 not.  The fast g is within a few ulp of the reference's, so the
+// outcome is the same unless the two sides of the test lie within 1e-8 relative of each other;
+// inside that band kExact decides from the reference's expression of g, and a solve without
+// kExact returns -1 (its caller defers the drop to a solve with kExact).
+template <bool kExact>
+__device__ __forceinline__ int newton_converged(double rtol, double atol, const ODEConsts& c, double subdelt,
+                                                double rprev, double z0, double g0, double z1, double g1) {
+  const double converged = rtol * fabs(g1) + atol;
+  const double current = fabs(g1 - g0);
+  if (fabs(current - converged) > 1e-8 * converged) return current < converged ? 1 : 0;
+  if constexpr (!kExact)
+    return -1;
+  else
+    return newton_converged_reference(rtol, atol, c, subdelt, rprev, z0, z1) ? 1 : 0;
+}
 // Newton state: z with g(z) and g'(z) already evaluated
 struct NewtonState {
   double z, g, dg;
 };
-// one Newton update + convergence test (impliciteuler.cpp:240-255, impliciteuler.hpp:214-219).
-// The reference evaluates g at the new z for the test and again as the next update's numerator;
-// both are the same expression of the same operands, so the value is carried over.
-__device__ __forceinline__ bool newton_iterate(const NewtonCfg& cfg, const ODEConsts& c, double subdelt,
-                                               double rprev, NewtonState& s, uint32_t& iters) {
+// one Newton update + convergence test (impliciteuler.cpp:240-255, impliciteuler.hpp:214-219):
+// 1 converged, 0 not, -1 undecided (kExact false only).  The reference evaluates g at the new z for
+// the test and again as the next update's numerator; both are the same expression of the same
+// operands, so the value is carried over.
+template <bool kExact>
+__device__ __forceinline__ int newton_iterate(const NewtonCfg& cfg, const ODEConsts& c, double subdelt,
+                                              double rprev, NewtonState& s, uint32_t& iters) {
   ++iters;
   double z = s.z * (1 - quotient(s.g, s.dg));
   z = fmax(z, 1e-8);
   double g1, dg1;
   ode_g_dg(c, subdelt, rprev, z, g1, dg1);
-  const double converged = cfg.rtol * fabs(g1) + cfg.atol;
-  const double current = fabs(g1 - s.g);
+  const int conv = newton_converged<kExact>(cfg.rtol, cfg.atol, c, subdelt, rprev, s.z, s.g, z, g1);
   s = NewtonState{z, g1, dg1};
-  return current < converged;
+  return conv;
 }
-// newtonraphson_untilconverged (impliciteuler.cpp:207-227): false when `limit` is exceeded
-// With a budget, the loop also stops once `iters` reaches it and flags the solve as deferred (the
-// caller abandons it; the drop is solved again without a budget).
-__device__ __forceinline__ bool newton_until(const NewtonCfg& cfg, const ODEConsts& c, uint32_t limit,
-                                             double subdelt, double rprev, NewtonState& s, uint32_t& iters,
-                                             bool& deferred) {
-  uint32_t niter = 1;
-  bool conv = false;
-  while (!conv) {
-    if (niter > limit) return false;
-    if (cfg.budget && iters >= cfg.budget) {
-      deferred = true;
-      return false;
-    }
-    conv = newton_iterate(cfg, c, subdelt, rprev, s, iters);
-    niter += 1;
+// outcome of a run of Newton iterations: converged, `limit` exceeded, or abandoned as deferred
+// (cfg.budget reached, or without kExact a convergence test too close to call; the drop is then
+// solved again from the start, without a budget and with kExact).  Returned by value, not through
+// flags passed by reference, which the compiler kept in scratch memory.
+enum : int { kNewtonDeferred = -1, kNewtonLimit = 0, kNewtonConverged = 1 };
+// newtonraphson_untilconverged (impliciteuler.cpp:207-227)
+template <bool kExact>
+__device__ __forceinline__ int newton_until(const NewtonCfg& cfg, const ODEConsts& c, uint32_t limit, double subdelt,
+                                            double rprev, NewtonState& s, uint32_t& iters) {
+  for (uint32_t niter = 1;; ++niter) {
+    if (niter > limit) return kNewtonLimit;
+    if (cfg.budget && iters >= cfg.budget) return kNewtonDeferred;
+    const int conv = newton_iterate<kExact>(cfg, c, subdelt, rprev, s, iters);
+    if (conv != 0) return conv < 0 ? kNewtonDeferred : kNewtonConverged;
   }
-  return true;
 }
 // integrate_condensation_ode (impliciteuler.cpp:128-141): 2 iterations, then restart from the guess
-__device__ __forceinline__ double newton_integrate(const NewtonCfg& cfg, const ODEConsts& c, double subdelt,
-                                                   double rprev, double zguess, uint32_t& iters,
-                                                   bool& maxed, bool& deferred) {
+// (only the second iteration's test decides); the new z and the outcome
+struct NewtonRun {
+  double z;
+  int status;
+};
+template <bool kExact>
+__device__ __forceinline__ NewtonRun newton_integrate(const NewtonCfg& cfg, const ODEConsts& c, double subdelt,
+                                                      double rprev, double zguess, uint32_t& iters) {
   NewtonState s0;
   s0.z = zguess;
   ode_g_dg(c, subdelt, rprev, zguess, s0.g, s0.dg);
   NewtonState s = s0;
-  bool conv = newton_iterate(cfg, c, subdelt, rprev, s, iters);
-  conv = newton_iterate(cfg, c, subdelt, rprev, s, iters);
-  if (conv) return s.z;
+  (void)newton_iterate<kExact>(cfg, c, subdelt, rprev, s, iters);
+  const int conv = newton_iterate<kExact>(cfg, c, subdelt, rprev, s, iters);
+  if (conv < 0) return NewtonRun{0.0, kNewtonDeferred};
+  if (conv == 1) return NewtonRun{s.z, kNewtonConverged};
   s = s0;
-  if (!newton_until(cfg, c, cfg.maxniters, subdelt, rprev, s, iters, deferred) && !deferred) maxed = true;
-  return s.z;
+  const int st = newton_until<kExact>(cfg, c, cfg.maxniters, subdelt, rprev, s, iters);
+  return NewtonRun{s.z, st};
 }
 __device__ __forceinline__ double initialguess(const ODEConsts& c, double rprev) {
   const double s_act = 1 + sqrt(4.0 * cube_rn(c.akoh) / 27 / c.bkoh);
@@ -388,6 +477,7 @@ struct SolveOut {
 // Learned path (no reference counterpart): the reference's Newton update and convergence test
 // iterated from `warm` until converged, at most `warm_limit` iterations; if it has not converged
 // by then, the reference path from the plain guess.
+template <bool kExact = true>
 __device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, double s_ratio, double akoh,
                                                      double bkoh, double ffactor, double rprev, double warm,
                                                      uint32_t warm_limit, SolveOut& o) {
@@ -397,34 +487,51 @@ __device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, doubl
   bool ucrit;
   ode_setup(c, rprev, cfg.delt, ziter, ucrit);
   double rsqrd;
+  NewtonRun run{0.0, kNewtonConverged};
   if (ucrit) {
+    bool plain = true;
     if (warm == warm) {
       NewtonState s;
       s.z = warm;
       ode_g_dg(c, cfg.delt, rprev, warm, s.g, s.dg);
-      if (newton_until(cfg, c, warm_limit, cfg.delt, rprev, s, o.iters, o.deferred)) {
-        rsqrd = s.z;
-      } else {
-        if (o.deferred) return 0.0;
-        o.restarted = true;
-        rsqrd = newton_integrate(cfg, c, cfg.delt, rprev, ziter, o.iters, o.maxed, o.deferred);
+      const int st = newton_until<kExact>(cfg, c, warm_limit, cfg.delt, rprev, s, o.iters);
+      if (st == kNewtonDeferred) {
+        o.deferred = true;
+        return 0.0;
       }
-    } else {
-      rsqrd = newton_integrate(cfg, c, cfg.delt, rprev, ziter, o.iters, o.maxed, o.deferred);
+      plain = st != kNewtonConverged;
+      o.restarted = plain;
+      run.z = s.z;
     }
+    if (plain) {
+      run = newton_integrate<kExact>(cfg, c, cfg.delt, rprev, ziter, o.iters);
+      if (run.status == kNewtonDeferred) {
+        o.deferred = true;
+        return 0.0;
+      }
+      o.maxed = run.status == kNewtonLimit;
+    }
+    rsqrd = run.z;
   } else {
     o.substep = true;
     const double critdelt = critical_timestep(c);
     const double mindelt = fmax(critdelt, cfg.minsubdelt);
     double remdelt = cfg.delt;
     double rp = rprev;
+    bool maxed = false;
     while (remdelt > 0.0) {
       const double subdelt = fmin(mindelt, remdelt);
-      ziter = newton_integrate(cfg, c, subdelt, rp, ziter, o.iters, o.maxed, o.deferred);
-      if (o.deferred) return 0.0;
+      run = newton_integrate<kExact>(cfg, c, subdelt, rp, ziter, o.iters);
+      if (run.status == kNewtonDeferred) {
+        o.deferred = true;
+        return 0.0;
+      }
+      maxed = maxed || run.status == kNewtonLimit;
+      ziter = run.z;
       rp = sqrt(ziter);  // pow(ziter, 0.5)
       remdelt -= subdelt;
     }
+    o.maxed = maxed;
     rsqrd = ziter;
   }
   return sqrt(rsqrd);

@@ -123,9 +123,7 @@ __device__ __forceinline__ void cond_gbx(const CondArgs& a, uint32_t g, int lane
     qc = rho_cond / rho_dry;
   }
   if (temp <= 0 && lane == 0) set_error(a.ctl, CLEO_ERR_TEMP_NONPOSITIVE, g, base);
-  const double psat = saturation_pressure(temp);
-  const double s_ratio = supersaturation_ratio(press, qv, psat);
-  const double ffactor = diffusion_factor(press, temp, psat);
+  const double s_ratio = a.gs[2 * g], ffactor = a.gs[2 * g + 1];  // from k_gbx_thermo
 
   double tot = 0.0;
   uint32_t iters = 0, subs = 0, restarts = 0;
@@ -1490,9 +1488,21 @@ __global__ void k_set_u32(uint32_t* out, uint32_t v, const uint32_t* flag) {
   *out = v;
 }
 
+// supersaturation ratio and diffusion factor of every gridbox (thread per gridbox) for
+// k_condensation: the correctly rounded exp of the saturation pressure, evaluated here, would
+// otherwise cost the per-gridbox solve kernel its registers
+__global__ void k_gbx_thermo(GState st, uint32_t ngbx, double* gs) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ngbx; g += gridDim.x * blockDim.x) {
+    const double temp = st.temp[g], press = st.press[g];
+    const double psat = saturation_pressure(temp);
+    gs[2 * g] = supersaturation_ratio(press, st.qv[g], psat);
+    gs[2 * g + 1] = diffusion_factor(press, temp, psat);
+  }
+}
 void launch_condensation(cleo_ctx* c, const CondArgs& args) {
   if (args.ngbx == 0) return;
   CondArgs a = args;
+  hipLaunchKernelGGL(k_gbx_thermo, dim3(stride_blocks(a.ngbx)), dim3(256), 0, c->stream, a.st, a.ngbx, a.gs);
   const unsigned nb = gbx_blocks(a.ngbx, a.wpb, c->tiles.cond_grid_waves);
   a.stat = stat_slots(c, static_cast<uint64_t>(nb) * a.wpb);
   kev_mark(c, 0);
@@ -1859,8 +1869,9 @@ struct CondStats {
 // factors, Newton, new radius rnew, condensed-mass change * xi (dm) and condensed mass * xi before
 // the solve (qm, the drop's term of the qcond recompute).  Errors and the iteration count are
 // recorded under position k; xi() gives the multiplicity (read after the solve).  Returns false,
-// with nothing recorded, when cfg.budget ran out: the drop is solved again later from the same r.
-template <class Xi>
+// with nothing recorded, when the solve was deferred (cfg.budget ran out, or without kExact a
+// convergence test too close to call): the drop is solved again later from the same r.
+template <bool kExact, class Xi>
 __device__ __forceinline__ bool cond_solve(const CondFlatArgs& a, const NewtonCfg& cfg, uint32_t g, uint64_t k, double r,
                                            double msol, Xi xi_of, double warm, CondStats& st, double& rnew, double& dm,
                                            double& qm) {
@@ -1869,7 +1880,7 @@ __device__ __forceinline__ bool cond_solve(const CondFlatArgs& a, const NewtonCf
   double akoh, bkoh;
   kohler_factors(msol, temp, a.sol, akoh, bkoh);
   SolveOut o{0, false, false, false, false};
-  const double newr = solve_condensation(cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
+  const double newr = solve_condensation<kExact>(cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
   if (o.deferred) return false;
   bool bad = false;
   const double old_m = condensate_mass(r, msol, a.sol, &bad);
@@ -1890,7 +1901,7 @@ __device__ __forceinline__ bool cond_solve(const CondFlatArgs& a, const NewtonCf
 __device__ __forceinline__ bool cond_drop(const CondFlatArgs& a, const NewtonCfg& cfg, uint32_t g, uint64_t k,
                                           double warm, CondStats& st) {
   double rnew, dm, qm;
-  if (!cond_solve(a, cfg, g, k, a.s.r[k], a.s.msol[k], [&] { return a.s.xi[k]; }, warm, st, rnew, dm, qm))
+  if (!cond_solve<true>(a, cfg, g, k, a.s.r[k], a.s.msol[k], [&] { return a.s.xi[k]; }, warm, st, rnew, dm, qm))
     return false;
   a.s.r[k] = rnew;
   a.dm[k] = dm;
@@ -1948,19 +1959,27 @@ __global__ void __launch_bounds__(256) k_cond_guess(CondFlatArgs a) {
 }
 
 // Deferred drops of a wave are first listed in the wave's own segment of a.dseg (`dsegcap` each,
-// no contention); defer_drop appends those of the wave's deferring lanes (position q) and
-// returns false for a deferring lane whose drop found the segment full (the caller solves it at
-// once).  Every lane of the wave calls it.
-__device__ __forceinline__ bool defer_drop(const CondFlatArgs& a, uint32_t* seg, uint32_t& ndef, bool deferred,
+// no contention); defer_drop appends those of the wave's deferring lanes (position q), and a
+// deferring lane whose drop finds the segment full appends it to the shared list at once, with
+// an atomic of its own.  The shared list holds dcap >= ntot entries and a launch defers each drop
+// at most once, so every deferred drop is listed (an overflow would be recorded as a bounds
+// error).  Every lane of the wave calls it.
+__device__ __forceinline__ void defer_drop(const CondFlatArgs& a, uint32_t* seg, uint32_t& ndef, bool deferred,
                                            uint32_t q) {
   const uint64_t m = __ballot(deferred);
-  if (!m) return true;
+  if (!m) return;
   const uint32_t idx = ndef + static_cast<uint32_t>(__popcll(m & lanemask_lt()));
   ndef = min(a.dsegcap, ndef + static_cast<uint32_t>(__popcll(m)));
-  if (!deferred) return true;
-  if (idx >= a.dsegcap) return false;
-  seg[idx] = q;
-  return true;
+  if (!deferred) return;
+  if (idx < a.dsegcap) {
+    seg[idx] = q;
+    return;
+  }
+  const uint32_t j = atomicAdd(a.dcount, 1u);
+  if (j < a.dcap)
+    a.dlist[j] = q;
+  else
+    set_error(a.ctl, CLEO_ERR_BOUNDS, kOOB, q);
 }
 // At the end of the kernel the wave moves its segment to the shared list with one atomic, so that
 // k_cond_deferred finds the deferred drops of all waves packed 64 to a wave.  The shared list has
@@ -1986,8 +2005,6 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_drops(CondFlatAr
   CondStats st{0, 0, 0, 0};
   uint32_t ndef = 0;
   uint32_t* seg = a.dseg + static_cast<uint64_t>(wave_index()) * a.dsegcap;
-  NewtonCfg nobudget = a.cfg;
-  nobudget.budget = 0;
   for (uint64_t k0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x; k0 < ndom;
        k0 += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     const uint64_t k = k0 + threadIdx.x;
@@ -2001,7 +2018,7 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_drops(CondFlatAr
     if (active && a.warm) warm = a.warm[k];
     bool deferred = false;
     if (active) deferred = !cond_drop(a, a.cfg, g, k, warm, st);
-    if (!defer_drop(a, seg, ndef, deferred, static_cast<uint32_t>(k))) cond_drop(a, nobudget, g, k, warm, st);
+    defer_drop(a, seg, ndef, deferred, static_cast<uint32_t>(k));
   }
   if (a.dcount) defer_flush(a, seg, ndef, lane);
   put_cond_stats(a.stat, st);
@@ -2053,8 +2070,6 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_sort_gather_cond(Nsor
   CondStats st{0, 0, 0, 0};
   uint32_t ndef = 0;
   uint32_t* seg = a.dseg + static_cast<uint64_t>(wave_index()) * a.dsegcap;
-  NewtonCfg nobudget = a.cfg;
-  nobudget.budget = 0;
   const double nowarm = __longlong_as_double(0x7FF8000000000000ll);
   // Each block takes one contiguous range of destinations, in block-uniform chunks (every wave
   // of the block stays in the loop for the ballots below).  A drop that changed gridbox is read
@@ -2089,23 +2104,24 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_sort_gather_cond(Nsor
     const double warm = a.warm ? a.warm[sp] : nowarm;  // learned guesses by source position
     double rnew = r, dm = 0.0, qm = 0.0;
     bool solved = false;
-    if (active) solved = cond_solve(a, a.cfg, g, q, r, msol, [&] { return xi; }, warm, st, rnew, dm, qm);
-    const bool deferred = active && !solved;
-    if (!defer_drop(a, seg, ndef, deferred, static_cast<uint32_t>(q)))
-      solved = cond_solve(a, nobudget, g, q, r, msol, [&] { return xi; }, warm, st, rnew, dm, qm);
+    if (active) solved = cond_solve<false>(a, a.cfg, g, q, r, msol, [&] { return xi; }, warm, st, rnew, dm, qm);
+    defer_drop(a, seg, ndef, active && !solved, static_cast<uint32_t>(q));
+    // non-temporal stores: the sorted store and dm, qm are read back only after the whole pass,
+    // from HBM either way, and streaming them past the L2 leaves it to the source lines that
+    // neighbouring destinations still read (5.31 -> 5.11 ms)
     if (okp) {
-      n.dst.gbx[q] = g;
-      n.dst.c3[q] = c3;
-      n.dst.c1[q] = c1;
-      n.dst.c2[q] = c2;
-      n.dst.xi[q] = xi;
-      n.dst.r[q] = solved ? rnew : r;
-      n.dst.msol[q] = msol;
-      n.dst.id[q] = id;
+      __builtin_nontemporal_store(g, &n.dst.gbx[q]);
+      __builtin_nontemporal_store(c3, &n.dst.c3[q]);
+      __builtin_nontemporal_store(c1, &n.dst.c1[q]);
+      __builtin_nontemporal_store(c2, &n.dst.c2[q]);
+      __builtin_nontemporal_store(xi, &n.dst.xi[q]);
+      __builtin_nontemporal_store(solved ? rnew : r, &n.dst.r[q]);
+      __builtin_nontemporal_store(msol, &n.dst.msol[q]);
+      __builtin_nontemporal_store(id, &n.dst.id[q]);
     }
     if (solved) {
-      a.dm[q] = dm;
-      if (a.qm) a.qm[q] = qm;
+      __builtin_nontemporal_store(dm, &a.dm[q]);
+      if (a.qm) __builtin_nontemporal_store(qm, &a.qm[q]);
     }
   }
   if (a.dcount) defer_flush(a, seg, ndef, lane);
@@ -2264,7 +2280,14 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_cond_queue(CondFlatAr
         }
       } else {
         ++iters;
-        const bool conv = fabs(g1 - gz) < cfg.rtol * fabs(g1) + cfg.atol;
+        int cv = newton_converged<false>(cfg.rtol, cfg.atol, c, subdelt, rprev, z, gz, zn, g1);
+        if (cv < 0) {  // too close to call: the reference's g, with ffactor / vent as it divides by it
+          ODEConsts cr = c;
+          double inv;
+          ffactor_over_vent(a.gs[2 * a.s.gbx[k] + 1], r0, cr.ffactor_fv, inv);
+          cv = newton_converged_reference(cfg.rtol, cfg.atol, cr, subdelt, rprev, z, zn) ? 1 : 0;
+        }
+        const bool conv = cv == 1;
         z = zn;
         gz = g1;
         dgz = dg1;
@@ -2521,6 +2544,24 @@ void launch_mass_moments(cleo_ctx* c, bool rain, uint64_t* mom0, float* mom1, fl
                        c->solute, c->ctl, mom0, mom1, mom2);
 }
 
+// the thermodynamic functions of k_cond_prep and k_condensation at n states, thread per state
+__global__ void k_thermo_eval(uint64_t n, const double* temp, const double* press, const double* qvap,
+                              double* psat, double* s_ratio, double* ffactor) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const double ps = saturation_pressure(temp[i]);
+    psat[i] = ps;
+    s_ratio[i] = supersaturation_ratio(press[i], qvap[i], ps);
+    ffactor[i] = diffusion_factor(press[i], temp[i], ps);
+  }
+}
+void launch_thermo_eval(cleo_ctx* c, uint64_t n, const double* temp, const double* press, const double* qvap,
+                        double* psat, double* s_ratio, double* ffactor) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_thermo_eval, dim3(stride_blocks(n)), dim3(256), 0, c->stream, n, temp, press, qvap, psat,
+                     s_ratio, ffactor);
+}
+
 // p[i] = i for i < n, skipped unless *flag when flag is non-null
 __global__ void k_iota_u32(uint32_t* p, uint64_t n, const uint32_t* flag) {
   if (flag && *flag == 0u) return;
@@ -2586,20 +2627,22 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_motion_drops(MotionAr
     const GbxGeom Ga = gbx_geom(b, ga), Gb = gbx_geom(b, gb);
     const Moved Ma = move_drop(b, ga, Ga, pa, la, acta);
     const Moved Mb = move_drop(b, gb, Gb, pb, lb, actb);
+    // non-temporal stores: the moved coordinates and steps are read back by the sort's passes,
+    // after the whole launch (2.27 -> 2.23 ms)
     if (kWrite && acta) {
-      a.s.c3[ka] = Ma.c3;
-      a.s.c1[ka] = Ma.c1;
-      a.s.c2[ka] = Ma.c2;
-      a.s.gbx[ka] = Ma.idx;
+      __builtin_nontemporal_store(Ma.c3, &a.s.c3[ka]);
+      __builtin_nontemporal_store(Ma.c1, &a.s.c1[ka]);
+      __builtin_nontemporal_store(Ma.c2, &a.s.c2[ka]);
+      __builtin_nontemporal_store(Ma.idx, &a.s.gbx[ka]);
     }
     if (kWrite && actb) {
-      a.s.c3[kb] = Mb.c3;
-      a.s.c1[kb] = Mb.c1;
-      a.s.c2[kb] = Mb.c2;
-      a.s.gbx[kb] = Mb.idx;
+      __builtin_nontemporal_store(Mb.c3, &a.s.c3[kb]);
+      __builtin_nontemporal_store(Mb.c1, &a.s.c1[kb]);
+      __builtin_nontemporal_store(Mb.c2, &a.s.c2[kb]);
+      __builtin_nontemporal_store(Mb.idx, &a.s.gbx[kb]);
     }
-    if (a.step && acta) a.step[ka] = static_cast<uint8_t>(Ma.dir);
-    if (a.step && actb) a.step[kb] = static_cast<uint8_t>(Mb.dir);
+    if (a.step && acta) __builtin_nontemporal_store(static_cast<uint8_t>(Ma.dir), &a.step[ka]);
+    if (a.step && actb) __builtin_nontemporal_store(static_cast<uint8_t>(Mb.dir), &a.step[kb]);
     nmoved += static_cast<uint32_t>(__popcll(__ballot(acta))) + static_cast<uint32_t>(__popcll(__ballot(actb)));
     count_keys(a.cnt, motion_key(Ma, ga), acta, lane);
     count_keys(a.cnt, motion_key(Mb, gb), actb, lane);
@@ -2773,13 +2816,14 @@ void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatAr
   const uint32_t gw = c->tiles.cond_grid_waves;
   const uint64_t ntiles = (ngbx + kWave - 1) / kWave;
   // a deferred drop is solved again by its sorted position k; its learned guess is that of source
-  // position perm[k]
+  // position perm[k].  The gather defers drops whose convergence test is too close to call from
+  // its fast evaluation of g (and, with a budget, those that outlive it), so the list is always on.
   const uint32_t budget = c->tiles.cond_defer;
   if (a.mlp_inline) a.warm_perm = n.perm;
   a.cfg.budget = budget;
-  a.dcap = budget ? c->dlist_cap : 0u;
+  a.dcap = c->dlist_cap;
   a.dlist = c->dlist;
-  a.dcount = budget ? c->dcount : nullptr;  // zeroed by k_cond_prep
+  a.dcount = c->dcount;  // zeroed by k_cond_prep
   hipLaunchKernelGGL(k_cond_prep, dim3(gbx_blocks(ntiles, 4, gw)), dim3(256), 0, c->stream, a);
   if (a.mlp_inline) {  // learned guesses of the drops where motion left them
     CondFlatArgs ga = a;
@@ -2795,10 +2839,8 @@ void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatAr
   kev_mark(c, 0);
   launch_gather_cond(c, n, a, move_in_gather ? &m : nullptr, nb);
   kev_mark(c, 1);
-  if (budget) {
-    launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
-    hipLaunchKernelGGL(k_cond_deferred, dim3(nb), dim3(256), 0, c->stream, a);
-  }
+  launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
+  hipLaunchKernelGGL(k_cond_deferred, dim3(nb), dim3(256), 0, c->stream, a);
   launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
   hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, n, c->ntot);
   hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, static_cast<const uint32_t*>(nullptr));

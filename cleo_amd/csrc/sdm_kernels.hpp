@@ -36,6 +36,7 @@ struct CondArgs {
   const double* warm;
   uint32_t warm_limit;
   StatSlots stat;  // ST_COND_DROPS .. ST_WARM_RESTARTS
+  double* gs;      // per gridbox: supersaturation ratio, diffusion factor (written by k_gbx_thermo)
 };
 
 struct QcondArgs {
@@ -193,6 +194,9 @@ void materialize_order(cleo_ctx* c);
 // 0th, 1st and 2nd mass moments of every gridbox into device arrays of ngbx entries (raindrops
 // only when rain)
 void launch_mass_moments(cleo_ctx* c, bool rain, uint64_t* mom0, float* mom1, float* mom2);
+// saturation pressure, supersaturation ratio and diffusion factor of n states (device arrays)
+void launch_thermo_eval(cleo_ctx* c, uint64_t n, const double* temp, const double* press, const double* qvap,
+                        double* psat, double* s_ratio, double* ffactor);
 // one coupling step of prescribed dynamics (device arrays) into every gridbox's State
 void launch_receive_dynamics(cleo_ctx* c, const cleo_dynamics_step& dyn);
 // the identity as the pending order (the store unchanged)

@@ -46,6 +46,9 @@ def load():
         lib.cleo_ndomain.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         lib.cleo_autotune.argtypes = [C.c_void_p, C.c_void_p]
         lib.cleo_time_kernel.argtypes = [C.c_void_p, C.c_int32, C.c_uint32, C.POINTER(C.c_float)]
+        dp = C.POINTER(C.c_double)
+        lib.cleo_thermo_eval.argtypes = [C.c_void_p, C.c_uint64, dp, dp, dp, dp, dp, dp]
+        lib.cleo_thermo_eval.restype = C.c_int
         _backend = b
     return _backend
 

@@ -306,6 +306,12 @@ int cleo_download_niters(cleo_ctx* ctx, uint32_t* niters, uint64_t n);
    condensation calls. */
 int cleo_record_guess(cleo_ctx* ctx, int32_t enable);
 int cleo_download_guess(cleo_ctx* ctx, double* z, uint64_t n);
+/* The device's thermodynamic functions at n gridbox states (dimensionless, host arrays):
+   saturation_pressure (thermodynamic_equations.cpp:39-52), supersaturation_ratio
+   (thermodynamic_equations.hpp:79-90) and diffusion_factor (thermodynamic_equations.cpp:94-114), as
+   the condensation evaluates them per gridbox. */
+int cleo_thermo_eval(cleo_ctx* ctx, uint64_t n, const double* temp, const double* press, const double* qvap,
+                     double* psat, double* s_ratio, double* ffactor);
 
 /* ---- CLEO binary initialisation files (host only: no context, no GPU) ---------------------- */
 /* The layout of initialise/readbinary.cpp:22-99 (written by cleopy/writebinary.py): 4 uint32
