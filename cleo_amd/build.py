@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libcleo_hip.so")
-SOURCES = ["sdm_kernels.hip", "capi.hip", "mlp.hip", "binfiles.hip"]
+SOURCES = ["sdm_kernels.hip", "capi.hip", "mlp.hip", "binfiles.hip", "noalloc.hip"]
 ARCH = os.environ.get("CLEO_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -313,6 +313,21 @@ int cleo_download_guess(cleo_ctx* ctx, double* z, uint64_t n);
 int cleo_thermo_eval(cleo_ctx* ctx, uint64_t n, const double* temp, const double* press, const double* qvap,
                      double* psat, double* s_ratio, double* ffactor);
 
+/* ---- without allocation ---------------------------------------------------------------- */
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc).  The sort that
+   cleo_upload_supers_device applies (stable by sdgbxindex, keys >= ngbx out of the domain after
+   every gridbox; SortSupersBySdgbxindex + set_gridboxes_refs, sortsupers.hpp:93-226), from the
+   device SoA `in` to the device SoA `out` (n drops each, not overlapping), with refs (device, ngbx+1
+   entries) the first position of each gridbox and refs[ngbx] the in-domain count, enqueued on
+   `stream` (a hipStream_t; NULL is the null stream) without synchronising.  No context, no
+   hipMalloc: the scratch is a fixed device array of the library (about 17 MB per GPU, held until the
+   process ends), so n <= CLEO_NOALLOC_MAX_SUPERS (CLEO_E_ARG beyond).  Calls that overlap on one GPU
+   share that array and race; calls in one stream order themselves. */
+#define CLEO_NOALLOC_MAX_SUPERS (1u << 20)
+int cleo_sort_supers_noalloc(const cleo_supers_soa* in, cleo_supers_soa* out, uint32_t* refs, uint64_t n,
+                             uint32_t ngbx, void* stream);
+
 /* ---- CLEO binary initialisation files (host only: no context, no GPU) ---------------------- */
 /* The layout of initialise/readbinary.cpp:22-99 (written by cleopy/writebinary.py): 4 uint32
    (first data byte, bytes of the global metadata string, number of variables, bytes of metadata

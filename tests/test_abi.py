@@ -41,6 +41,8 @@ def test_oracle_exports_mirror(oracle):
                                   "cleo_device_info", "cleo_upload_supers_aos", "cleo_download_supers_aos",
                                   "cleo_upload_supers_device", "cleo_run_steps_graph", "cleo_synchronize",
                                   "cleo_record_guess", "cleo_download_guess", "cleo_ndomain",
+                                  # no context: the sort cleo_upload_supers_device's parity covers
+                                  "cleo_sort_supers_noalloc",
                                   # device functions the oracle exports one by one (orc_saturation_pressure, ...)
                                   "cleo_thermo_eval",
                                   # host-only file readers: they compute nothing the oracle checks
