@@ -309,74 +309,36 @@ __device__ __forceinline__ void ode_g_dg(const ODEConsts& c, double subdelt, dou
   const double alpha_d = a_r - 3.0 * b_r3;
   dg = 1 - alpha_d * beta_d;
 }
-// g(z) exactly as the reference spells it (impliciteuler.cpp:270-281): square root and divisions
-// rounded as IEEE operations, no contraction, pow(r, 3.0) as the correctly rounded cube.  Used only
-// to decide a convergence test that the fast evaluation leaves too close to call.
-__device__ __forceinline__ double ode_g_reference(const ODEConsts& c, double subdelt, double rprev, double rsqrd) {
-#pragma clang fp contract(off)
-  const double radius = sqrt(rsqrd);
-  const double alpha = c.s_ratio - 1 - c.akoh / radius + c.bkoh / cube_rn(radius);
-  const double beta = 2.0 * subdelt / (rsqrd * c.ffactor_fv);
-  const double q = rprev / radius;
-  const double gamma = q * q;
-  return 1 - gamma - alpha * beta;
-}
-// check_for_convergence (impliciteuler.hpp:214-219) after the update z0 -> z1, from the reference's g
-__device__ __forceinline__ bool newton_converged_reference(double rtol, double atol, const ODEConsts& c,
-                                                           double subdelt, double rprev, double z0, double z1) {
-  const double r0 = ode_g_reference(c, subdelt, rprev, z0), r1 = ode_g_reference(c, subdelt, rprev, z1);
-  return fabs(r1 - r0) < rtol * fabs(r1) + atol;
-}
-// check_for_convergence (impliciteuler.hpp:214-219) after the update z0 -> z1, with g0, g1 from the
-// fast evaluation: 1 converged, 0 not.  The fast g is within a few ulp of the reference's, so the
-// outcome is the same unless the two sides of the test lie within 1e-8 relative of each other;
-// inside that band kExact decides from the reference's expression of g, and a solve without
-// kExact returns -1 (its caller defers the drop to a solve with kExact).
-template <bool kExact>
-__device__ __forceinline__ int newton_converged(double rtol, double atol, const ODEConsts& c, double subdelt,
-                                                double rprev, double z0, double g0, double z1, double g1) {
-  const double converged = rtol * fabs(g1) + atol;
-  const double current = fabs(g1 - g0);
-  if (fabs(current - converged) > 1e-8 * converged) return current < converged ? 1 : 0;
-  if constexpr (!kExact)
-    return -1;
-  else
-    return newton_converged_reference(rtol, atol, c, subdelt, rprev, z0, z1) ? 1 : 0;
-}
 // Newton state: z with g(z) and g'(z) already evaluated
 struct NewtonState {
   double z, g, dg;
 };
-// one Newton update + convergence test (impliciteuler.cpp:240-255, impliciteuler.hpp:214-219):
-// 1 converged, 0 not, -1 undecided (kExact false only).  The reference evaluates g at the new z for
-// the test and again as the next update's numerator; both are the same expression of the same
-// operands, so the value is carried over.
-template <bool kExact>
-__device__ __forceinline__ int newton_iterate(const NewtonCfg& cfg, const ODEConsts& c, double subdelt,
-                                              double rprev, NewtonState& s, uint32_t& iters) {
+// one Newton update + convergence test (impliciteuler.cpp:240-255, impliciteuler.hpp:214-219).
+// The reference evaluates g at the new z for the test and again as the next update's numerator;
+// both are the same expression of the same operands, so the value is carried over.
+__device__ __forceinline__ bool newton_iterate(const NewtonCfg& cfg, const ODEConsts& c, double subdelt,
+                                               double rprev, NewtonState& s, uint32_t& iters) {
   ++iters;
   double z = s.z * (1 - quotient(s.g, s.dg));
   z = fmax(z, 1e-8);
   double g1, dg1;
   ode_g_dg(c, subdelt, rprev, z, g1, dg1);
-  const int conv = newton_converged<kExact>(cfg.rtol, cfg.atol, c, subdelt, rprev, s.z, s.g, z, g1);
+  const double converged = cfg.rtol * fabs(g1) + cfg.atol;
+  const bool conv = fabs(g1 - s.g) < converged;
   s = NewtonState{z, g1, dg1};
   return conv;
 }
 // outcome of a run of Newton iterations: converged, `limit` exceeded, or abandoned as deferred
-// (cfg.budget reached, or without kExact a convergence test too close to call; the drop is then
-// solved again from the start, without a budget and with kExact).  Returned by value, not through
-// flags passed by reference, which the compiler kept in scratch memory.
+// (cfg.budget reached; the drop is then solved again from the start, without a budget).  Returned
+// by value, not through flags passed by reference, which the compiler kept in scratch memory.
 enum : int { kNewtonDeferred = -1, kNewtonLimit = 0, kNewtonConverged = 1 };
 // newtonraphson_untilconverged (impliciteuler.cpp:207-227)
-template <bool kExact>
 __device__ __forceinline__ int newton_until(const NewtonCfg& cfg, const ODEConsts& c, uint32_t limit, double subdelt,
                                             double rprev, NewtonState& s, uint32_t& iters) {
   for (uint32_t niter = 1;; ++niter) {
     if (niter > limit) return kNewtonLimit;
     if (cfg.budget && iters >= cfg.budget) return kNewtonDeferred;
-    const int conv = newton_iterate<kExact>(cfg, c, subdelt, rprev, s, iters);
-    if (conv != 0) return conv < 0 ? kNewtonDeferred : kNewtonConverged;
+    if (newton_iterate(cfg, c, subdelt, rprev, s, iters)) return kNewtonConverged;
   }
 }
 // integrate_condensation_ode (impliciteuler.cpp:128-141): 2 iterations, then restart from the guess
@@ -385,19 +347,16 @@ struct NewtonRun {
   double z;
   int status;
 };
-template <bool kExact>
 __device__ __forceinline__ NewtonRun newton_integrate(const NewtonCfg& cfg, const ODEConsts& c, double subdelt,
                                                       double rprev, double zguess, uint32_t& iters) {
   NewtonState s0;
   s0.z = zguess;
   ode_g_dg(c, subdelt, rprev, zguess, s0.g, s0.dg);
   NewtonState s = s0;
-  (void)newton_iterate<kExact>(cfg, c, subdelt, rprev, s, iters);
-  const int conv = newton_iterate<kExact>(cfg, c, subdelt, rprev, s, iters);
-  if (conv < 0) return NewtonRun{0.0, kNewtonDeferred};
-  if (conv == 1) return NewtonRun{s.z, kNewtonConverged};
+  (void)newton_iterate(cfg, c, subdelt, rprev, s, iters);
+  if (newton_iterate(cfg, c, subdelt, rprev, s, iters)) return NewtonRun{s.z, kNewtonConverged};
   s = s0;
-  const int st = newton_until<kExact>(cfg, c, cfg.maxniters, subdelt, rprev, s, iters);
+  const int st = newton_until(cfg, c, cfg.maxniters, subdelt, rprev, s, iters);
   return NewtonRun{s.z, st};
 }
 __device__ __forceinline__ double initialguess(const ODEConsts& c, double rprev) {
@@ -475,9 +434,10 @@ struct SolveOut {
 };
 // solve_condensation (impliciteuler.cpp:35-54). `warm` is a learned guess for z = r^2 or NaN.
 // Learned path (no reference counterpart): the reference's Newton update and convergence test
-// iterated from `warm` until converged, at most `warm_limit` iterations; if it has not converged
-// by then, the reference path from the plain guess.
-template <bool kExact = true>
+// iterated from `warm` until converged, at most `warm_limit` iterations, and if it has not
+// converged by then the reference path from the plain guess.  Where |g| at the plain guess is
+// below atol the plain guess is already the root to the tolerance (a drop at equilibrium) and the
+// reference path runs at once: its two iterations are the fewest any start takes there.
 __device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, double s_ratio, double akoh,
                                                      double bkoh, double ffactor, double rprev, double warm,
                                                      uint32_t warm_limit, SolveOut& o) {
@@ -491,10 +451,15 @@ __device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, doubl
   if (ucrit) {
     bool plain = true;
     if (warm == warm) {
+      double gp, dgp;
+      ode_g_dg(c, cfg.delt, rprev, ziter, gp, dgp);
+      if (fabs(gp) < cfg.atol) warm = __longlong_as_double(0x7FF8000000000000ll);
+    }
+    if (warm == warm) {
       NewtonState s;
       s.z = warm;
       ode_g_dg(c, cfg.delt, rprev, warm, s.g, s.dg);
-      const int st = newton_until<kExact>(cfg, c, warm_limit, cfg.delt, rprev, s, o.iters);
+      const int st = newton_until(cfg, c, warm_limit, cfg.delt, rprev, s, o.iters);
       if (st == kNewtonDeferred) {
         o.deferred = true;
         return 0.0;
@@ -504,7 +469,7 @@ __device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, doubl
       run.z = s.z;
     }
     if (plain) {
-      run = newton_integrate<kExact>(cfg, c, cfg.delt, rprev, ziter, o.iters);
+      run = newton_integrate(cfg, c, cfg.delt, rprev, ziter, o.iters);
       if (run.status == kNewtonDeferred) {
         o.deferred = true;
         return 0.0;
@@ -521,7 +486,7 @@ __device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, doubl
     bool maxed = false;
     while (remdelt > 0.0) {
       const double subdelt = fmin(mindelt, remdelt);
-      run = newton_integrate<kExact>(cfg, c, subdelt, rp, ziter, o.iters);
+      run = newton_integrate(cfg, c, subdelt, rp, ziter, o.iters);
       if (run.status == kNewtonDeferred) {
         o.deferred = true;
         return 0.0;

@@ -1869,9 +1869,8 @@ struct CondStats {
 // factors, Newton, new radius rnew, condensed-mass change * xi (dm) and condensed mass * xi before
 // the solve (qm, the drop's term of the qcond recompute).  Errors and the iteration count are
 // recorded under position k; xi() gives the multiplicity (read after the solve).  Returns false,
-// with nothing recorded, when the solve was deferred (cfg.budget ran out, or without kExact a
-// convergence test too close to call): the drop is solved again later from the same r.
-template <bool kExact, class Xi>
+// with nothing recorded, when cfg.budget ran out: the drop is solved again later from the same r.
+template <class Xi>
 __device__ __forceinline__ bool cond_solve(const CondFlatArgs& a, const NewtonCfg& cfg, uint32_t g, uint64_t k, double r,
                                            double msol, Xi xi_of, double warm, CondStats& st, double& rnew, double& dm,
                                            double& qm) {
@@ -1880,7 +1879,7 @@ __device__ __forceinline__ bool cond_solve(const CondFlatArgs& a, const NewtonCf
   double akoh, bkoh;
   kohler_factors(msol, temp, a.sol, akoh, bkoh);
   SolveOut o{0, false, false, false, false};
-  const double newr = solve_condensation<kExact>(cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
+  const double newr = solve_condensation(cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
   if (o.deferred) return false;
   bool bad = false;
   const double old_m = condensate_mass(r, msol, a.sol, &bad);
@@ -1901,7 +1900,7 @@ __device__ __forceinline__ bool cond_solve(const CondFlatArgs& a, const NewtonCf
 __device__ __forceinline__ bool cond_drop(const CondFlatArgs& a, const NewtonCfg& cfg, uint32_t g, uint64_t k,
                                           double warm, CondStats& st) {
   double rnew, dm, qm;
-  if (!cond_solve<true>(a, cfg, g, k, a.s.r[k], a.s.msol[k], [&] { return a.s.xi[k]; }, warm, st, rnew, dm, qm))
+  if (!cond_solve(a, cfg, g, k, a.s.r[k], a.s.msol[k], [&] { return a.s.xi[k]; }, warm, st, rnew, dm, qm))
     return false;
   a.s.r[k] = rnew;
   a.dm[k] = dm;
@@ -2104,7 +2103,7 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_sort_gather_cond(Nsor
     const double warm = a.warm ? a.warm[sp] : nowarm;  // learned guesses by source position
     double rnew = r, dm = 0.0, qm = 0.0;
     bool solved = false;
-    if (active) solved = cond_solve<false>(a, a.cfg, g, q, r, msol, [&] { return xi; }, warm, st, rnew, dm, qm);
+    if (active) solved = cond_solve(a, a.cfg, g, q, r, msol, [&] { return xi; }, warm, st, rnew, dm, qm);
     defer_drop(a, seg, ndef, active && !solved, static_cast<uint32_t>(q));
     // non-temporal stores: the sorted store and dm, qm are read back only after the whole pass,
     // from HBM either way, and streaming them past the L2 leaves it to the source lines that
@@ -2198,7 +2197,13 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_cond_queue(CondFlatAr
         R.inv_ffv[slot] = cc.inv_ffv;
         R.r0[slot] = r;
         R.zplain[slot] = zg;
-        R.zwarm[slot] = a.warm ? a.warm[kk] : __longlong_as_double(0x7FF8000000000000ll);
+        double zw = a.warm ? a.warm[kk] : __longlong_as_double(0x7FF8000000000000ll);
+        if (uniq && zw == zw) {  // no learned start where the plain guess is the root (solve_condensation)
+          double gp, dg;
+          ode_g_dg(cc, cfg.delt, r, zg, gp, dg);
+          if (fabs(gp) < cfg.atol) zw = __longlong_as_double(0x7FF8000000000000ll);
+        }
+        R.zwarm[slot] = zw;
         R.mindelt[slot] = uniq ? 0.0 : fmax(critdelt, cfg.minsubdelt);
         R.dryr[slot] = dryradius(ms, a.sol);
         R.oldm[slot] = om;
@@ -2280,14 +2285,7 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_cond_queue(CondFlatAr
         }
       } else {
         ++iters;
-        int cv = newton_converged<false>(cfg.rtol, cfg.atol, c, subdelt, rprev, z, gz, zn, g1);
-        if (cv < 0) {  // too close to call: the reference's g, with ffactor / vent as it divides by it
-          ODEConsts cr = c;
-          double inv;
-          ffactor_over_vent(a.gs[2 * a.s.gbx[k] + 1], r0, cr.ffactor_fv, inv);
-          cv = newton_converged_reference(cfg.rtol, cfg.atol, cr, subdelt, rprev, z, zn) ? 1 : 0;
-        }
-        const bool conv = cv == 1;
+        const bool conv = fabs(g1 - gz) < cfg.rtol * fabs(g1) + cfg.atol;
         z = zn;
         gz = g1;
         dgz = dg1;
@@ -2816,14 +2814,13 @@ void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatAr
   const uint32_t gw = c->tiles.cond_grid_waves;
   const uint64_t ntiles = (ngbx + kWave - 1) / kWave;
   // a deferred drop is solved again by its sorted position k; its learned guess is that of source
-  // position perm[k].  The gather defers drops whose convergence test is too close to call from
-  // its fast evaluation of g (and, with a budget, those that outlive it), so the list is always on.
+  // position perm[k]
   const uint32_t budget = c->tiles.cond_defer;
   if (a.mlp_inline) a.warm_perm = n.perm;
   a.cfg.budget = budget;
-  a.dcap = c->dlist_cap;
+  a.dcap = budget ? c->dlist_cap : 0u;
   a.dlist = c->dlist;
-  a.dcount = c->dcount;  // zeroed by k_cond_prep
+  a.dcount = budget ? c->dcount : nullptr;  // zeroed by k_cond_prep
   hipLaunchKernelGGL(k_cond_prep, dim3(gbx_blocks(ntiles, 4, gw)), dim3(256), 0, c->stream, a);
   if (a.mlp_inline) {  // learned guesses of the drops where motion left them
     CondFlatArgs ga = a;
@@ -2839,8 +2836,10 @@ void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatAr
   kev_mark(c, 0);
   launch_gather_cond(c, n, a, move_in_gather ? &m : nullptr, nb);
   kev_mark(c, 1);
-  launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
-  hipLaunchKernelGGL(k_cond_deferred, dim3(nb), dim3(256), 0, c->stream, a);
+  if (budget) {
+    launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
+    hipLaunchKernelGGL(k_cond_deferred, dim3(nb), dim3(256), 0, c->stream, a);
+  }
   launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
   hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, n, c->ntot);
   hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, static_cast<const uint32_t*>(nullptr));

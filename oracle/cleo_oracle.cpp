@@ -35,6 +35,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 #ifdef _OPENMP
@@ -337,7 +338,10 @@ static inline double solve_condensation(const NewtonCfg& cfg, double s_ratio, do
   const bool ucrit2 = (cfg.delt <= critical_timestep(c));
   double rsqrd = 0.0;
   if (ucrit1 || ucrit2) {
-    if (warm == warm) { /* learned guess */
+    /* learned guess, except where the plain guess is the root to atol (|g| below it there) */
+    if (warm == warm && std::abs(ode_gfunc(c, cfg.delt, rprev, ziter)) < cfg.atol)
+      warm = std::numeric_limits<double>::quiet_NaN();
+    if (warm == warm) {
       bool ok = true;
       rsqrd = until_converged(cfg, c, warm_limit, cfg.delt, rprev, warm, cnt, ok);
       if (!ok) {

@@ -267,6 +267,30 @@ def test_newton_warm_start_from_root_takes_at_most_two(oracle):
     assert bad == pytest.approx(plain, rel=1e-12) and nbad == 1 + nplain
 
 
+def test_learned_guess_skipped_where_plain_guess_is_the_root(oracle):
+    """The learned path starts from the learned guess except where |g| at the plain guess is below
+    atol.  A drop at its equilibrium radius, whose plain guess r^2 is the root, keeps the reference's
+    two iterations and its result bit for bit under a learned guess 1e-3 off; the same guess on a
+    drop far from its root is taken and needs fewer iterations than the plain one."""
+    temp = 285.0 / 273.15
+    a = 3.3e-7 / (273.15 * 1e-6) / temp
+    b = a * 0.2 * 0.2
+    s = 1 + 0.5 * math.sqrt(4 * a ** 3 / 27 / b)  # below activation
+    lo, hi = (b / a) ** 0.5 * 1.0001, math.sqrt(3 * b / a)  # S - 1 - a/r + b/r^3 changes sign inside
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if s - 1 - a / mid + b / mid ** 3 > 0 else (lo, mid)
+    req = 0.5 * (lo + hi)
+    F = _F(oracle, req)
+    plain, nplain, sub, _ = _solve(oracle, s, a, b, F, req)
+    w, nw, _, _ = _solve(oracle, s, a, b, F, req, warm=req * req * (1 + 1e-3))
+    assert sub == 0 and nplain == 2 and nw == 2 and w == plain
+    # from 3 r_eq the plain guess (r^2) is far from the root; a guess 1e-3 off it is taken
+    far, nfar, _, _ = _solve(oracle, s, a, b, F, 3 * req)
+    wf, nwf, _, _ = _solve(oracle, s, a, b, F, 3 * req, warm=far * far * (1 + 1e-3))
+    assert nfar > 3 and nwf < nfar and wf == pytest.approx(far, rel=1e-9)
+
+
 # ---------------------------------------------------------------------------------------------
 # domain-level: shuffle, sort, condensation feedback, motion
 
