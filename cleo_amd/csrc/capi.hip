@@ -415,7 +415,7 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   double* s = c->d_state;
   c->st = GState{s, s + ngbx, s + 2 * ngbx, s + 3 * ngbx, s + 4 * ngbx, s + 5 * ngbx,
                  s + 6 * ngbx, s + 7 * ngbx, s + 8 * ngbx, s + 9 * ngbx, s + 10 * ngbx};
-  HIP_TRY(c, dalloc(&c->cnt, 28 * ngbx));
+  HIP_TRY(c, dalloc(&c->cnt, 56 * ngbx));  // counts, then the run offsets of the neighbour sort
   HIP_TRY(c, dalloc(&c->tot, ngbx + 1));
   HIP_TRY(c, dalloc(&c->oobpre, ngbx + 1));
   HIP_TRY(c, dalloc(&c->oobcnt, ngbx));

@@ -71,7 +71,7 @@ struct cleo_ctx {
   double* d_state = nullptr;
   cleo::Solute solute{};
   // neighbour-sort scratch
-  uint32_t* cnt = nullptr;      // ngbx*28
+  uint32_t* cnt = nullptr;      // ngbx*56: sort counts (ngbx*28), then run offsets (ngbx*28)
   uint32_t* tot = nullptr;      // ngbx+1
   uint32_t* oobpre = nullptr;   // ngbx+1
   uint32_t* oobcnt = nullptr;   // ngbx: out-of-domain count per gridbox (compact copy for its scan)

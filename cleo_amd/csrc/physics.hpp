@@ -436,8 +436,9 @@ struct SolveOut {
 // Learned path (no reference counterpart): the reference's Newton update and convergence test
 // iterated from `warm` until converged, at most `warm_limit` iterations, and if it has not
 // converged by then the reference path from the plain guess.  Where |g| at the plain guess is
-// below atol the plain guess is already the root to the tolerance (a drop at equilibrium) and the
-// reference path runs at once: its two iterations are the fewest any start takes there.
+// below atol the plain guess is already the root to the tolerance (a drop at equilibrium), and the
+// same iteration starts from the plain guess instead: its test passes after one update, where the
+// reference path always makes two and a learned guess (a few 1e-3 off) needs three or four.
 __device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, double s_ratio, double akoh,
                                                      double bkoh, double ffactor, double rprev, double warm,
                                                      uint32_t warm_limit, SolveOut& o) {
@@ -451,14 +452,14 @@ __device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, doubl
   if (ucrit) {
     bool plain = true;
     if (warm == warm) {
-      double gp, dgp;
-      ode_g_dg(c, cfg.delt, rprev, ziter, gp, dgp);
-      if (fabs(gp) < cfg.atol) warm = __longlong_as_double(0x7FF8000000000000ll);
-    }
-    if (warm == warm) {
+      // the start: the plain guess where it is already the root to the tolerance, else the learned
       NewtonState s;
-      s.z = warm;
-      ode_g_dg(c, cfg.delt, rprev, warm, s.g, s.dg);
+      s.z = ziter;
+      ode_g_dg(c, cfg.delt, rprev, ziter, s.g, s.dg);
+      if (!(fabs(s.g) < cfg.atol)) {
+        s.z = warm;
+        ode_g_dg(c, cfg.delt, rprev, warm, s.g, s.dg);
+      }
       const int st = newton_until(cfg, c, warm_limit, cfg.delt, rprev, s, o.iters);
       if (st == kNewtonDeferred) {
         o.deferred = true;

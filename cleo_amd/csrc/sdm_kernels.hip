@@ -828,13 +828,18 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_motion(MotionArgs a) 
 // ============================ neighbour sort ==================================================
 __device__ __forceinline__ bool guard_off(const uint32_t* flag) { return flag && *flag == 0u; }
 
-// Per destination d (thread per d): the drops arriving in d, and in place of each count of row d
-// the drops that reach d from source gridboxes of lower index (the stable order puts them first).
+// Per destination d (thread per d): the drops arriving in d, and for each step s the drops that
+// reach d from source gridboxes of lower index than the source of s (the stable order puts them
+// first).  The counts are destination-major (row d, column s: arrivals at d by the step s) and
+// read as one row; the run offsets are written source-major into `runs` (row g, column s: the
+// offset of g's drops taking the step s), so the rank pass reads its gridbox's 27 offsets as one
+// contiguous row.  Every entry of `runs` belongs to one destination (g + step s): no two threads
+// write the same entry.
 __global__ void k_nsort_tot(NsortArgs a) {
   if (guard_off(a.flag)) return;
   const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= a.grid.ngbx) return;
-  uint32_t* row = a.cnt + 28ull * d;
+  const uint32_t* row = a.cnt + 28ull * d;
   a.oobcnt[d] = row[27];  // the out-of-domain count, compact for its scan
   uint32_t cnt[27];
   uint32_t t = 0;
@@ -844,28 +849,40 @@ __global__ void k_nsort_tot(NsortArgs a) {
     t += cnt[s];
   }
   const Grid& gr = a.grid;
-  const uint32_t gz = d % gr.nz, gx = (d / gr.nz) % gr.nx, gy = d / (gr.nz * gr.nx);
-  const bool wraps = (gr.per[0] && (gz == 0 || gz + 1 == gr.nz)) || (gr.per[1] && (gx == 0 || gx + 1 == gr.nx)) ||
-                     (gr.per[2] && (gy == 0 || gy + 1 == gr.ny));
+  const int nz = static_cast<int>(gr.nz), nx = static_cast<int>(gr.nx), ny = static_cast<int>(gr.ny);
+  const int gz = static_cast<int>(d % gr.nz), gx = static_cast<int>((d / gr.nz) % gr.nx),
+            gy = static_cast<int>(d / (gr.nz * gr.nx));
+  auto axis = [](int i, int dd, int n, bool per) {
+    int j = i - dd;
+    if (j < 0) j = per ? j + n : -1;
+    if (j >= n) j = per ? j - n : -1;
+    return j;
+  };
+  uint32_t src[27];  // the source of each step (kOOB: off a finite axis, no drops)
+#pragma unroll
+  for (int s = 0; s < 27; ++s) {
+    const int jz = axis(gz, s % 3 - 1, nz, gr.per[0] != 0), jx = axis(gx, (s / 3) % 3 - 1, nx, gr.per[1] != 0),
+              jy = axis(gy, s / 9 - 1, ny, gr.per[2] != 0);
+    src[s] = (jz >= 0 && jx >= 0 && jy >= 0) ? static_cast<uint32_t>(jz + nz * (jx + nx * jy)) : kOOB;
+  }
+  const bool wraps = (gr.per[0] && (gz == 0 || gz + 1 == nz)) || (gr.per[1] && (gx == 0 || gx + 1 == nx)) ||
+                     (gr.per[2] && (gy == 0 || gy + 1 == ny));
   if (!wraps) {
     // no step wraps: the source of step s is d - (dz + nz dx + nz nx dy), so the sources in index
-    // order are the steps in decreasing order (sources off a finite axis hold no drops)
+    // order are the steps in decreasing order
     uint32_t before = 0;
 #pragma unroll
     for (int s = 26; s >= 0; --s) {
-      row[s] = before;
+      if (src[s] != kOOB) a.runs[28ull * src[s] + static_cast<uint32_t>(s)] = before;
       before += cnt[s];
     }
   } else {
-    uint32_t src[27];
-#pragma unroll
-    for (int s = 0; s < 27; ++s) src[s] = source_of(gr, d, s);
 #pragma unroll
     for (int s = 0; s < 27; ++s) {
       uint32_t before = 0;
 #pragma unroll
       for (int q = 0; q < 27; ++q) before += src[q] < src[s] ? cnt[q] : 0u;
-      row[s] = before;
+      if (src[s] != kOOB) a.runs[28ull * src[s] + static_cast<uint32_t>(s)] = before;
     }
   }
   a.tot[d] = t;
@@ -873,10 +890,11 @@ __global__ void k_nsort_tot(NsortArgs a) {
 
 // Stable scatter (sortsupers.hpp:63-76,167-193 under Serial): drop p of source gridbox g with
 // key d lands at  base[d] + (drops reaching d from gridboxes < g) + (earlier drops of g with key d);
-// the middle term is row d of the count table after k_nsort_tot, at the step g -> d.  The 27 run
-// starts of g (one per step) and the start of its out-of-domain run are loaded by lanes 0..27
-// together with the keys (one round trip; loading only the steps that occur costs a second one),
-// so the ranking below touches no memory.  The destination of each drop is
+// the middle term is entry (g, step g -> d) of the `runs` table k_nsort_tot writes, so the 27
+// run offsets of g are its own contiguous row.  The 27 run starts of g (one per step, with base[d])
+// and the start of its out-of-domain run are loaded by lanes 0..27 together with the keys (one
+// round trip; loading only the steps that occur costs a second one), so the ranking below touches
+// no memory.  The destination of each drop is
 // recorded as perm[dest] = p; k_nsort_gather then moves the records, so that each cache line of the
 // sorted store is written whole by one wave (runs scattered from many source gridboxes into one
 // line cost HBM read-modify-writes).
@@ -928,7 +946,7 @@ __device__ __forceinline__ RankHead rank_head(const NsortArgs& a, uint64_t g, in
     }
   }
   const uint32_t b = a.base[d];
-  const uint32_t extra = lane < 27 ? a.cnt[28ull * (d < ngbx ? d : 0u) + static_cast<uint32_t>(lane)] : a.oobpre[gc];
+  const uint32_t extra = lane < 27 ? a.runs[28ull * gc + static_cast<uint32_t>(lane)] : a.oobpre[gc];
   h.trun = ok ? b + extra : 0u;
   return h;
 }
@@ -1200,7 +1218,7 @@ __device__ __forceinline__ void motion_scatter_gbx(const MotionArgs& a, const Ns
         run = n.base[a.grid.ngbx] + n.oobpre[g];
       } else {
         const uint32_t d = neighbour_of(a.grid, g, lane);
-        run = n.base[d] + n.cnt[28ull * d + lane];
+        run = n.base[d] + n.runs[28ull * g + lane];
       }
     }
     have |= need;
@@ -1612,6 +1630,7 @@ static NsortArgs nsort_args(cleo_ctx* c, const SoA& src, const SoA& dst, const u
   a.off = c->off;
   a.base = c->off_alt;
   a.cnt = c->cnt;
+  a.runs = c->cnt + 28ull * c->grid.ngbx;
   a.oobpre = c->oobpre;
   a.tot = c->tot;
   a.oobcnt = c->oobcnt;
@@ -2052,7 +2071,7 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_deferred(CondFla
 // the gridbox it reaches, so motion writes nothing of its own.  kLds: the coordinate bounds are read
 // from an LDS copy (dynamic shared memory of nz + nx + ny + 3 doubles).
 template <bool kMove, bool kLds>
-__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_sort_gather_cond(NsortArgs n, CondFlatArgs a, MotionArgs m) {
+__global__ void __launch_bounds__(256) CLEO_OCCUPANCY(kMove ? 4 : 5) k_sort_gather_cond(NsortArgs n, CondFlatArgs a, MotionArgs m) {
   extern __shared__ double bounds_lds[];
   MotionArgs b = m;
   if constexpr (kMove && kLds) {
@@ -2101,23 +2120,24 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_sort_gather_cond(Nsor
     }
     const bool active = okp && g < a.ngbx;
     const double warm = a.warm ? a.warm[sp] : nowarm;  // learned guesses by source position
-    double rnew = r, dm = 0.0, qm = 0.0;
-    bool solved = false;
-    if (active) solved = cond_solve(a, a.cfg, g, q, r, msol, [&] { return xi; }, warm, st, rnew, dm, qm);
-    defer_drop(a, seg, ndef, active && !solved, static_cast<uint32_t>(q));
     // non-temporal stores: the sorted store and dm, qm are read back only after the whole pass,
     // from HBM either way, and streaming them past the L2 leaves it to the source lines that
-    // neighbouring destinations still read (5.31 -> 5.11 ms)
+    // neighbouring destinations still read (5.31 -> 5.11 ms).  Every field but the radius is
+    // final before the solve and is written first, so its registers are free during the solve.
     if (okp) {
       __builtin_nontemporal_store(g, &n.dst.gbx[q]);
       __builtin_nontemporal_store(c3, &n.dst.c3[q]);
       __builtin_nontemporal_store(c1, &n.dst.c1[q]);
       __builtin_nontemporal_store(c2, &n.dst.c2[q]);
       __builtin_nontemporal_store(xi, &n.dst.xi[q]);
-      __builtin_nontemporal_store(solved ? rnew : r, &n.dst.r[q]);
       __builtin_nontemporal_store(msol, &n.dst.msol[q]);
       __builtin_nontemporal_store(id, &n.dst.id[q]);
     }
+    double rnew = r, dm = 0.0, qm = 0.0;
+    bool solved = false;
+    if (active) solved = cond_solve(a, a.cfg, g, q, r, msol, [&] { return xi; }, warm, st, rnew, dm, qm);
+    defer_drop(a, seg, ndef, active && !solved, static_cast<uint32_t>(q));
+    if (okp) __builtin_nontemporal_store(solved ? rnew : r, &n.dst.r[q]);
     if (solved) {
       __builtin_nontemporal_store(dm, &a.dm[q]);
       if (a.qm) __builtin_nontemporal_store(qm, &a.qm[q]);

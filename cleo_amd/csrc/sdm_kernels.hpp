@@ -87,7 +87,8 @@ struct NsortArgs {
   SoA src, dst;
   const uint32_t* off;     // current refs (ngbx+1)
   const uint32_t* base;    // new refs (ngbx+1)
-  uint32_t* cnt;           // ngbx*28: arrivals per (destination, step), then start offsets
+  uint32_t* cnt;           // ngbx*28: arrivals per (destination, step)
+  uint32_t* runs;          // ngbx*28: per (source, step) the run's offset among its destination's arrivals
   const uint32_t* oobpre;  // ngbx+1
   uint32_t* tot;
   uint32_t* oobcnt;        // ngbx: drops of each gridbox that leave the domain (row d's last count)

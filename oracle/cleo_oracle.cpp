@@ -327,7 +327,8 @@ static inline double critical_timestep(const ODEConsts& c) {
 /* solve_condensation (impliciteuler.cpp:35-54); warm = NaN for the plain path.
    Learned-guess variant (no reference counterpart): the reference's Newton update and
    convergence test iterated from `warm` until converged, at most `warm_limit` iterations; if it
-   has not converged by then, the reference path from the plain guess. */
+   has not converged by then, the reference path from the plain guess.  Where |g| at the plain
+   guess is below atol, the same iteration starts from the plain guess instead. */
 static inline double solve_condensation(const NewtonCfg& cfg, double s_ratio, double akoh,
                                         double bkoh, double ffactor, double rprev, double warm,
                                         uint64_t warm_limit, NewtonCount& cnt, const ErrSink& e) {
@@ -338,9 +339,9 @@ static inline double solve_condensation(const NewtonCfg& cfg, double s_ratio, do
   const bool ucrit2 = (cfg.delt <= critical_timestep(c));
   double rsqrd = 0.0;
   if (ucrit1 || ucrit2) {
-    /* learned guess, except where the plain guess is the root to atol (|g| below it there) */
-    if (warm == warm && std::abs(ode_gfunc(c, cfg.delt, rprev, ziter)) < cfg.atol)
-      warm = std::numeric_limits<double>::quiet_NaN();
+    /* learned path from the learned guess, or from the plain guess where that is already the
+       root to atol (|g| below it there) */
+    if (warm == warm && std::abs(ode_gfunc(c, cfg.delt, rprev, ziter)) < cfg.atol) warm = ziter;
     if (warm == warm) {
       bool ok = true;
       rsqrd = until_converged(cfg, c, warm_limit, cfg.delt, rprev, warm, cnt, ok);

@@ -267,11 +267,12 @@ def test_newton_warm_start_from_root_takes_at_most_two(oracle):
     assert bad == pytest.approx(plain, rel=1e-12) and nbad == 1 + nplain
 
 
-def test_learned_guess_skipped_where_plain_guess_is_the_root(oracle):
+def test_learned_path_starts_from_plain_guess_where_it_is_the_root(oracle):
     """The learned path starts from the learned guess except where |g| at the plain guess is below
-    atol.  A drop at its equilibrium radius, whose plain guess r^2 is the root, keeps the reference's
-    two iterations and its result bit for bit under a learned guess 1e-3 off; the same guess on a
-    drop far from its root is taken and needs fewer iterations than the plain one."""
+    atol.  A drop at its equilibrium radius, whose plain guess r^2 is the root, is then iterated
+    from r^2 under a learned guess 1e-3 off: one update passes the test (the reference path makes
+    two) and the radius is the reference's to 1e-14.  The same guess on a drop far from its root is
+    taken and needs fewer iterations than the plain one."""
     temp = 285.0 / 273.15
     a = 3.3e-7 / (273.15 * 1e-6) / temp
     b = a * 0.2 * 0.2
@@ -284,7 +285,10 @@ def test_learned_guess_skipped_where_plain_guess_is_the_root(oracle):
     F = _F(oracle, req)
     plain, nplain, sub, _ = _solve(oracle, s, a, b, F, req)
     w, nw, _, _ = _solve(oracle, s, a, b, F, req, warm=req * req * (1 + 1e-3))
-    assert sub == 0 and nplain == 2 and nw == 2 and w == plain
+    assert sub == 0 and nplain == 2 and nw == 1 and w == pytest.approx(plain, rel=1e-14)
+    # the learned guess alone (no plain root to start from) needs more than one update there
+    g_only, ng_only, _, _ = _solve(oracle, s, a, b, F, req, warm=req * req * (1 + 1e-3), atol=1e-30, rtol=1e-12)
+    assert ng_only >= 2 and g_only == pytest.approx(plain, rel=1e-12)
     # from 3 r_eq the plain guess (r^2) is far from the root; a guess 1e-3 off it is taken
     far, nfar, _, _ = _solve(oracle, s, a, b, F, 3 * req)
     wf, nwf, _, _ = _solve(oracle, s, a, b, F, 3 * req, warm=far * far * (1 + 1e-3))
