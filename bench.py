@@ -479,6 +479,8 @@ def _leg_coal_breakup(args, b, torch):
     torch.cuda.empty_cache()
     assert d.set_sdm_chain([abi.coll_process(kp)]) == 0
     d.check(L.cleo_run_steps_graph(d.h, 0, 100, 2), "warmup")
+    tiles = abi.Tiles()
+    d.check(L.cleo_autotune(d.h, C.byref(tiles)), "autotune")
     ms, stats = _graph_steps(d, L, torch, 200, k)
     d.close()
     cpu, _ = _cpu_leg(torch, P.coal_breakup_device, (64, 64, 16), P.NACL, [abi.coll_process(kp)], 10)
@@ -488,7 +490,7 @@ def _leg_coal_breakup(args, b, torch):
                  "counter RNG, hipGraph", ndrops=n, ms_per_step=ms, value=val, unit="superdroplet-steps/s",
         coll_events_per_step=stats["coll_events"] / k,
         hbm_model_bytes_per_drop=48, hbm_model_GBs=48 * n / (ms / 1e3) / 1e9, cpu_baseline=cpu,
-        gpu_over_cpu=val / cpu["value"])
+        gpu_over_cpu=val / cpu["value"], tiles={f: int(getattr(tiles, f)) for f, _ in tiles._fields_})
     return out
 
 

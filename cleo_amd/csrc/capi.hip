@@ -172,7 +172,7 @@ int prepare_injected(cleo_ctx* c) {
 }
 
 // one collision event over every gridbox: the drops stay in buf[cur], their shuffled order goes to
-// c->order (a pending order of an earlier event is applied first)
+// c->order (composed with a pending order of an earlier event)
 // the kinds a cleo_coll_params names must be ones the kernels know; the Low-List probabilities
 // and the kinetic-energy NFrags / flags read a terminal velocity formula
 bool tv_known(int32_t tv) { return tv == CLEO_TV_NULL || tv == CLEO_TV_SIMMEL || tv == CLEO_TV_ROGERS_YAU || tv == CLEO_TV_ROGERS_GK; }
@@ -209,10 +209,13 @@ int enqueue_collision_event(cleo_ctx* c, const cleo_coll_params& p) {
     const int rc = prepare_injected(c);
     if (rc != CLEO_OK) return rc;
   }
-  materialize_order(c);
+  // a pending order of an earlier event is composed with this one's shuffle: the drops stay where
+  // they are, and the new order goes to the other order buffer
+  const bool compose = c->order_live;
   CollArgs a{};
   a.s = c->buf[c->cur];
-  a.order = c->order;
+  a.order = compose ? c->order_alt : c->order;
+  a.order_in = compose ? c->order : nullptr;
   a.off = c->off;
   a.nact = c->nact;
   a.vol = c->st.vol;
@@ -229,6 +232,7 @@ int enqueue_collision_event(cleo_ctx* c, const cleo_coll_params& p) {
   a.big = c->big_scratch;
   launch_collisions(c, a);
   launch_advance_event(c);
+  if (compose) std::swap(c->order, c->order_alt);
   c->order_live = true;
   if (inj) c->rng_mode = 0;  // consumed
   return launch_check(c);
@@ -440,6 +444,7 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   HIP_TRY(c, dalloc(&c->perm, std::max<uint64_t>(maxn, 1)));
   HIP_TRY(c, dalloc(&c->dstep, std::max<uint64_t>(maxn, 1)));
   HIP_TRY(c, dalloc(&c->order, std::max<uint64_t>(maxn, 1)));
+  HIP_TRY(c, dalloc(&c->order_alt, std::max<uint64_t>(maxn, 1)));
   HIP_TRY(c, dalloc(&c->qm, std::max<uint64_t>(maxn, 1)));
   c->big_scratch_n = 2 * maxn;
   HIP_TRY(c, dalloc(&c->ctl, 1));
@@ -470,7 +475,7 @@ int cleo_destroy(cleo_ctx* c) {
                   static_cast<void*>(c->rs_key[1]), static_cast<void*>(c->rs_idx[0]), static_cast<void*>(c->rs_idx[1]),
                   static_cast<void*>(c->rs_cnt), static_cast<void*>(c->gs), static_cast<void*>(c->dm),
                   static_cast<void*>(c->sp), static_cast<void*>(c->dlist), static_cast<void*>(c->dcount),
-                  static_cast<void*>(c->dseg), c->mom, static_cast<void*>(c->perm), static_cast<void*>(c->dstep), static_cast<void*>(c->order),
+                  static_cast<void*>(c->dseg), c->mom, static_cast<void*>(c->perm), static_cast<void*>(c->dstep), static_cast<void*>(c->order), static_cast<void*>(c->order_alt),
                   static_cast<void*>(c->qm)})
     if (p) hipFree(p);
   if (c->h_ctl) hipHostFree(c->h_ctl);
@@ -830,28 +835,35 @@ int cleo_run_steps_graph(cleo_ctx* c, uint32_t t_mdl, uint32_t span, uint32_t ns
   uint32_t per_graph = 1;
   int cur0 = c->cur;
   bool live0 = c->order_live;
+  uint32_t* ord0 = c->order;  // which of the two order buffers holds the pending order
+  auto set_order = [&](uint32_t* ord) {
+    if (c->order != ord) std::swap(c->order, c->order_alt);
+  };
   int rc = CLEO_OK;
   for (int attempt = 0; attempt < 3; ++attempt) {
     cur0 = c->cur;
     live0 = c->order_live;
+    ord0 = c->order;
     HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     rc = enqueue_run_step(c, t_mdl, t_mdl + span);
     per_graph = 1;
-    if (rc == CLEO_OK && (c->cur != cur0 || c->order_live != live0)) {
+    if (rc == CLEO_OK && (c->cur != cur0 || c->order_live != live0 || c->order != ord0)) {
       rc = enqueue_run_step(c, t_mdl + span, t_mdl + 2 * span);
       per_graph = 2;
     }
     const hipError_t e = hipStreamEndCapture(c->stream, &graph);
     const int cur1 = c->cur;
     const bool live1 = c->order_live;
+    uint32_t* const ord1 = c->order;
     c->cur = cur0;
     c->order_live = live0;
+    set_order(ord0);
     if (rc != CLEO_OK || e != hipSuccess) {
       if (graph) hipGraphDestroy(graph);
       if (rc != CLEO_OK) return rc;
       return fail(c, CLEO_E_HIP, std::string("capture: ") + hipGetErrorString(e));
     }
-    if (cur1 == cur0 && live1 == live0) break;
+    if (cur1 == cur0 && live1 == live0 && ord1 == ord0) break;
     hipGraphDestroy(graph);
     graph = nullptr;
     if (live1 && !live0) {
@@ -872,6 +884,7 @@ int cleo_run_steps_graph(cleo_ctx* c, uint32_t t_mdl, uint32_t span, uint32_t ns
   }
   c->cur = cur0;  // every replay starts and ends in the same state
   c->order_live = live0;
+  set_order(ord0);
   const uint32_t nreplay = nsteps / per_graph;
   for (uint32_t i = 0; i < nreplay; ++i) {
     e = hipGraphLaunch(exec, c->stream);

@@ -131,9 +131,11 @@ struct cleo_ctx {
   uint8_t* dstep = nullptr;
   // Collisions leave the drops in place and record the shuffled order: order[q] is the position
   // of the drop at slot q of the domain's logical order.  While order_live is set the store's
-  // physical order may differ from its logical order; the next sort applies the order and
-  // materialize_order() applies it for callers that need the drops in order.
+  // physical order may differ from its logical order; the next sort applies the order,
+  // materialize_order() applies it for callers that need the drops in order, and a further
+  // collision event composes its shuffle with it.
   uint32_t* order = nullptr;
+  uint32_t* order_alt = nullptr;  // the other buffer: a collision event composes order into it
   bool order_live = false;
   // per drop: condensed mass * xi before a condensation solve (the qcond recompute after motion)
   double* qm = nullptr;

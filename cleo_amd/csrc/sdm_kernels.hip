@@ -248,6 +248,13 @@ __device__ __forceinline__ uint32_t fy_index(const CollArgs& a, uint32_t g, uint
   return j;
 }
 
+// position of the drop at slot `slot` of gridbox `base`'s logical order: through a pending order
+// of an earlier event when there is one (the drops then stay where they are and the orders
+// compose), else the slot itself
+__device__ __forceinline__ uint32_t coll_pos(const CollArgs& a, uint32_t base, uint32_t slot) {
+  return a.order_in ? a.order_in[base + slot] : base + slot;
+}
+
 // One collision pair (2jj, 2jj+1) of gridbox g (n active drops) from the drops at ka, kb
 // (collisions.hpp:180-230): probability and enactment on (xi, r, msol) of both drops.
 struct PairOut {
@@ -409,11 +416,13 @@ __device__ __forceinline__ void coll_gbx(const CollArgs& a, int lane, uint32_t g
   for (uint32_t c = 0; c < npairs; c += kWave) {
     const uint32_t jj = c + lane;
     bool nul = false;
-    if (jj < npairs) nul = coll_pair<Kinetic>(a, g, base, n, jj, base + perm[2 * jj], base + perm[2 * jj + 1], event, events);
+    if (jj < npairs)
+      nul = coll_pair<Kinetic>(a, g, base, n, jj, coll_pos(a, base, perm[2 * jj]), coll_pos(a, base, perm[2 * jj + 1]),
+                               event, events);
     nulls += static_cast<uint32_t>(__popcll(__ballot(nul)));
   }
   // unpaired last drop and the inactive tail (nulls of earlier sub-steps) keep their places
-  for (uint32_t k = 2 * npairs + lane; k < nfull; k += kWave) a.order[base + k] = base + (k < n ? perm[k] : k);
+  for (uint32_t k = 2 * npairs + lane; k < nfull; k += kWave) a.order[base + k] = coll_pos(a, base, k < n ? perm[k] : k);
   if (nulls > 0) coll_order_nulls(a, g, base, n, nulls, lane);
   acc[0] += npairs;
   acc[1] += wave_sum_u32(events);
@@ -543,13 +552,15 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_coll_tiles(CollArgs a
       const uint32_t i = tile_owner(L.ppre, f);
       const uint32_t jj = f - L.ppre[i], bi = L.base[i];
       const uint8_t* pi = pb + L.loff[i];
-      if (coll_pair<Kinetic>(a, tile * kWave + i, bi, L.n[i], jj, bi + pi[2 * jj], bi + pi[2 * jj + 1], event, events))
+      if (coll_pair<Kinetic>(a, tile * kWave + i, bi, L.n[i], jj, coll_pos(a, bi, pi[2 * jj]),
+                             coll_pos(a, bi, pi[2 * jj + 1]), event, events))
         atomicAdd(&L.nulls[i], 1u);
     }
     // the unpaired last drop and the inactive tail of the lane's own gridbox keep their places
     if (small) {
       const uint8_t* p = pb + loff;
-      for (uint32_t k = 2u * np; k < nfull; ++k) a.order[base + k] = base + (k < n ? static_cast<uint32_t>(p[k]) : k);
+      for (uint32_t k = 2u * np; k < nfull; ++k)
+        a.order[base + k] = coll_pos(a, base, k < n ? static_cast<uint32_t>(p[k]) : k);
     }
     wave_sync();
     const uint32_t mynulls = L.nulls[lane];

@@ -52,6 +52,9 @@ struct QcondArgs {
 struct CollArgs {
   SoA s;            // the store: drops keep their places, changed attributes are written in place
   uint32_t* order;  // out: position of the drop at each slot of the shuffled (logical) order
+  // in: a pending order of an earlier event (position of the drop at each slot), composed with
+  // this event's shuffle; null when the store is in its logical order
+  const uint32_t* order_in;
   const uint32_t* off;
   uint32_t* nact;
   const double* vol;
