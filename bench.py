@@ -572,6 +572,7 @@ def run_cpu(args):
     from cleo_amd import populations as P
     from oracle import pyoracle
 
+    host = host_cpu()  # before the OpenMP runtime binds this thread to one core
     b = pyoracle.load("omp")
     ndims = (128, 128, args.cpu_ny)
     pop = P.coupled(ndims=ndims)
@@ -614,7 +615,7 @@ def run_cpu(args):
                        f"(gridbox-parallel, 72-byte AoS, atomic counting sort, g++ -O3 -march=x86-64-v3)",
                 one_thread=dict(value=one.n / dt1, unit="superdroplet-steps/s",
                                 sample=f"one step of 128x128x2 gridboxes ({one.n} drops) after one warm-up step"),
-                host=host_cpu())
+                host=host)
 
 
 def main():

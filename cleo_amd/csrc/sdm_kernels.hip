@@ -845,7 +845,8 @@ __device__ __forceinline__ bool guard_off(const uint32_t* flag) { return flag &&
 // read as one row; the run offsets are written source-major into `runs` (row g, column s: the
 // offset of g's drops taking the step s), so the rank pass reads its gridbox's 27 offsets as one
 // contiguous row.  Every entry of `runs` belongs to one destination (g + step s): no two threads
-// write the same entry.
+// write the same entry.  Only the entries of steps that some drop takes are written (a source off
+// a finite axis has none); the others keep older values that no pass reads.
 __global__ void k_nsort_tot(NsortArgs a) {
   if (guard_off(a.flag)) return;
   const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
@@ -884,7 +885,7 @@ __global__ void k_nsort_tot(NsortArgs a) {
     uint32_t before = 0;
 #pragma unroll
     for (int s = 26; s >= 0; --s) {
-      if (src[s] != kOOB) a.runs[28ull * src[s] + static_cast<uint32_t>(s)] = before;
+      if (cnt[s] > 0u && src[s] != kOOB) a.runs[28ull * src[s] + static_cast<uint32_t>(s)] = before;
       before += cnt[s];
     }
   } else {
@@ -893,7 +894,7 @@ __global__ void k_nsort_tot(NsortArgs a) {
       uint32_t before = 0;
 #pragma unroll
       for (int q = 0; q < 27; ++q) before += src[q] < src[s] ? cnt[q] : 0u;
-      if (src[s] != kOOB) a.runs[28ull * src[s] + static_cast<uint32_t>(s)] = before;
+      if (cnt[s] > 0u && src[s] != kOOB) a.runs[28ull * src[s] + static_cast<uint32_t>(s)] = before;
     }
   }
   a.tot[d] = t;
@@ -1267,11 +1268,12 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_motion_scatter(Motion
 __global__ void k_cnt_from_nact(const uint32_t* off, const uint32_t* nact, uint32_t ngbx, uint32_t* cnt,
                                 const uint32_t* flag) {
   if (guard_off(flag)) return;
-  const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-  if (i >= 28ull * ngbx) return;
-  const uint32_t g = static_cast<uint32_t>(i / 28), s = static_cast<uint32_t>(i % 28);
-  const uint32_t n = nact[g], nfull = off[g + 1] - off[g];
-  cnt[i] = s == 13 ? n : (s == 27 ? nfull - n : 0u);
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < 28ull * ngbx;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t g = static_cast<uint32_t>(i / 28), s = static_cast<uint32_t>(i % 28);
+    const uint32_t n = nact[g], nfull = off[g + 1] - off[g];
+    cnt[i] = s == 13 ? n : (s == 27 ? nfull - n : 0u);
+  }
 }
 
 __global__ void k_set_nact(const uint32_t* off, uint32_t ngbx, uint32_t* nact, const uint32_t* flag) {
@@ -1511,6 +1513,11 @@ static inline unsigned stride_blocks(uint64_t n) {
   const uint64_t b = (n + 255) / 256;
   return static_cast<unsigned>(b == 0 ? 1 : (b > 65536 ? 65536 : b));
 }
+// blocks of a grid-stride loop that usually has little or nothing to do: a launch guarded by a
+// device flag that is mostly clear (the re-sort after collisions), or the out-of-domain tail.
+// 2048 blocks of 256 threads still fill every CU when the work is there; a launch of many
+// thousands of blocks that exit at once costs tens of microseconds.
+static inline unsigned sparse_blocks(uint64_t n) { return std::min<unsigned>(stride_blocks(n), 2048u); }
 
 __global__ void k_set_u32(uint32_t* out, uint32_t v, const uint32_t* flag) {
   if (guard_off(flag)) return;
@@ -1597,7 +1604,7 @@ void launch_inj_sizes(cleo_ctx* c, uint32_t* nj, uint32_t* np) {
   hipLaunchKernelGGL(k_inj_sizes, dim3(nblocks(c->grid.ngbx, 256)), dim3(256), 0, c->stream, c->nact, c->grid.ngbx, nj, np);
 }
 void launch_copy_oob_region(cleo_ctx* c, int src, int dst) {
-  hipLaunchKernelGGL(k_copy_oob_region, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, c->buf[src],
+  hipLaunchKernelGGL(k_copy_oob_region, dim3(sparse_blocks(c->ntot)), dim3(256), 0, c->stream, c->buf[src],
                      c->buf[dst], c->off, c->grid.ngbx, c->ntot);
 }
 
@@ -1658,8 +1665,10 @@ static NsortArgs nsort_args(cleo_ctx* c, const SoA& src, const SoA& dst, const u
 static void nsort_move(cleo_ctx* c, const NsortArgs& a) {
   const uint32_t ngbx = c->grid.ngbx;
   launch_nsort_perm(c, a, ngbx);
-  // streaming copy: one pair of positions per thread (measured faster than a capped grid)
-  hipLaunchKernelGGL(k_nsort_gather, dim3(motion_blocks(c->ntot, 0)), dim3(256), 0, c->stream, a);
+  // streaming copy: one pair of positions per thread (measured faster than a capped grid), except
+  // under a device flag that is mostly clear
+  const unsigned nb = a.flag ? sparse_blocks(c->ntot) : motion_blocks(c->ntot, 0);
+  hipLaunchKernelGGL(k_nsort_gather, dim3(nb), dim3(256), 0, c->stream, a);
 }
 // totals per destination, start offsets in the count table, new refs and the out-of-domain prefix
 static void nsort_tables(cleo_ctx* c, const NsortArgs& a, const uint32_t* flag) {
@@ -1685,7 +1694,7 @@ void neighbour_sort_to(cleo_ctx* c, const SoA& src, const SoA& dst, int commit_d
   a.step = step;
   nsort_tables(c, a, nullptr);
   nsort_move(c, a);
-  hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, a, c->ntot);
+  hipLaunchKernelGGL(k_nsort_oob, dim3(sparse_blocks(c->ntot)), dim3(256), 0, c->stream, a, c->ntot);
   hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, static_cast<const uint32_t*>(nullptr));
   if (commit_dst >= 0) nsort_commit(c, commit_dst);
 }
@@ -1697,21 +1706,22 @@ void neighbour_sort(cleo_ctx* c, int src, int dst, const uint32_t* flag, bool fr
   }
   const uint32_t ngbx = c->grid.ngbx;
   NsortArgs a = nsort_args(c, c->buf[src], c->buf[dst], flag);
-  if (from_nact)
-    hipLaunchKernelGGL(k_cnt_from_nact, dim3(nblocks(28ull * ngbx, 256)), dim3(256), 0, c->stream, c->off, c->nact, ngbx,
-                       c->cnt, flag);
+  if (from_nact) {
+    const unsigned nb = flag ? sparse_blocks(28ull * ngbx) : nblocks(28ull * ngbx, 256);
+    hipLaunchKernelGGL(k_cnt_from_nact, dim3(nb), dim3(256), 0, c->stream, c->off, c->nact, ngbx, c->cnt, flag);
+  }
   nsort_tables(c, a, flag);
   nsort_move(c, a);
-  hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, a, c->ntot);
+  hipLaunchKernelGGL(k_nsort_oob, dim3(sparse_blocks(c->ntot)), dim3(256), 0, c->stream, a, c->ntot);
   hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, flag);
   if (flag == nullptr) {
     nsort_commit(c, dst);
   } else {
-    hipLaunchKernelGGL(k_copy_soa, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, c->buf[dst], c->buf[src],
+    hipLaunchKernelGGL(k_copy_soa, dim3(sparse_blocks(c->ntot)), dim3(256), 0, c->stream, c->buf[dst], c->buf[src],
                        0ull, c->ntot, flag);
     // the copied-back store is in its logical order: if the sort ran, the order becomes the identity
     if (c->order_live)
-      hipLaunchKernelGGL(k_iota_u32, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, c->order, c->ntot, flag);
+      hipLaunchKernelGGL(k_iota_u32, dim3(sparse_blocks(c->ntot)), dim3(256), 0, c->stream, c->order, c->ntot, flag);
     hipLaunchKernelGGL(k_copy_u32, dim3(stride_blocks(ngbx + 1ull)), dim3(256), 0, c->stream, c->off_alt, c->off,
                        ngbx + 1ull, flag);
     hipLaunchKernelGGL(k_set_nact, dim3(nblocks(ngbx, 256)), dim3(256), 0, c->stream, c->off, ngbx, c->nact, flag);
@@ -1756,7 +1766,7 @@ void motion_sort_fused(cleo_ctx* c, const MotionArgs& args, const SoA& dst, int 
   hipLaunchKernelGGL(k_motion_scatter, dim3(gbx_blocks(ngbx, n.wpb, c->tiles.motion_grid_waves)), dim3(64 * n.wpb), 0,
                      c->stream, a, n);
   kev_mark(c, 1);
-  hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, n, c->ntot);
+  hipLaunchKernelGGL(k_nsort_oob, dim3(sparse_blocks(c->ntot)), dim3(256), 0, c->stream, n, c->ntot);
   hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, static_cast<const uint32_t*>(nullptr));
   if (commit_dst >= 0) nsort_commit(c, commit_dst);
 }
@@ -1782,7 +1792,7 @@ void motion_sort_gather(cleo_ctx* c, const MotionArgs& args, const SoA& dst, int
   nsort_tables(c, n, nullptr);
   launch_nsort_perm(c, n, ngbx);
   launch_gather_move(c, n, a);
-  hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, n, c->ntot);
+  hipLaunchKernelGGL(k_nsort_oob, dim3(sparse_blocks(c->ntot)), dim3(256), 0, c->stream, n, c->ntot);
   hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, static_cast<const uint32_t*>(nullptr));
   if (commit_dst >= 0) nsort_commit(c, commit_dst);
 }
@@ -2872,7 +2882,7 @@ void motion_sort_condense(cleo_ctx* c, const MotionArgs& margs, const CondFlatAr
     hipLaunchKernelGGL(k_cond_deferred, dim3(nb), dim3(256), 0, c->stream, a);
   }
   launch_stats_reduce(c, 4u * nb, ST_COND_DROPS, 4);
-  hipLaunchKernelGGL(k_nsort_oob, dim3(stride_blocks(c->ntot)), dim3(256), 0, c->stream, n, c->ntot);
+  hipLaunchKernelGGL(k_nsort_oob, dim3(sparse_blocks(c->ntot)), dim3(256), 0, c->stream, n, c->ntot);
   hipLaunchKernelGGL(k_sorts_stat, dim3(1), dim3(1), 0, c->stream, c->ctl, static_cast<const uint32_t*>(nullptr));
   if (commit_dst >= 0) nsort_commit(c, commit_dst);
   hipLaunchKernelGGL(k_cond_state, dim3(gbx_blocks(ntiles, 4, gw)), dim3(256), 0, c->stream, a);
