@@ -22,6 +22,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPUs this process may use, read before an OpenMP runtime binds the thread to one core
+HOST_CPUS = len(os.sched_getaffinity(0))
 # the CPU baseline's OpenMP threads: one per core, packed (set before any OpenMP runtime starts)
 os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
@@ -91,7 +93,7 @@ def host_cpu():
                     break
     except OSError:
         pass
-    return dict(model=model, cpus_available=len(os.sched_getaffinity(0)), omp_proc_bind=os.environ.get("OMP_PROC_BIND"),
+    return dict(model=model, cpus_available=HOST_CPUS, omp_proc_bind=os.environ.get("OMP_PROC_BIND"),
                 omp_places=os.environ.get("OMP_PLACES"), omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
 
 
@@ -572,7 +574,7 @@ def run_cpu(args):
     from cleo_amd import populations as P
     from oracle import pyoracle
 
-    host = host_cpu()  # before the OpenMP runtime binds this thread to one core
+    host = host_cpu()
     b = pyoracle.load("omp")
     ndims = (128, 128, args.cpu_ny)
     pop = P.coupled(ndims=ndims)
