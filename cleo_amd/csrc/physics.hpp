@@ -663,8 +663,11 @@ struct DropAttrs {
   uint64_t xi;
   double r, msol;
 };
-// coalescence_gamma (coalescence.cpp:68-79)
+// coalescence_gamma (coalescence.cpp:68-79).  A probability of 2^64 or more (or NaN) overflows
+// the reference's double -> uint64 conversion, whose result is then undefined; gamma is then the
+// cap floor(xi1/xi2) that any such gamma reaches.
 __device__ __forceinline__ uint64_t coal_gamma(uint64_t xi1, uint64_t xi2, double prob, double phi) {
+  if (!(prob < 0x1p64)) return xi1 / xi2;
   uint64_t gamma = static_cast<uint64_t>(floor(prob));
   if (phi < (prob - static_cast<double>(gamma))) ++gamma;
   const uint64_t maxgamma = xi1 / xi2;

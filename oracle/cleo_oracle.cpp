@@ -530,8 +530,12 @@ static inline double pair_prob(const cleo_coll_params& p, const Superdrop& a, co
 }
 
 /* ---------------- enactments ---------------- */
-/* coalescence.cpp:68-79 (also coalbure.hpp:74-84) */
+/* coalescence.cpp:68-79 (also coalbure.hpp:74-84).  A probability of 2^64 or more (or NaN)
+   overflows the reference's double -> uint64 conversion, whose result is then undefined (0 on
+   x86-64, saturated in a CUDA build); here gamma is the cap floor(xi1/xi2) that any such
+   gamma reaches, as on the GPU. */
 static inline uint64_t coal_gamma(uint64_t xi1, uint64_t xi2, double prob, double phi) {
+  if (!(prob < 0x1p64)) return xi1 / xi2;
   uint64_t gamma = static_cast<uint64_t>(std::floor(prob));
   if (phi < (prob - gamma)) ++gamma;
   const uint64_t maxgamma = xi1 / xi2;

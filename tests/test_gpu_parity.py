@@ -191,6 +191,33 @@ def test_nulls_and_resort(gpu, oracle, tiled):
     assert g.stats()["nulls"] == o.stats()["nulls"]
 
 
+@pytest.mark.parametrize("tiled", [0, 1])
+def test_successive_collision_events_compose_orders(gpu, oracle, tiled):
+    """Four collision events in a row with no call between them that needs the drops in order:
+    each event composes its shuffle with the pending order of the one before (the drops stay where
+    they are), nulls partition the composed order, and the result is the oracle's, which moves the
+    drops at every event.  Tiles of 64 gridboxes of ~150 drops overflow the tile's LDS, so the
+    tiled variant also sends some gridboxes through the per-gridbox path."""
+    _say(f"case: 4 successive collision events (composed orders, nulls), tiled={tiled}")
+    pop = P.coal_breakup(SEED, ndims=(5, 4, 3), mean=150)
+    n = len(pop.supers["xi"])
+    rng = np.random.default_rng(23)
+    pop.supers["xi"][:] = rng.integers(1, 4, n).astype(np.uint64)  # twins with xi 1 give nulls
+    g, o = _both(gpu, oracle, pop)
+    _coll_variant(gpu, g, tiled)
+    g.rng_counter(31)
+    o.rng_counter(31)
+    p = coll_params(prob=abi.PROB_CONST, param=1e30, enact=abi.ENACT_COAL)  # every pair coalesces
+    for event in range(4):
+        assert g.collisions(p) == 0
+        assert o.collisions(p) == 0
+    assert g.error()[0] == 0
+    assert_supers_match(g.download(), o.download(), what="composed orders")
+    assert np.array_equal(g.refs(), o.refs())
+    sg, so = g.stats(), o.stats()
+    assert so["nulls"] > 0 and sg["nulls"] == so["nulls"] and sg["coll_events"] == so["coll_events"]
+
+
 @pytest.mark.parametrize("ndims,mean", [((4, 4, 4), 200), ((2, 2, 2), 300), ((9, 5, 3), 3)])
 def test_collisions_tiled_mixed_sizes(gpu, oracle, ndims, mean):
     _say(f"case: tiled collisions, ndims={ndims} mean={mean} (tile LDS overflow, >255-drop gridboxes)")

@@ -80,6 +80,19 @@ def test_coalescence_gamma_cap_and_no_event(oracle):
     assert xi == [10, 4] and r == [2.0, 5.0] and res == 0
 
 
+def test_coalescence_gamma_beyond_uint64(oracle):
+    """A probability of 2^64 or more has no defined gamma in the reference (its double -> uint64
+    conversion overflows); it is taken as the cap floor(xi1/xi2), which any such gamma reaches."""
+    for prob in (2.0 ** 64, 1e30, float("inf")):
+        xi, r, m, res, err = _enact(oracle, abi.ENACT_COAL, (9, 4), (2.0, 5.0), (0.1, 0.3), prob, 0.5)
+        assert xi == [1, 4] and err == 0  # gamma = 9 // 4 = 2: different case, 9 - 2*4
+    xi, r, m, res, err = _enact(oracle, abi.ENACT_COAL, (3, 1), (2.0, 5.0), (0.1, 0.3), 1e30, 0.5)
+    assert xi == [0, 1] and (res & 1) == 1  # gamma 3: twin with xi2 = 1, drop 1 becomes null
+    # just below 2^64 the reference's conversion applies
+    xi, r, m, res, err = _enact(oracle, abi.ENACT_COAL, (9, 4), (2.0, 5.0), (0.1, 0.3), 2.0 ** 63, 0.5)
+    assert xi == [1, 4]
+
+
 def test_coalescence_twin_and_null(oracle):
     xi, r, m, res, err = _enact(oracle, abi.ENACT_COAL, (8, 4), (2.0, 5.0), (0.1, 0.3), 2.5, 0.1)
     # gamma = min(3, 2) = 2 and 8 - 2*4 == 0: twin, xi2/2 each, shared radius and solute
