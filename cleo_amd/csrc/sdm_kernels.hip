@@ -742,6 +742,10 @@ struct DropPos {
 __device__ __forceinline__ DropPos drop_pos(const MotionArgs& a, uint64_t k) {
   return DropPos{a.s.c3[k], a.s.c1[k], a.s.c2[k], a.s.r[k]};
 }
+__device__ __forceinline__ DropPos drop_pos_nt(const MotionArgs& a, uint64_t k) {
+  return DropPos{__builtin_nontemporal_load(&a.s.c3[k]), __builtin_nontemporal_load(&a.s.c1[k]),
+                 __builtin_nontemporal_load(&a.s.c2[k]), __builtin_nontemporal_load(&a.s.r[k])};
+}
 __device__ __forceinline__ Moved move_drop(const MotionArgs& a, uint32_t g, const GbxGeom& G, const DropPos& P,
                                            uint64_t k, bool report) {
   const uint32_t nz = a.grid.nz, nx = a.grid.nx, ny = a.grid.ny;
@@ -2532,9 +2536,9 @@ __global__ void __launch_bounds__(256) k_mass_moments(SoA s, const uint32_t* off
     auto load = [&](uint32_t w) {
       for (uint32_t i = 0; i < kPer; ++i) {
         const uint32_t k = min(w + static_cast<uint32_t>(lane) + i * kWave, t_hi - 1u);
-        rx[i] = s.xi[k];
-        rr[i] = s.r[k];
-        rm[i] = s.msol[k];
+        rx[i] = __builtin_nontemporal_load(&s.xi[k]);  // read once (0.63 -> 0.60 ms)
+        rr[i] = __builtin_nontemporal_load(&s.r[k]);
+        rm[i] = __builtin_nontemporal_load(&s.msol[k]);
       }
     };
     load(t_lo);
@@ -2661,8 +2665,9 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_motion_drops(MotionAr
     const bool acta = ka < ndom, actb = kb < ndom;
     // past the end a lane reads drop k0 (in the domain) and discards the result
     const uint64_t la = acta ? ka : k0, lb = actb ? kb : k0;
-    const uint32_t ga = a.s.gbx[la], gb = a.s.gbx[lb];
-    const DropPos pa = drop_pos(a, la), pb = drop_pos(a, lb);
+    // non-temporal loads: each drop's fields are read once here (2.21 -> 1.94 ms)
+    const uint32_t ga = __builtin_nontemporal_load(&a.s.gbx[la]), gb = __builtin_nontemporal_load(&a.s.gbx[lb]);
+    const DropPos pa = drop_pos_nt(a, la), pb = drop_pos_nt(a, lb);
     const GbxGeom Ga = gbx_geom(b, ga), Gb = gbx_geom(b, gb);
     const Moved Ma = move_drop(b, ga, Ga, pa, la, acta);
     const Moved Mb = move_drop(b, gb, Gb, pb, lb, actb);
