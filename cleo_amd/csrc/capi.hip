@@ -1262,7 +1262,7 @@ int cleo_autotune(cleo_ctx* c, cleo_tiles* chosen) {
           t.cond_grid_waves = gw;
           t.cond_defer = budget;
           float tmin = 0.f;
-          const int rc = time_stage(c, 3, t, 2, false, &tmin, nullptr);
+          const int rc = time_stage(c, 3, t, 4, false, &tmin, nullptr);  // 4 reps: with 2 the pick varied between runs
           if (rc != CLEO_OK) return rc;
           if (tmin < best) {
             best = tmin;
