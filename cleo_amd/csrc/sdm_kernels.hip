@@ -2245,10 +2245,10 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_cond_queue(CondFlatAr
         R.r0[slot] = r;
         R.zplain[slot] = zg;
         double zw = a.warm ? a.warm[kk] : __longlong_as_double(0x7FF8000000000000ll);
-        if (uniq && zw == zw) {  // no learned start where the plain guess is the root (solve_condensation)
-          double gp, dg;
+        if (uniq && zw == zw) {  // the learned path starts from the plain guess where that is the root
+          double gp, dg;       // to the tolerance (solve_condensation)
           ode_g_dg(cc, cfg.delt, r, zg, gp, dg);
-          if (fabs(gp) < cfg.atol) zw = __longlong_as_double(0x7FF8000000000000ll);
+          if (fabs(gp) < cfg.atol) zw = zg;
         }
         R.zwarm[slot] = zw;
         R.mindelt[slot] = uniq ? 0.0 : fmax(critdelt, cfg.minsubdelt);
@@ -2315,7 +2315,7 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_cond_queue(CondFlatAr
       // one evaluation of g: the initial state of a segment, or a Newton iteration
       // (newton_iterate, impliciteuler.cpp:240-255)
       double zn = z;
-      if (phase != PH_EVAL) zn = fmax(z * (1 - gz / dgz), 1e-8);
+      if (phase != PH_EVAL) zn = fmax(z * (1 - quotient(gz, dgz)), 1e-8);  // as newton_iterate
       double g1, dg1;
       ode_g_dg(c, subdelt, rprev, zn, g1, dg1);
       bool segok = false, segfail = false;
