@@ -35,6 +35,12 @@ __device__ __forceinline__ void wave_sync() {
 __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int l) {
   return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), l));
 }
+// the wave's lanes where c holds, as a bit mask
+__device__ __forceinline__ uint64_t ballot(bool c) { return __builtin_amdgcn_ballot_w64(c); }
+// lanes below this one whose bit is set in m (v_mbcnt_lo/hi: two instructions)
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
 // occupancy floor of a kernel: at least W waves per SIMD, so at most 512/W VGPRs per lane
 #define CLEO_OCCUPANCY(W) __attribute__((amdgpu_waves_per_eu(W)))
 

@@ -1049,15 +1049,19 @@ __device__ __forceinline__ void rank_gbx(const NsortArgs& a, const RankHead& h, 
         ok = false;
       }
     }
+    // one round per distinct step in the chunk: its lanes take consecutive places from the run's
+    // next place (held by lane v), in lane order
     uint32_t dest = 0;
-    uint64_t mask = __ballot(ok);
+    const uint64_t okm = ballot(ok);
+    uint64_t mask = okm;
     while (mask) {
       const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
       const int v = static_cast<int>(readlane_u32(static_cast<uint32_t>(sv), leader));
-      const uint64_t m = __ballot(ok && sv == v);
+      const uint64_t m = ballot(sv == v) & okm;
       const uint32_t start = readlane_u32(trun, v);
-      if ((m >> lane) & 1ull) dest = start + static_cast<uint32_t>(__popcll(m & lanemask_lt()));
-      if (lane == v) trun = start + static_cast<uint32_t>(__popcll(m));
+      const uint32_t here = start + lanes_below(m);
+      dest = (ok && sv == v) ? here : dest;
+      trun = lane == v ? start + static_cast<uint32_t>(__popcll(m)) : trun;
       mask &= ~m;
     }
     if (ok && dest >= a.ntot) {  // counts inconsistent with keys: never write out of range
