@@ -428,7 +428,7 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   c->scan_tmp_n = scan_tmp_words(28 * ngbx + 1);
   HIP_TRY(c, dalloc(&c->scan_tmp, c->scan_tmp_n));
   HIP_TRY(c, dalloc(&c->big_scratch, 2 * maxn));
-  HIP_TRY(c, dalloc(&c->gs, 2 * ngbx));
+  HIP_TRY(c, dalloc(&c->gs, kGsStride * ngbx));
   HIP_TRY(c, dalloc(&c->dm, maxn));
   // a gridbox launch has at most ngbx + 3 waves, a drop launch 4 per 256 drops
   c->sp_slots = std::max<uint64_t>(ngbx + 4, 4 * ((maxn + 255) / 256) + 4);
@@ -454,7 +454,7 @@ int cleo_create(cleo_ctx** out, uint64_t maxn, const cleo_domain_desc* dom) {
   HIP_TRY(c, ctx_fill(c, c->off_alt, 0, (ngbx + 1) * 4));
   HIP_TRY(c, ctx_fill(c, c->nact, 0, ngbx * 4));
   HIP_TRY(c, ctx_fill(c, c->d_state, 0, 11 * ngbx * 8));
-  c->solute = Solute{2016.5 / dimless::RHO0, 0.05844277 / dimless::MR0, 2.0};
+  c->solute = make_solute(2016.5 / dimless::RHO0, 0.05844277 / dimless::MR0, 2.0);
   HIP_TRY(c, mlp_create(c));
   *out = c;
   return CLEO_OK;
@@ -683,7 +683,7 @@ int cleo_download_gbx_state(cleo_ctx* c, cleo_gbx_state* h) {
 
 int cleo_set_solute(cleo_ctx* c, const cleo_solute* s) {
   if (!c || !s || !(s->rho_sol > 0) || !(s->mr_sol > 0)) return CLEO_E_ARG;
-  c->solute = Solute{s->rho_sol, s->mr_sol, s->ionic};
+  c->solute = make_solute(s->rho_sol, s->mr_sol, s->ionic);
   return CLEO_OK;
 }
 

@@ -56,7 +56,13 @@ constexpr double kDblMax = 1.7976931348623157e308;
 
 struct Solute {
   double rho_sol, mr_sol, ionic;
+  double dens_factor;  // 1 - Rho_l / rho_sol (of drop_mass), rounded once when the solute is set
 };
+
+// the solute with its derived constant (host and device evaluate the same IEEE expression)
+inline Solute make_solute(double rho_sol, double mr_sol, double ionic) {
+  return Solute{rho_sol, mr_sol, ionic, 1.0 - dimless::Rho_l / rho_sol};
+}
 
 // ---- device control block ------------------------------------------------------------------
 enum StatIdx {
@@ -137,8 +143,7 @@ __device__ __forceinline__ double at_least_dry(double r, double msol, const Solu
 }
 __device__ __forceinline__ double drop_mass(double r, double msol, const Solute& s) {
   constexpr double massconst(4.0 / 3.0 * kPi * dimless::Rho_l);
-  const double density_factor = 1.0 - dimless::Rho_l / s.rho_sol;
-  double m = msol * density_factor;
+  double m = msol * s.dens_factor;
   m += massconst * rcubed(r);
   return m;
 }
@@ -257,12 +262,20 @@ __device__ __forceinline__ void ffactor_over_vent(double ffactor, double radius,
   ffv = num / den;
   inv_ffv = den / num;
 }
+// the Koehler a factor depends on the gridbox's temperature only (kept per gridbox with the
+// saturation ratio), b on the drop's solute mass
+__device__ __forceinline__ double kohler_a(double temp) {
+  constexpr double akoh_constant = 3.3e-7 / (dimless::TEMP0 * dimless::R0);
+  return akoh_constant / temp;
+}
+__device__ __forceinline__ double kohler_b(double msol, const Solute& s) {
+  constexpr double bkoh_constant = 4.3e-6 * dimless::RHO0 / dimless::MR0;
+  return bkoh_constant * msol * s.ionic / s.mr_sol;
+}
 __device__ __forceinline__ void kohler_factors(double msol, double temp, const Solute& s, double& akoh,
                                                double& bkoh) {
-  constexpr double akoh_constant = 3.3e-7 / (dimless::TEMP0 * dimless::R0);
-  akoh = akoh_constant / temp;
-  constexpr double bkoh_constant = 4.3e-6 * dimless::RHO0 / dimless::MR0;
-  bkoh = bkoh_constant * msol * s.ionic / s.mr_sol;
+  akoh = kohler_a(temp);
+  bkoh = kohler_b(msol, s);
 }
 
 // ---- implicit Euler: impliciteuler.{hpp,cpp} -------------------------------------------------

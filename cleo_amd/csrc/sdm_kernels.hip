@@ -123,7 +123,8 @@ __device__ __forceinline__ void cond_gbx(const CondArgs& a, uint32_t g, int lane
     qc = rho_cond / rho_dry;
   }
   if (temp <= 0 && lane == 0) set_error(a.ctl, CLEO_ERR_TEMP_NONPOSITIVE, g, base);
-  const double s_ratio = a.gs[2 * g], ffactor = a.gs[2 * g + 1];  // from k_gbx_thermo
+  const double s_ratio = a.gs[kGsStride * g], ffactor = a.gs[kGsStride * g + 1];  // from k_gbx_thermo
+  const double akoh_g = a.gs[kGsStride * g + 2];
 
   double tot = 0.0;
   uint32_t iters = 0, subs = 0, restarts = 0;
@@ -135,8 +136,7 @@ __device__ __forceinline__ void cond_gbx(const CondArgs& a, uint32_t g, int lane
       const double msol = a.s.msol[k];
       const uint64_t xi = a.s.xi[k];
       const double old_m = condensate_mass(r, msol, a.sol, &bad_cm);
-      double akoh, bkoh;
-      kohler_factors(msol, temp, a.sol, akoh, bkoh);
+      const double akoh = akoh_g, bkoh = kohler_b(msol, a.sol);
       const double warm = a.warm ? a.warm[k] : __longlong_as_double(0x7FF8000000000000ll);
       SolveOut o{0, false, false, false, false};
       const double newr = solve_condensation(a.cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
@@ -1541,8 +1541,9 @@ __global__ void k_gbx_thermo(GState st, uint32_t ngbx, double* gs) {
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ngbx; g += gridDim.x * blockDim.x) {
     const double temp = st.temp[g], press = st.press[g];
     const double psat = saturation_pressure(temp);
-    gs[2 * g] = supersaturation_ratio(press, st.qv[g], psat);
-    gs[2 * g + 1] = diffusion_factor(press, temp, psat);
+    gs[kGsStride * g] = supersaturation_ratio(press, st.qv[g], psat);
+    gs[kGsStride * g + 1] = diffusion_factor(press, temp, psat);
+    gs[kGsStride * g + 2] = kohler_a(temp);
   }
 }
 void launch_condensation(cleo_ctx* c, const CondArgs& args) {
@@ -1903,8 +1904,9 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_prep(CondFlatArg
       }
       if (temp <= 0) set_error(a.ctl, CLEO_ERR_TEMP_NONPOSITIVE, g, a.off[g]);
       const double psat = saturation_pressure(temp);
-      a.gs[2 * g] = supersaturation_ratio(press, qv, psat);
-      a.gs[2 * g + 1] = diffusion_factor(press, temp, psat);
+      a.gs[kGsStride * g] = supersaturation_ratio(press, qv, psat);
+      a.gs[kGsStride * g + 1] = diffusion_factor(press, temp, psat);
+      a.gs[kGsStride * g + 2] = kohler_a(temp);
     }
   }
 }
@@ -1924,10 +1926,8 @@ template <class Xi>
 __device__ __forceinline__ bool cond_solve(const CondFlatArgs& a, const NewtonCfg& cfg, uint32_t g, uint64_t k, double r,
                                            double msol, Xi xi_of, double warm, CondStats& st, double& rnew, double& dm,
                                            double& qm) {
-  const double s_ratio = a.gs[2 * g], ffactor = a.gs[2 * g + 1];
-  const double temp = a.st.temp[g];
-  double akoh, bkoh;
-  kohler_factors(msol, temp, a.sol, akoh, bkoh);
+  const double s_ratio = a.gs[kGsStride * g], ffactor = a.gs[kGsStride * g + 1];
+  const double akoh = a.gs[kGsStride * g + 2], bkoh = kohler_b(msol, a.sol);
   SolveOut o{0, false, false, false, false};
   const double newr = solve_condensation(cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
   if (o.deferred) return false;
@@ -1993,7 +1993,7 @@ __global__ void __launch_bounds__(256) k_cond_guess(CondFlatArgs a) {
       valid = g < a.ngbx && (a.unsorted || k < a.off[g] + a.nact[g]);
       if (valid) {
         r = a.s.r[k];
-        drop_features(r, a.s.msol[k], a.gs[2 * g], a.st.temp[g], a.st.press[g], a.cfg.delt, f);
+        drop_features(r, a.s.msol[k], a.gs[kGsStride * g], a.st.temp[g], a.st.press[g], a.cfg.delt, f);
       }
     }
     float xn[kIn];
@@ -2229,13 +2229,12 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_cond_queue(CondFlatAr
       const uint64_t m = __ballot(act);
       if (act) {
         const uint32_t slot = (tail + static_cast<uint32_t>(__popcll(m & lanemask_lt()))) % kRing;
-        const double s_ratio = a.gs[2 * g], ffactor = a.gs[2 * g + 1], temp = a.st.temp[g];
+        const double s_ratio = a.gs[kGsStride * g], ffactor = a.gs[kGsStride * g + 1];
         const double r = a.s.r[kk], ms = a.s.msol[kk];
         bool bad = false;
         const double om = condensate_mass(r, ms, a.sol, &bad);
         if (bad) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, g, kk);
-        double akoh, bkoh;
-        kohler_factors(ms, temp, a.sol, akoh, bkoh);
+        const double akoh = a.gs[kGsStride * g + 2], bkoh = kohler_b(ms, a.sol);
         ODEConsts cc{s_ratio, akoh, bkoh, 0.0, 0.0};
         ffactor_over_vent(ffactor, r, cc.ffactor_fv, cc.inv_ffv);
         double zg;

@@ -17,6 +17,9 @@ constexpr uint32_t kBigGbx = 0xFFFFFFFFu;  // tile LDS marker of a gridbox on th
 constexpr uint32_t kRing = 128;
 constexpr uint32_t kCondRingBytes = kRing * (11u * 8u + 8u + 2u * 4u);
 constexpr uint32_t kCondQueueWavesPerCu = 12;
+// per-gridbox condensation inputs in gs: saturation ratio, diffusion factor, Koehler a (one
+// 32-byte line each)
+constexpr uint32_t kGsStride = 4;
 // LDS window (values) of the per-gridbox drop-order sums
 constexpr uint32_t kSumWin = 1024;
 
@@ -36,7 +39,7 @@ struct CondArgs {
   const double* warm;
   uint32_t warm_limit;
   StatSlots stat;  // ST_COND_DROPS .. ST_WARM_RESTARTS
-  double* gs;      // per gridbox: supersaturation ratio, diffusion factor (written by k_gbx_thermo)
+  double* gs;      // per gridbox (kGsStride): supersaturation ratio, diffusion factor, Koehler a (k_gbx_thermo)
 };
 
 struct QcondArgs {
@@ -120,7 +123,7 @@ struct CondFlatArgs {
   uint32_t* niters;
   const double* warm;
   uint32_t warm_limit;
-  double* gs;  // 2 per gridbox: saturation ratio, diffusion factor
+  double* gs;  // kGsStride per gridbox: saturation ratio, diffusion factor, Koehler a
   double* dm;  // per drop: condensed mass * xi
   // non-null (with fuse_qcond): per drop the condensed mass * xi before the solve; k_cond_state
   // recomputes the gridbox qcond from it instead of a separate pass in k_cond_prep
