@@ -116,7 +116,7 @@ struct cleo_ctx {
   } sdm;
   bool sdm_set = false;
   cleo_tiles tiles{1, 1, 1, 0, 0, 0, 0, 0, 0, 0};
-  double* gs = nullptr;  // kGsStride*ngbx: per gridbox saturation ratio, diffusion factor, Koehler a
+  double* gs = nullptr;  // kGsStride*ngbx: per gridbox saturation ratio, diffusion factor, Koehler a, 1/a
   double* dm = nullptr;  // maxn: per-drop condensed mass (flat condensation)
   // per-wave statistics: up to kStatSlotsMax counters per wave, reduced into DevCtl::stats
   uint32_t* sp = nullptr;

@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(256) k_mlp_samples(SampleArgs a) {
     double akoh, bkoh;
     kohler_factors(msol, temp, a.sol, akoh, bkoh);
     SolveOut o{0, false, false, false, false};
-    const double newr = solve_condensation(a.cfg, s_ratio, akoh, bkoh, ffactor, r,
+    const double newr = solve_condensation(a.cfg, s_ratio, akoh, bkoh, rcp_nr(akoh), ffactor, r,
                                            __longlong_as_double(0x7FF8000000000000ll), 0u, o);
     if (!o.substep && !o.maxed && newr > 0.0) {
       float f[kIn];

@@ -245,7 +245,11 @@ __device__ __forceinline__ double diffusion_factor(double press, double temp, do
 // ffactor / vent = ffactor (u + v) / (u + v + uv).  x^A and x^B come from one logarithm:
 // exp(A log x) is within ~|A log x| ulp of pow(x, A) (a few 1e-15 relative here), and both
 // results are within a few ulp of the reference's quotients.
-__device__ __forceinline__ void ffactor_over_vent(double ffactor, double radius, double& ffv, double& inv_ffv) {
+// The quotient is returned as num / den (ffv_num, ffv_den), with inv_ffv = den / num rounded once:
+// the Newton evaluations use inv_ffv, and ffactor / vent itself (num / den, rounded once as
+// before) is formed only where the critical timestep needs it.
+__device__ __forceinline__ void ffactor_over_vent(double ffactor, double radius, double& ffv_num, double& ffv_den,
+                                                  double& inv_ffv) {
   constexpr double vent_factor_max = 20;
   constexpr double C1 = 6.954e+7, A = 1.963, C2 = 1.069e+3, B = 0.702;
   const double lx = log(radius * dimless::R0);
@@ -259,7 +263,8 @@ __device__ __forceinline__ void ffactor_over_vent(double ffactor, double radius,
     num = ffactor;
     den = 1.0;
   }
-  ffv = num / den;
+  ffv_num = num;
+  ffv_den = den;
   inv_ffv = den / num;
 }
 // the Koehler a factor depends on the gridbox's temperature only (kept per gridbox with the
@@ -293,8 +298,10 @@ __device__ __forceinline__ double cube_rn(double x) {
 }
 
 struct ODEConsts {
-  double s_ratio, akoh, bkoh, ffactor_fv;
-  double inv_ffv;  // 1 / ffactor_fv
+  double s_ratio, akoh, bkoh;
+  double ffv_num, ffv_den;  // ffactor / ventilation factor = ffv_num / ffv_den
+  double inv_ffv;           // ffv_den / ffv_num
+  double inv_akoh;          // rcp_nr(akoh), kept per gridbox
 };
 struct NewtonCfg {
   double delt, minsubdelt, rtol, atol;
@@ -392,7 +399,8 @@ __device__ __forceinline__ bool first_unique_criteria(const ODEConsts& c, double
 __device__ __forceinline__ double critical_timestep(const ODEConsts& c) {
   const double x = 5.0 * c.bkoh / c.akoh;
   const double cuberoot = x * sqrt(x);  // pow(x, 1.5) to within 1 ulp
-  return 2.5 * c.ffactor_fv / c.akoh * cuberoot;
+  const double ffactor_fv = c.ffv_num / c.ffv_den;
+  return 2.5 * ffactor_fv / c.akoh * cuberoot;
 }
 
 // Three-valued comparisons: +1 true, -1 false, 0 too close to call from approximate operands.
@@ -418,7 +426,7 @@ __device__ __forceinline__ int tri_sqrt_sign(double d, double y) {
 // roots are exact to a few ulp, so the outcome is the same); a drop with any comparison inside
 // that band evaluates the reference's expressions instead.
 __device__ __forceinline__ void ode_setup(const ODEConsts& c, double rprev, double delt, double& ziter, bool& ucrit) {
-  const double ia = rcp_nr(c.akoh), ib = rcp_nr(c.bkoh);
+  const double ia = c.inv_akoh, ib = rcp_nr(c.bkoh);
   const double y = (c.akoh * c.akoh * c.akoh) * ib * (4.0 / 27.0);  // (s_act - 1)^2
   const int sact = tri_sqrt_sign(c.s_ratio - 1.0, y);                // s_ratio > s_act
   constexpr double bigr(1e-3 / dimless::R0);
@@ -427,10 +435,11 @@ __device__ __forceinline__ void ode_setup(const ODEConsts& c, double rprev, doub
   const double rcrit = 3.0 * c.bkoh * ia;  // rcritsqrd
   const int unact = min(tri_less(rprev * rprev, rcrit), tri_less(ziter, rcrit));
   const int ucrit1 = min(unact, -sact);
-  // delt <= critdelt = 2.5 ffv/akoh x^1.5 with x = 5 bkoh/akoh, compared squared
-  const double x = 5.0 * c.bkoh * ia, k = 2.5 * c.ffactor_fv * ia;
+  // delt <= critdelt = 2.5 (num/den)/akoh x^1.5 with x = 5 bkoh/akoh, compared squared and
+  // multiplied by den^2 (no division)
+  const double x = 5.0 * c.bkoh * ia, k = 2.5 * c.ffv_num * ia;
   const double crit2 = (k * k) * (x * x * x);
-  const double d2 = delt * delt;
+  const double d2 = (delt * delt) * (c.ffv_den * c.ffv_den);
   const int ucrit2 = delt > 0.0 ? (d2 < crit2 * (1.0 - 1e-8) ? 1 : (d2 > crit2 * (1.0 + 1e-8) ? -1 : 0)) : 0;
   const int uc = max(ucrit1, ucrit2);
   ucrit = uc > 0;
@@ -452,11 +461,12 @@ struct SolveOut {
 // below atol the plain guess is already the root to the tolerance (a drop at equilibrium), and the
 // same iteration starts from the plain guess instead: its test passes after one update, where the
 // reference path always makes two and a learned guess (a few 1e-3 off) needs three or four.
+// inv_akoh = rcp_nr(akoh).
 __device__ __forceinline__ double solve_condensation(const NewtonCfg& cfg, double s_ratio, double akoh,
-                                                     double bkoh, double ffactor, double rprev, double warm,
-                                                     uint32_t warm_limit, SolveOut& o) {
-  ODEConsts c{s_ratio, akoh, bkoh, 0.0, 0.0};
-  ffactor_over_vent(ffactor, rprev, c.ffactor_fv, c.inv_ffv);
+                                                     double bkoh, double inv_akoh, double ffactor, double rprev,
+                                                     double warm, uint32_t warm_limit, SolveOut& o) {
+  ODEConsts c{s_ratio, akoh, bkoh, 0.0, 0.0, 0.0, inv_akoh};
+  ffactor_over_vent(ffactor, rprev, c.ffv_num, c.ffv_den, c.inv_ffv);
   double ziter;
   bool ucrit;
   ode_setup(c, rprev, cfg.delt, ziter, ucrit);

@@ -124,7 +124,7 @@ __device__ __forceinline__ void cond_gbx(const CondArgs& a, uint32_t g, int lane
   }
   if (temp <= 0 && lane == 0) set_error(a.ctl, CLEO_ERR_TEMP_NONPOSITIVE, g, base);
   const double s_ratio = a.gs[kGsStride * g], ffactor = a.gs[kGsStride * g + 1];  // from k_gbx_thermo
-  const double akoh_g = a.gs[kGsStride * g + 2];
+  const double akoh_g = a.gs[kGsStride * g + 2], inv_akoh_g = a.gs[kGsStride * g + 3];
 
   double tot = 0.0;
   uint32_t iters = 0, subs = 0, restarts = 0;
@@ -139,7 +139,7 @@ __device__ __forceinline__ void cond_gbx(const CondArgs& a, uint32_t g, int lane
       const double akoh = akoh_g, bkoh = kohler_b(msol, a.sol);
       const double warm = a.warm ? a.warm[k] : __longlong_as_double(0x7FF8000000000000ll);
       SolveOut o{0, false, false, false, false};
-      const double newr = solve_condensation(a.cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
+      const double newr = solve_condensation(a.cfg, s_ratio, akoh, bkoh, inv_akoh_g, ffactor, r, warm, a.warm_limit, o);
       const double rnew = at_least_dry(newr, msol, a.sol);
       a.s.r[k] = rnew;
       dm = (condensate_mass(rnew, msol, a.sol, &bad_cm) - old_m) * static_cast<double>(xi);
@@ -1543,7 +1543,9 @@ __global__ void k_gbx_thermo(GState st, uint32_t ngbx, double* gs) {
     const double psat = saturation_pressure(temp);
     gs[kGsStride * g] = supersaturation_ratio(press, st.qv[g], psat);
     gs[kGsStride * g + 1] = diffusion_factor(press, temp, psat);
-    gs[kGsStride * g + 2] = kohler_a(temp);
+    const double akoh = kohler_a(temp);
+    gs[kGsStride * g + 2] = akoh;
+    gs[kGsStride * g + 3] = rcp_nr(akoh);
   }
 }
 void launch_condensation(cleo_ctx* c, const CondArgs& args) {
@@ -1906,7 +1908,9 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(4) k_cond_prep(CondFlatArg
       const double psat = saturation_pressure(temp);
       a.gs[kGsStride * g] = supersaturation_ratio(press, qv, psat);
       a.gs[kGsStride * g + 1] = diffusion_factor(press, temp, psat);
-      a.gs[kGsStride * g + 2] = kohler_a(temp);
+      const double akoh = kohler_a(temp);
+      a.gs[kGsStride * g + 2] = akoh;
+      a.gs[kGsStride * g + 3] = rcp_nr(akoh);
     }
   }
 }
@@ -1929,7 +1933,8 @@ __device__ __forceinline__ bool cond_solve(const CondFlatArgs& a, const NewtonCf
   const double s_ratio = a.gs[kGsStride * g], ffactor = a.gs[kGsStride * g + 1];
   const double akoh = a.gs[kGsStride * g + 2], bkoh = kohler_b(msol, a.sol);
   SolveOut o{0, false, false, false, false};
-  const double newr = solve_condensation(cfg, s_ratio, akoh, bkoh, ffactor, r, warm, a.warm_limit, o);
+  const double newr =
+      solve_condensation(cfg, s_ratio, akoh, bkoh, a.gs[kGsStride * g + 3], ffactor, r, warm, a.warm_limit, o);
   if (o.deferred) return false;
   bool bad = false;
   const double old_m = condensate_mass(r, msol, a.sol, &bad);
@@ -2210,7 +2215,7 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_cond_queue(CondFlatAr
   // lane state
   bool busy = false, restarted = false, maxed = false;
   uint32_t k = 0, phase = PH_EVAL, mode = MODE_PLAIN, niter = 0, limit = 0, iters = 0;
-  ODEConsts c{0, 0, 0, 0, 0};
+  ODEConsts c{0, 0, 0, 0, 0, 0, 0};
   double rprev = 0, r0 = 0, subdelt = 0, remdelt = 0, mindelt = 0, zplain = 0;
   double z = 0, gz = 0, dgz = 0, z0 = 0, g0 = 0, dg0 = 0, dryr = 0, oldm = 0, msol = 0;
   uint64_t xi = 0;
@@ -2235,8 +2240,8 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_cond_queue(CondFlatAr
         const double om = condensate_mass(r, ms, a.sol, &bad);
         if (bad) set_error(a.ctl, CLEO_ERR_NEG_CONDENSATE, g, kk);
         const double akoh = a.gs[kGsStride * g + 2], bkoh = kohler_b(ms, a.sol);
-        ODEConsts cc{s_ratio, akoh, bkoh, 0.0, 0.0};
-        ffactor_over_vent(ffactor, r, cc.ffactor_fv, cc.inv_ffv);
+        ODEConsts cc{s_ratio, akoh, bkoh, 0.0, 0.0, 0.0, a.gs[kGsStride * g + 3]};
+        ffactor_over_vent(ffactor, r, cc.ffv_num, cc.ffv_den, cc.inv_ffv);
         double zg;
         bool uniq;
         ode_setup(cc, r, cfg.delt, zg, uniq);
@@ -2273,7 +2278,7 @@ __global__ void __launch_bounds__(256) CLEO_OCCUPANCY(3) k_cond_queue(CondFlatAr
       const uint32_t rank = static_cast<uint32_t>(__popcll(midle & lanemask_lt()));
       if (rank < avail) {
         const uint32_t slot = (head + rank) % kRing;
-        c = ODEConsts{R.s_ratio[slot], R.akoh[slot], R.bkoh[slot], 0.0, R.inv_ffv[slot]};
+        c = ODEConsts{R.s_ratio[slot], R.akoh[slot], R.bkoh[slot], 0.0, 0.0, R.inv_ffv[slot], 0.0};
         r0 = R.r0[slot];
         zplain = R.zplain[slot];
         mindelt = R.mindelt[slot];

@@ -17,8 +17,8 @@ constexpr uint32_t kBigGbx = 0xFFFFFFFFu;  // tile LDS marker of a gridbox on th
 constexpr uint32_t kRing = 128;
 constexpr uint32_t kCondRingBytes = kRing * (11u * 8u + 8u + 2u * 4u);
 constexpr uint32_t kCondQueueWavesPerCu = 12;
-// per-gridbox condensation inputs in gs: saturation ratio, diffusion factor, Koehler a (one
-// 32-byte line each)
+// per-gridbox condensation inputs in gs: saturation ratio, diffusion factor, Koehler a and its
+// refined reciprocal (one 32-byte line each)
 constexpr uint32_t kGsStride = 4;
 // LDS window (values) of the per-gridbox drop-order sums
 constexpr uint32_t kSumWin = 1024;
